@@ -1,0 +1,105 @@
+/* C ABI of the MI355X (gfx950) reacting-flow right-hand-side library.
+ *
+ * Every array argument is a device pointer (HIP device memory) laid out as the
+ * reference's ManyVector (euler3D.hpp:62-65, multirate_chem_hydro_main.cpp:320-352):
+ *   w[0..4]  rho, mx, my, mz, et   each nx*ny*nz doubles, cell i + nx*(j + ny*k)
+ *   w[5]     chemistry            nx*ny*nz*nchem doubles, species fastest
+ * A context holds the grid, boundary conditions, unit factors, the HIP stream all work
+ * is queued on, and the device scratch the reference keeps in EulerData/cvklu_data.
+ *
+ * Return codes follow the SUNDIALS callback convention the reference uses:
+ *   0 success, > 0 recoverable failure, < 0 unrecoverable failure.
+ * Calls on one context are not reentrant (as the reference's, which share flux scratch
+ * and the per-cell rate cache). Every call is synchronous with respect to the host
+ * unless its name ends in _async.
+ */
+#ifndef MVD_RHS_H
+#define MVD_RHS_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mvd_context mvd_context;
+
+/* error codes (all < 0 are unrecoverable) */
+#define MVD_SUCCESS         0
+#define MVD_ILLEGAL_STATE  -1   /* fEuler's legal_state failure, utilities.cpp:83-84,132-133 */
+#define MVD_BAD_ARGUMENT   -2
+#define MVD_DEVICE_ERROR   -3
+#define MVD_UNSUPPORTED    -4
+#define MVD_NOT_READY      -5   /* chemistry used before mvd_chem_setup */
+
+/* boundary condition codes, euler3D.hpp:90-93 */
+#define MVD_BC_PERIODIC   0
+#define MVD_BC_NEUMANN    1
+#define MVD_BC_DIRICHLET  2
+#define MVD_BC_REFLECTING 3
+
+/* Problem description; the fields of EulerData that the hot path reads
+ * (euler3D.hpp:189-233). Single rank: the local grid is the global grid. */
+typedef struct {
+  long nx, ny, nz;         /* grid (each >= 3) */
+  int nchem;               /* advected species per cell (0, 1, 2, 4, 6 or 10) */
+  int bc[6];               /* xl, xr, yl, yr, zl, zr */
+  double gamma;            /* ratio of specific heats */
+  double xl, xr, yl, yr, zl, zr;
+  double cfl;              /* stability() safety fraction */
+  double mass_units, length_units, time_units;   /* code -> CGS factors (UpdateUnits, euler3D.hpp:385-393) */
+} mvd_params;
+
+/* Context lifetime. Replaces EulerData::SetupDecomp's allocations (euler3D.hpp:396-574)
+ * for one rank. stream: a hipStream_t, or NULL for the null stream. */
+int mvd_create(const mvd_params *p, void *stream, mvd_context **out);
+void mvd_destroy(mvd_context *ctx);
+int mvd_set_stream(mvd_context *ctx, void *stream);
+
+/* fEuler (utilities.cpp:17-253; ARKRhsFn signature euler3D.hpp:1480) with zero external
+ * forcing: wdot = -div F(w). Returns MVD_ILLEGAL_STATE when any cell has rho, et or p <= 0
+ * (mvd_last_state_flags gives the 1/2/4 bits). */
+int mvd_fe(mvd_context *ctx, double t, const double *const *w, double *const *wdot);
+/* The same result from the plain one-thread-per-cell kernels with the reference's operation
+ * order and no FMA contraction (the comparator; bitwise reproducible on the CPU). */
+int mvd_fe_plain(mvd_context *ctx, double t, const double *const *w, double *const *wdot);
+/* Queue fe() without waiting or checking the legal-state flag (for stage loops and graph
+ * capture); the flag is read by the next mvd_last_state_flags call. */
+int mvd_fe_async(mvd_context *ctx, double t, const double *const *w, double *const *wdot);
+int mvd_last_state_flags(mvd_context *ctx);
+
+/* stability (utilities.cpp:483-528; ARKStep stability callback, euler3D_main.cpp:272-275):
+ * alpha = max over cells of |mx/rho| + c, dt_stab = cfl * min(dx,dy,dz) / alpha. */
+int mvd_max_wave_speed(mvd_context *ctx, const double *const *w, double *alpha);
+int mvd_stability(mvd_context *ctx, const double *const *w, double t, double *dt_stab);
+
+/* fslow (multirate_chem_hydro_main.cpp:996-1083) / fexpl (imex_chem_hydro_main.cpp:910-1000):
+ * scales chem by the Dengo scale, overwrites w[4] (et) from the gas energy and kinetic
+ * energy, calls fe(), moves etdot into the gas-energy slot of wdot[5], zeroes wdot[4] and
+ * unscales chem. Requires nchem > 0 and mvd_chem_setup. */
+int mvd_fslow(mvd_context *ctx, double t, double *const *w, double *const *wdot);
+
+/* ---- primordial chemistry (raja_primordial_network.cpp) ---- */
+
+/* cvklu_setup_data (raja_primordial_network.cpp:45-254): tables_host points to the 50
+ * rate/cooling/gamma tables (50 x 1024 doubles, host memory, committed blob order);
+ * ncell cells; current_z redshift (the demo uses -1). Ts starts at 1000 K, scale at 1. */
+int mvd_chem_setup(mvd_context *ctx, const double *tables_host, long ncell, double current_z);
+/* prepare_Dengo_structures (primordial_blast.cpp:368-400): scale <- chem, inv_scale <- 1/chem,
+ * chem <- 1, then setting_up_extra_variables (raja_primordial_network.cpp:1969-2018). */
+int mvd_chem_prepare(mvd_context *ctx, double *chem);
+/* apply_/unapply_Dengo_scaling (primordial_blast.cpp:404-458): chem *= scale / chem /= scale */
+int mvd_chem_apply_scaling(mvd_context *ctx, double *chem);
+int mvd_chem_unapply_scaling(mvd_context *ctx, double *chem);
+/* calculate_rhs_cvklu (raja_primordial_network.cpp:1015-1389): ydot(y); updates the per-cell
+ * temperature Ts (warm start of the next call) and dTs/dge. */
+int mvd_chem_rhs(mvd_context *ctx, double t, const double *y, double *ydot);
+/* calculate_jacobian_cvklu (raja_primordial_network.cpp:1393-1965): dense column-major
+ * 10x10 blocks J[100*cell + 10*col + row] (DENSEIDX, line 40), evaluated from the
+ * temperature of the last mvd_chem_rhs call. */
+int mvd_chem_jac(mvd_context *ctx, double t, const double *y, double *J);
+/* device pointers of per-cell chemistry state: 0 scale, 1 inv_scale, 2 Ts, 3 dTs_ge,
+ * 4 mdensity, 5 inv_mdensity, 6 cie_optical_depth, 7 h2_optical_depth */
+double *mvd_chem_state(mvd_context *ctx, int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

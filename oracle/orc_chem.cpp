@@ -1,0 +1,217 @@
+// Oracle drivers for the primordial chemistry and the Dengo scaling helpers --
+// TEST INFRASTRUCTURE ONLY (see orc_common.hpp).
+#include <cstring>
+#include <vector>
+#include "orc_chem.hpp"
+#include "oracle.h"
+
+using namespace orc;
+
+namespace {
+struct Chem {
+  long n;
+  std::vector<double> tab;
+  TableGrid g;
+  double z;
+  std::vector<double> scale, inv_scale, Ts, dTs_ge, mdensity, inv_mdensity, cie_oda, h2_oda,
+      rs, drs;
+};
+}  // namespace
+
+// cvklu_setup_data (raja_primordial_network.cpp:45-254) with the tables given as the
+// committed blob: Ts = 1000 K, bounds [1,1e5], nbins = 1023,
+// dbin = (log(1e5)-log(1))/nbins, idbin = 1.0L/dbin (long-double quotient as in line 195),
+// scale = inv_scale = 1 (primordial_blast.cpp:337-341).
+extern "C" void *orc_chem_create(long n, const double *tables, double current_z) {
+  Chem *c = new Chem;
+  c->n = n;
+  c->tab.assign(tables, tables + NTAB * TLEN);
+  c->g.tab = c->tab.data();
+  c->g.tmin = 1.0;
+  c->g.tmax = 100000.0;
+  c->g.nbins = TLEN - 1;
+  c->g.dbin = (std::log(c->g.tmax) - std::log(c->g.tmin)) / c->g.nbins;
+  c->g.idbin = (double)(1.0L / (long double)c->g.dbin);
+  c->g.lb = std::log(c->g.tmin);
+  c->z = current_z;
+  c->scale.assign(NSP * n, 1.0);
+  c->inv_scale.assign(NSP * n, 1.0);
+  c->Ts.assign(n, 1000.0);
+  c->dTs_ge.assign(n, 0.0);
+  c->mdensity.assign(n, 0.0);
+  c->inv_mdensity.assign(n, 0.0);
+  c->cie_oda.assign(n, 0.0);
+  c->h2_oda.assign(n, 0.0);
+  c->rs.assign((size_t)NRATE * n, 0.0);
+  c->drs.assign((size_t)NRATE * n, 0.0);
+  return c;
+}
+
+extern "C" void orc_chem_destroy(void *h) { delete (Chem *)h; }
+
+// 0 scale, 1 inv_scale, 2 Ts, 3 dTs_ge, 4 mdensity, 5 inv_mdensity, 6 cie_oda, 7 h2_oda,
+// 8 rate values [NRATE][n], 9 rate derivatives [NRATE][n]
+extern "C" double *orc_chem_array(void *h, int which) {
+  Chem *c = (Chem *)h;
+  switch (which) {
+    case 0: return c->scale.data();
+    case 1: return c->inv_scale.data();
+    case 2: return c->Ts.data();
+    case 3: return c->dTs_ge.data();
+    case 4: return c->mdensity.data();
+    case 5: return c->inv_mdensity.data();
+    case 6: return c->cie_oda.data();
+    case 7: return c->h2_oda.data();
+    case 8: return c->rs.data();
+    case 9: return c->drs.data();
+  }
+  return nullptr;
+}
+
+extern "C" double orc_chem_const(void *h, int which) {
+  Chem *c = (Chem *)h;
+  return which == 0 ? c->g.dbin : which == 1 ? c->g.idbin : c->g.lb;
+}
+
+// setting_up_extra_variables (raja_primordial_network.cpp:1969-2018), from scale.
+extern "C" void orc_chem_extra(void *h) {
+  Chem *c = (Chem *)h;
+  for (long i = 0; i < c->n; i++) {
+    const double *in = &c->scale[NSP * i];
+    double md = 0.0;
+    md += in[0] * 2.0;
+    md += in[1] * 2.0;
+    md += in[2] * 1.00794;
+    md += in[3] * 1.00794;
+    md += in[4] * 1.00794;
+    md += in[5] * 4.002602;
+    md += in[6] * 4.002602;
+    md += in[7] * 4.002602;
+    md *= 1.67e-24;
+    double tau = std::pow(md / 3.3e-8, 2.8);
+    tau = std::fmax(tau, 1.0e-5);
+    c->mdensity[i] = md;
+    c->inv_mdensity[i] = 1.0 / md;
+    c->cie_oda[i] = std::fmin(1.0, (1.0 - std::exp(-tau)) / tau);
+    c->h2_oda[i] = std::fmin(1.0, std::pow(md / (1.34e-14), -0.45));
+  }
+}
+
+// prepare_Dengo_structures (primordial_blast.cpp:368-400): scale <- chem, inv_scale <- 1/chem,
+// chem <- 1, then the auxiliary per-cell values.
+extern "C" void orc_chem_prepare(void *h, double *chem) {
+  Chem *c = (Chem *)h;
+  for (long i = 0; i < NSP * c->n; i++) {
+    c->scale[i] = chem[i];
+    c->inv_scale[i] = 1.0 / c->scale[i];
+    chem[i] = 1.0;
+  }
+  orc_chem_extra(h);
+}
+
+// apply/unapply_Dengo_scaling (primordial_blast.cpp:404-458)
+extern "C" void orc_chem_apply_scaling(void *h, double *chem) {
+  Chem *c = (Chem *)h;
+  for (long i = 0; i < NSP * c->n; i++) chem[i] *= c->scale[i];
+}
+extern "C" void orc_chem_unapply_scaling(void *h, double *chem) {
+  Chem *c = (Chem *)h;
+  for (long i = 0; i < NSP * c->n; i++) chem[i] /= c->scale[i];
+}
+
+static CellAux aux_of(const Chem *c, long i) {
+  return CellAux{c->mdensity[i], c->inv_mdensity[i], c->cie_oda[i], c->h2_oda[i]};
+}
+
+// calculate_rhs_cvklu (raja_primordial_network.cpp:1015-1389): updates Ts, dTs_ge and
+// the rate cache as side effects, then writes the scaled ydot.
+extern "C" int orc_chem_rhs(void *h, const double *y, double *ydot) {
+  Chem *c = (Chem *)h;
+  for (long i = 0; i < c->n; i++) {
+    const long j = NSP * i;
+    double ya[NSP];
+    for (int s = 0; s < NSP; s++) ya[s] = y[j + s] * c->scale[j + s];
+    temperature(c->g, ya, c->Ts[i], c->dTs_ge[i]);
+    double rs[NRATE], drs[NRATE];
+    interp_rates(c->g, c->Ts[i], rs, drs);
+    for (int r = 0; r < NRATE; r++) {
+      c->rs[(size_t)r * c->n + i] = rs[r];
+      c->drs[(size_t)r * c->n + i] = drs[r];
+    }
+    double f[NSP];
+    rhs_physical(ya, rs, c->Ts[i], c->z, aux_of(c, i), f);
+    for (int s = 0; s < NSP; s++) {
+      ydot[j + s] = f[s];
+      ydot[j + s] *= c->inv_scale[j + s];
+    }
+    ydot[j + 9] *= c->inv_mdensity[i];
+  }
+  return 0;
+}
+
+// calculate_jacobian_cvklu (raja_primordial_network.cpp:1393-1965), dense column-major
+// 10x10 blocks J[100*i + 10*col + row] (DENSEIDX, line 40), from the cached rates.
+extern "C" int orc_chem_jac(void *h, const double *y, double *Jout) {
+  Chem *c = (Chem *)h;
+  for (long i = 0; i < c->n; i++) {
+    const long j = NSP * i;
+    double ya[NSP], rs[NRATE], drs[NRATE];
+    for (int s = 0; s < NSP; s++) ya[s] = y[j + s] * c->scale[j + s];
+    for (int r = 0; r < NRATE; r++) {
+      rs[r] = c->rs[(size_t)r * c->n + i];
+      drs[r] = c->drs[(size_t)r * c->n + i];
+    }
+    double J[NSP][NSP];
+    bool nzm[NSP][NSP];
+    jac_physical(ya, rs, drs, c->Ts[i], c->z, c->dTs_ge[i], aux_of(c, i), J, nzm);
+    double *blk = Jout + (size_t)NSP * NSP * i;
+    for (int col = 0; col < NSP; col++)
+      for (int row = 0; row < NSP; row++) {
+        double v = 0.0;
+        if (nzm[row][col]) {
+          v = J[row][col];
+          v *= (c->inv_scale[j + row] * c->scale[j + col]);
+        }
+        blk[NSP * col + row] = v;
+      }
+  }
+  return 0;
+}
+
+// Rate interpolation and RHS at a fixed temperature (no temperature solve, no cache
+// update): used by tests to difference the RHS with the rates held fixed.
+extern "C" void orc_chem_rhs_fixed_T(void *h, const double *Tcell, const double *y,
+                                     double *ydot) {
+  Chem *c = (Chem *)h;
+  for (long i = 0; i < c->n; i++) {
+    const long j = NSP * i;
+    double ya[NSP], rs[NRATE], drs[NRATE], f[NSP];
+    for (int s = 0; s < NSP; s++) ya[s] = y[j + s] * c->scale[j + s];
+    interp_rates(c->g, Tcell[i], rs, drs);
+    rhs_physical(ya, rs, Tcell[i], c->z, aux_of(c, i), f);
+    for (int s = 0; s < NSP; s++) {
+      ydot[j + s] = f[s];
+      ydot[j + s] *= c->inv_scale[j + s];
+    }
+    ydot[j + 9] *= c->inv_mdensity[i];
+  }
+}
+
+// operation counts of one cell's RHS (temperature + rates + RHS) and Jacobian:
+// out = {add, mul, div, sqrt, cmp, transcendental} for the RHS, then the same for J.
+extern "C" void orc_chem_opcount(void *h, unsigned long long *out) {
+  Chem *c = (Chem *)h;
+  Cnt y[NSP], Ts(c->Ts[0]), dTs(0.0), rs[NRATE], drs[NRATE], f[NSP], J[NSP][NSP];
+  bool nzm[NSP][NSP];
+  for (int s = 0; s < NSP; s++) y[s] = Cnt(c->scale[s]);
+  OpCounter::reset();
+  temperature(c->g, y, Ts, dTs);
+  interp_rates(c->g, Ts, rs, drs);
+  rhs_physical(y, rs, Ts, Cnt(c->z), aux_of(c, 0), f);
+  out[0] = OpCounter::add; out[1] = OpCounter::mul; out[2] = OpCounter::div;
+  out[3] = OpCounter::sqrt_; out[4] = OpCounter::cmp; out[5] = OpCounter::trans;
+  OpCounter::reset();
+  jac_physical(y, rs, drs, Ts, Cnt(c->z), dTs, aux_of(c, 0), J, nzm);
+  out[6] = OpCounter::add; out[7] = OpCounter::mul; out[8] = OpCounter::div;
+  out[9] = OpCounter::sqrt_; out[10] = OpCounter::cmp; out[11] = OpCounter::trans;
+}

@@ -1,0 +1,371 @@
+// Primordial chemistry network (10 species: H2_1, H2_2, H_1, H_2, H_m0, He_1, He_2, He_3,
+// de, ge) for gfx950, FP64: per-cell temperature solve, rate interpolation, RHS, analytic
+// Jacobian. Device functions here are shared by the per-call kernels (chem.hip), the
+// batched Newton/DIRK kernel and the adjoint kernels.
+//
+// State kept per cell (cvklu_data, raja_primordial_network.hpp:104-276): scale, inv_scale,
+// Ts, dTs_ge, mdensity, inv_mdensity and the two optical-depth factors. The reference also
+// caches 92 interpolated rate values per cell for the Jacobian; the rates are a pure
+// function of the cached Ts, so here the Jacobian re-interpolates them from Ts instead of
+// streaming 736 bytes per cell through HBM.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mvd {
+
+constexpr int NSP = 10;       // species per cell
+constexpr int TLEN = 1024;    // table length
+constexpr int NRATE = 46;     // 21 reaction + 25 cooling tables
+constexpr int NTAB = 50;      // + 4 gamma tables
+
+// table order of the committed blob (the reference's read order)
+enum {
+  K01 = 0, K02, K03, K04, K05, K06, K07, K08, K09, K10, K11, K12, K13, K14, K15, K16, K17,
+  K18, K19, K21, K22,
+  C_BREM, C_CEHEI, C_CEHEII, C_CEHI, C_CIECO, C_CIHEI, C_CIHEII, C_CIHEIS, C_CIHI, C_COMPTON,
+  C_GAEL, C_GAH2, C_GAHE, C_GAHI, C_GAHP, C_H2LTE, C_H2MCOOL, C_H2MHEAT, C_NCRD1, C_NCRD2,
+  C_NCRN, C_REHEII1, C_REHEII2, C_REHEIII, C_REHII,
+  G_GH2_1, G_DGH2_1, G_GH2_2, G_DGH2_2
+};
+
+// Table grid constants (cvklu_setup_data, raja_primordial_network.cpp:191-195)
+struct ChemConst {
+  const double *tab;    // device, NTAB x TLEN
+  double dbin, idbin, lb, tmin, tmax, z;
+  int nbins;
+};
+
+struct ChemState {
+  bool ready = false;
+  long n = 0;
+  double *tab = nullptr;
+  ChemConst cc{};
+  double *scale = nullptr, *inv_scale = nullptr, *Ts = nullptr, *dTs_ge = nullptr;
+  double *mdensity = nullptr, *inv_mdensity = nullptr, *cie_oda = nullptr, *h2_oda = nullptr;
+  double *work = nullptr;     // solver scratch (lazily sized)
+  size_t work_bytes = 0;
+  int *counters = nullptr;    // work queue / statistics
+  int *host_counters = nullptr;
+};
+
+int chem_setup(ChemState &s, const double *tables_host, long n, double z, hipStream_t st);
+int chem_prepare(ChemState &s, double *chem, hipStream_t st);
+int chem_rhs(ChemState &s, const double *y, double *ydot, hipStream_t st, bool sync);
+int chem_jac(ChemState &s, const double *y, double *J, hipStream_t st, bool sync);
+double *chem_state_ptr(ChemState &s, int which);
+void chem_free(ChemState &s);
+
+// -------------------------------------------------------------------------------------
+// device functions (reference operation order, no FMA contraction)
+// -------------------------------------------------------------------------------------
+
+// (int)(idbin*(log T - lb)) clamped to [0, nbins-1] (raja_primordial_network.cpp:953-958)
+__device__ inline int chem_bin(const ChemConst &c, double logT) {
+  const double x = c.idbin * (logT - c.lb);
+  int b = (x == x) ? (int)fmin(fmax(x, -1.0), 2.0e9) : 0;
+  if (b <= 0) b = 0;
+  else if (b >= c.nbins) b = c.nbins - 1;
+  return b;
+}
+
+// cvklu_calculate_temperature (raja_primordial_network.cpp:908-1011): 10 Newton steps
+// from the warm start Ts, clamp to [1,1e5], dTs_ge = 1/dge_dT.
+__device__ inline void chem_temperature(const ChemConst &c, const double (&y)[NSP], double &Ts,
+                                        double &dTs_ge) {
+#pragma clang fp contract(off)
+  const double kb = 1.3806504e-16, mh = 1.67e-24;
+  const double gamma = 5.e0 / 3.e0;
+  const double gm1 = 1.0 / (gamma - 1);
+  const double density = 2.0 * y[0] + 2.0 * y[1] + 1.0079400000000001 * y[2] +
+                         1.0079400000000001 * y[3] + 1.0079400000000001 * y[4] +
+                         4.0026020000000004 * y[5] + 4.0026020000000004 * y[6] +
+                         4.0026020000000004 * y[7];
+  const double dm = density * mh;
+  double T = Ts, Tnew = T * 1.1, dge_dT = 0.0;
+  const double *g1t = c.tab + G_GH2_1 * TLEN, *dg1t = c.tab + G_DGH2_1 * TLEN;
+  const double *g2t = c.tab + G_GH2_2 * TLEN, *dg2t = c.tab + G_DGH2_2 * TLEN;
+  for (int it = 0; it < 10; it++) {
+    T = Ts;
+    const double lT = log(Ts);
+    const int b = chem_bin(c, lT);
+    const double t1 = c.lb + (double)b * c.dbin;
+    const double t2 = c.lb + (double)(b + 1) * c.dbin;
+    const double Tdef = (lT - t1) / (t2 - t1);
+    const double gH2_2 = g2t[b] + Tdef * (g2t[b + 1] - g2t[b]);
+    const double dgH2_2 = dg2t[b] + Tdef * (dg2t[b + 1] - dg2t[b]);
+    const double gH2_1 = g1t[b] + Tdef * (g1t[b + 1] - g1t[b]);
+    const double dgH2_1 = dg1t[b] + Tdef * (dg1t[b + 1] - dg1t[b]);
+    const double g1 = 1.0 / (gH2_1 - 1.0);
+    const double g2 = 1.0 / (gH2_2 - 1.0);
+    const double nsum = y[0] * g1 + y[1] * g2 + y[2] * gm1 + y[3] * gm1 + y[4] * gm1 +
+                        y[5] * gm1 + y[6] * gm1 + y[7] * gm1 + gm1 * y[8];
+    dge_dT = T * kb * ((-y[0]) * g1 * g1 * dgH2_1 - y[1] * g2 * g2 * dgH2_2) / dm + kb * nsum / dm;
+    const double dge = T * kb * nsum / dm - y[9];
+    Tnew = T - dge / dge_dT;
+    Ts = Tnew;
+  }
+  Ts = Tnew;
+  if (Ts < c.tmin) Ts = c.tmin;
+  else if (Ts > c.tmax) Ts = c.tmax;
+  dTs_ge = 1.0 / dge_dT;
+}
+
+// linear-in-log(T) interpolation of every rate table and its T derivative
+// (raja_primordial_network.cpp:1046-1245)
+__device__ inline void chem_rates(const ChemConst &c, double T, double (&k)[NRATE],
+                                  double (&dk)[NRATE]) {
+#pragma clang fp contract(off)
+  const double lT = log(T);
+  const int b = chem_bin(c, lT);
+  const double t1 = c.lb + (double)b * c.dbin;
+  const double t2 = c.lb + (double)(b + 1) * c.dbin;
+  const double Tdef = (lT - t1) / (t2 - t1);
+  const double Tfactor = (1.0 / T) / (t2 - t1);
+#pragma unroll
+  for (int r = 0; r < NRATE; r++) {
+    const double a0 = c.tab[r * TLEN + b], a1 = c.tab[r * TLEN + b + 1];
+    k[r] = a0 + Tdef * (a1 - a0);
+    dk[r] = (a1 - a0) * Tfactor;
+  }
+}
+
+// Only the rate values (no derivatives).
+__device__ inline void chem_rates_only(const ChemConst &c, double T, double (&k)[NRATE]) {
+#pragma clang fp contract(off)
+  const double lT = log(T);
+  const int b = chem_bin(c, lT);
+  const double t1 = c.lb + (double)b * c.dbin;
+  const double t2 = c.lb + (double)(b + 1) * c.dbin;
+  const double Tdef = (lT - t1) / (t2 - t1);
+#pragma unroll
+  for (int r = 0; r < NRATE; r++) {
+    const double a0 = c.tab[r * TLEN + b], a1 = c.tab[r * TLEN + b + 1];
+    k[r] = a0 + Tdef * (a1 - a0);
+  }
+}
+
+struct CellAux {
+  double mdensity, inv_mdensity, cie, h2o;
+};
+
+// The 21 two- and three-body reaction rates r_k = k_k * prod(reactants), each multiplied in
+// the order the network's expanded RHS uses, so that summing them in the RHS's term order
+// reproduces the expanded expressions exactly.
+struct Reactions {
+  double r01, r02, r03, r04, r05, r06, r07, r08, r09, r10, r11, r12, r13, r14, r15, r16, r17,
+      r18, r19, r21, r22;
+};
+
+__device__ inline Reactions chem_reactions(const double (&y)[NSP], const double (&k)[NRATE]) {
+#pragma clang fp contract(off)
+  const double H2_1 = y[0], H2_2 = y[1], H_1 = y[2], H_2 = y[3], H_m0 = y[4], He_1 = y[5],
+               He_2 = y[6], He_3 = y[7], de = y[8];
+  Reactions R;
+  R.r01 = k[K01] * H_1 * de;
+  R.r02 = k[K02] * H_2 * de;
+  R.r03 = k[K03] * He_1 * de;
+  R.r04 = k[K04] * He_2 * de;
+  R.r05 = k[K05] * He_2 * de;
+  R.r06 = k[K06] * He_3 * de;
+  R.r07 = k[K07] * H_1 * de;
+  R.r08 = k[K08] * H_1 * H_m0;
+  R.r09 = k[K09] * H_1 * H_2;
+  R.r10 = k[K10] * H2_2 * H_1;
+  R.r11 = k[K11] * H2_1 * H_2;
+  R.r12 = k[K12] * H2_1 * de;
+  R.r13 = k[K13] * H2_1 * H_1;
+  R.r14 = k[K14] * H_m0 * de;
+  R.r15 = k[K15] * H_1 * H_m0;
+  R.r16 = k[K16] * H_2 * H_m0;
+  R.r17 = k[K17] * H_2 * H_m0;
+  R.r18 = k[K18] * H2_2 * de;
+  R.r19 = k[K19] * H2_2 * H_m0;
+  R.r21 = k[K21] * H2_1 * H_1 * H_1;
+  R.r22 = k[K22] * H_1 * H_1 * H_1;
+  return R;
+}
+
+// Species and gas-energy RHS in physical units (raja_primordial_network.cpp:1302-1371,
+// before the inv_scale / inv_mdensity factors).
+__device__ inline void chem_rhs_physical(const double (&y)[NSP], const double (&k)[NRATE],
+                                         double T, double z, const CellAux &ax,
+                                         double (&f)[NSP]) {
+#pragma clang fp contract(off)
+  const Reactions R = chem_reactions(y, k);
+  f[0] = R.r08 + R.r10 - R.r11 - R.r12 - R.r13 + R.r19 + R.r21 + R.r22;
+  f[1] = R.r09 - R.r10 + R.r11 + R.r17 - R.r18 - R.r19;
+  f[2] = -R.r01 + R.r02 - R.r07 - R.r08 - R.r09 - R.r10 + R.r11 + 2.0 * R.r12 + 2.0 * R.r13 +
+         R.r14 + R.r15 + 2.0 * R.r16 + 2.0 * R.r18 + R.r19 - 2.0 * R.r21 - 2.0 * R.r22;
+  f[3] = R.r01 - R.r02 - R.r09 + R.r10 - R.r11 - R.r16 - R.r17;
+  f[4] = R.r07 - R.r08 - R.r14 - R.r15 - R.r16 - R.r17 - R.r19;
+  f[5] = -R.r03 + R.r04;
+  f[6] = R.r03 - R.r04 - R.r05 + R.r06;
+  f[7] = R.r05 - R.r06;
+  f[8] = R.r01 - R.r02 + R.r03 - R.r04 + R.r05 - R.r06 - R.r07 + R.r08 + R.r14 + R.r15 + R.r17 -
+         R.r18;
+  const double H2_1 = y[0], H_1 = y[2], H_2 = y[3], He_1 = y[5], He_2 = y[6], He_3 = y[7],
+               de = y[8];
+  const double cie = ax.cie, h2o = ax.h2o;
+  const double gS = H2_1 * k[C_GAH2] + H_1 * k[C_GAHI] + H_2 * k[C_GAHP] + He_1 * k[C_GAHE] +
+                    de * k[C_GAEL];
+  const double ncrD = H2_1 * k[C_NCRD2] + H_1 * k[C_NCRD1];
+  const double zp1 = z + 1.0;
+  const double zp4 = (zp1 * zp1) * (zp1 * zp1);
+  f[9] = -2.0158800000000001 * H2_1 * k[C_CIECO] * cie * ax.mdensity -
+         H2_1 * cie * k[C_H2LTE] * h2o / (k[C_H2LTE] / gS + 1.0) - H_1 * k[C_CEHI] * cie * de -
+         H_1 * k[C_CIHI] * cie * de - H_2 * cie * de * k[C_REHII] -
+         He_1 * k[C_CIHEI] * cie * de - He_2 * k[C_CEHEII] * cie * de -
+         He_2 * k[C_CEHEI] * cie * (de * de) - He_2 * k[C_CIHEII] * cie * de -
+         He_2 * k[C_CIHEIS] * cie * (de * de) - He_2 * cie * de * k[C_REHEII1] -
+         He_2 * cie * de * k[C_REHEII2] - He_3 * cie * de * k[C_REHEIII] -
+         k[C_BREM] * cie * de * (H_2 + He_2 + 4.0 * He_3) -
+         cie * k[C_COMPTON] * de * zp4 * (T - 2.73 * z - 2.73) +
+         0.5 * 1.0 / (k[C_NCRN] / ncrD + 1.0) *
+             ((-H2_1) * H_1 * k[C_H2MCOOL] + (H_1 * H_1 * H_1) * k[C_H2MHEAT]);
+}
+
+// Analytic Jacobian, physical units, row-major J[r][c] with the reference's extra
+// factors folded in (Tge on the ge column, 1/mdensity on the ge row, the replaced ge-ge
+// entry), raja_primordial_network.cpp:1518-1867. Structural zeros are left at 0.
+__device__ inline void chem_jac_physical(const double (&y)[NSP], const double (&k)[NRATE],
+                                         const double (&rk)[NRATE], double T, double z,
+                                         double Tge, const CellAux &ax,
+                                         double (&J)[NSP][NSP]) {
+#pragma clang fp contract(off)
+  const double H2_1 = y[0], H2_2 = y[1], H_1 = y[2], H_2 = y[3], H_m0 = y[4], He_1 = y[5],
+               He_2 = y[6], He_3 = y[7], de = y[8];
+#pragma unroll
+  for (int a = 0; a < NSP; a++)
+#pragma unroll
+    for (int b = 0; b < NSP; b++) J[a][b] = 0.0;
+  const double k01 = k[K01], k02 = k[K02], k03 = k[K03], k04 = k[K04], k05 = k[K05],
+               k06 = k[K06], k07 = k[K07], k08 = k[K08], k09 = k[K09], k10 = k[K10],
+               k11 = k[K11], k12 = k[K12], k13 = k[K13], k14 = k[K14], k15 = k[K15],
+               k16 = k[K16], k17 = k[K17], k18 = k[K18], k19 = k[K19], k21 = k[K21],
+               k22 = k[K22];
+  const double H1sq = H_1 * H_1;
+  // temperature derivatives of the species rows: d(rhs_s)/dT at fixed species, times Tge
+  double dk[NRATE];
+#pragma unroll
+  for (int r = 0; r < NRATE; r++) dk[r] = rk[r];
+  const Reactions D = chem_reactions(y, dk);   // rates built from dk/dT instead of k
+
+  J[0][0] = (-k11) * H_2 - k12 * de - k13 * H_1 + k21 * H1sq;
+  J[0][1] = k10 * H_1 + k19 * H_m0;
+  J[0][2] = k08 * H_m0 + k10 * H2_2 - k13 * H2_1 + 2.0 * k21 * H2_1 * H_1 + 3.0 * k22 * H1sq;
+  J[0][3] = (-k11) * H2_1;
+  J[0][4] = k08 * H_1 + k19 * H2_2;
+  J[0][8] = (-k12) * H2_1;
+  J[0][9] = (D.r08 + D.r10 - D.r11 - D.r12 - D.r13 + D.r19 + D.r21 + D.r22) * Tge;
+
+  J[1][0] = k11 * H_2;
+  J[1][1] = (-k10) * H_1 - k18 * de - k19 * H_m0;
+  J[1][2] = k09 * H_2 - k10 * H2_2;
+  J[1][3] = k09 * H_1 + k11 * H2_1 + k17 * H_m0;
+  J[1][4] = k17 * H_2 - k19 * H2_2;
+  J[1][8] = (-k18) * H2_2;
+  J[1][9] = (D.r09 - D.r10 + D.r11 + D.r17 - D.r18 - D.r19) * Tge;
+
+  J[2][0] = k11 * H_2 + 2.0 * k12 * de + 2.0 * k13 * H_1 - 2.0 * k21 * H1sq;
+  J[2][1] = (-k10) * H_1 + 2.0 * k18 * de + k19 * H_m0;
+  J[2][2] = (-k01) * de - k07 * de - k08 * H_m0 - k09 * H_2 - k10 * H2_2 + 2.0 * k13 * H2_1 +
+            k15 * H_m0 - 4.0 * k21 * H2_1 * H_1 - 6.0 * k22 * H1sq;
+  J[2][3] = k02 * de - k09 * H_1 + k11 * H2_1 + 2.0 * k16 * H_m0;
+  J[2][4] = (-k08) * H_1 + k14 * de + k15 * H_1 + 2.0 * k16 * H_2 + k19 * H2_2;
+  J[2][8] = (-k01) * H_1 + k02 * H_2 - k07 * H_1 + 2.0 * k12 * H2_1 + k14 * H_m0 +
+            2.0 * k18 * H2_2;
+  J[2][9] = (-D.r01 + D.r02 - D.r07 - D.r08 - D.r09 - D.r10 + D.r11 + 2.0 * D.r12 +
+             2.0 * D.r13 + D.r14 + D.r15 + 2.0 * D.r16 + 2.0 * D.r18 + D.r19 - 2.0 * D.r21 -
+             2.0 * D.r22) * Tge;
+
+  J[3][0] = (-k11) * H_2;
+  J[3][1] = k10 * H_1;
+  J[3][2] = k01 * de - k09 * H_2 + k10 * H2_2;
+  J[3][3] = (-k02) * de - k09 * H_1 - k11 * H2_1 - k16 * H_m0 - k17 * H_m0;
+  J[3][4] = (-k16) * H_2 - k17 * H_2;
+  J[3][8] = k01 * H_1 - k02 * H_2;
+  J[3][9] = (D.r01 - D.r02 - D.r09 + D.r10 - D.r11 - D.r16 - D.r17) * Tge;
+
+  J[4][1] = (-k19) * H_m0;
+  J[4][2] = k07 * de - k08 * H_m0 - k15 * H_m0;
+  J[4][3] = (-k16) * H_m0 - k17 * H_m0;
+  J[4][4] = (-k08) * H_1 - k14 * de - k15 * H_1 - k16 * H_2 - k17 * H_2 - k19 * H2_2;
+  J[4][8] = k07 * H_1 - k14 * H_m0;
+  J[4][9] = (D.r07 - D.r08 - D.r14 - D.r15 - D.r16 - D.r17 - D.r19) * Tge;
+
+  J[5][5] = (-k03) * de;
+  J[5][6] = k04 * de;
+  J[5][8] = (-k03) * He_1 + k04 * He_2;
+  J[5][9] = (-D.r03 + D.r04) * Tge;
+
+  J[6][5] = k03 * de;
+  J[6][6] = (-k04) * de - k05 * de;
+  J[6][7] = k06 * de;
+  J[6][8] = k03 * He_1 - k04 * He_2 - k05 * He_2 + k06 * He_3;
+  J[6][9] = (D.r03 - D.r04 - D.r05 + D.r06) * Tge;
+
+  J[7][6] = k05 * de;
+  J[7][7] = (-k06) * de;
+  J[7][8] = k05 * He_2 - k06 * He_3;
+  J[7][9] = (D.r05 - D.r06) * Tge;
+
+  J[8][1] = (-k18) * de;
+  J[8][2] = k01 * de - k07 * de + k08 * H_m0 + k15 * H_m0;
+  J[8][3] = (-k02) * de + k17 * H_m0;
+  J[8][4] = k08 * H_1 + k14 * de + k15 * H_1 + k17 * H_2;
+  J[8][5] = k03 * de;
+  J[8][6] = (-k04) * de + k05 * de;
+  J[8][7] = (-k06) * de;
+  J[8][8] = k01 * H_1 - k02 * H_2 + k03 * He_1 - k04 * He_2 + k05 * He_2 - k06 * He_3 -
+            k07 * H_1 + k14 * H_m0 - k18 * H2_2;
+  J[8][9] = (D.r01 - D.r02 + D.r03 - D.r04 + D.r05 - D.r06 - D.r07 + D.r08 + D.r14 + D.r15 +
+             D.r17 - D.r18) * Tge;
+
+  // gas-energy row
+  const double brem = k[C_BREM], ceHeI = k[C_CEHEI], ceHeII = k[C_CEHEII], ceHI = k[C_CEHI],
+               cieco = k[C_CIECO], ciHeI = k[C_CIHEI], ciHeII = k[C_CIHEII],
+               ciHeIS = k[C_CIHEIS], ciHI = k[C_CIHI], compton = k[C_COMPTON],
+               gael = k[C_GAEL], gaH2 = k[C_GAH2], gaHe = k[C_GAHE], gaHI = k[C_GAHI],
+               gaHp = k[C_GAHP], h2lte = k[C_H2LTE], h2mcool = k[C_H2MCOOL],
+               h2mheat = k[C_H2MHEAT], ncrd1 = k[C_NCRD1], ncrd2 = k[C_NCRD2], ncrn = k[C_NCRN],
+               reHeII1 = k[C_REHEII1], reHeII2 = k[C_REHEII2], reHeIII = k[C_REHEIII],
+               reHII = k[C_REHII];
+  const double inv_md = 1.0 / ax.mdensity;
+  const double h2o = ax.h2o;
+  const double S = H2_1 * gaH2 + H_1 * gaHI + H_2 * gaHp + He_1 * gaHe + de * gael;
+  const double Dn = H2_1 * ncrd2 + H_1 * ncrd1;
+  const double gden = h2lte / S + 1.0;
+  const double nden = ncrn / Dn + 1.0;
+  const double nden_m2 = 1.0 / (nden * nden);
+  const double Y = (-H2_1) * H_1 * h2mcool + (H_1 * H_1 * H_1) * h2mheat;
+  const double gq = (h2lte * h2lte) * h2o / ((gden * gden) * (S * S));
+  const double zp1 = z + 1.0;
+  const double zp4 = (zp1 * zp1) * (zp1 * zp1);
+  J[9][0] = ((-H2_1) * gaH2 * gq - 0.5 * H_1 * h2mcool * 1.0 / nden -
+             2.0158800000000001 * cieco * ax.mdensity - h2lte * h2o / gden +
+             0.5 * ncrd2 * ncrn * nden_m2 * Y / (Dn * Dn)) * inv_md;
+  J[9][2] = ((-H2_1) * gaHI * gq - ceHI * de - ciHI * de +
+             0.5 * ncrd1 * ncrn * nden_m2 * Y / (Dn * Dn) +
+             0.5 * ((-H2_1) * h2mcool + 3.0 * H1sq * h2mheat) * 1.0 / nden) * inv_md;
+  J[9][3] = ((-H2_1) * gaHp * gq - brem * de - de * reHII) * inv_md;
+  J[9][5] = ((-H2_1) * gaHe * gq - ciHeI * de) * inv_md;
+  J[9][6] = ((-brem) * de - ceHeII * de - ceHeI * (de * de) - ciHeII * de -
+             ciHeIS * (de * de) - de * reHeII1 - de * reHeII2) * inv_md;
+  J[9][7] = (-4.0 * brem * de - de * reHeIII) * inv_md;
+  J[9][8] = ((-H2_1) * gael * gq - H_1 * ceHI - H_1 * ciHI - H_2 * reHII - He_1 * ciHeI -
+             He_2 * ceHeII - 2.0 * He_2 * ceHeI * de - He_2 * ciHeII -
+             2.0 * He_2 * ciHeIS * de - He_2 * reHeII1 - He_2 * reHeII2 - He_3 * reHeIII -
+             brem * (H_2 + He_2 + 4.0 * He_3) - compton * zp4 * (T - 2.73 * z - 2.73)) * inv_md;
+  const double Rg = (-H2_1) * rk[C_GAH2] - H_1 * rk[C_GAHI] - H_2 * rk[C_GAHP] -
+                    He_1 * rk[C_GAHE] - de * rk[C_GAEL];
+  const double adhoc =
+      (-H2_1) * h2lte * h2o * ((-h2lte) * Rg / (S * S) - rk[C_H2LTE] / S) / (gden * gden) -
+      H2_1 * h2o * rk[C_H2LTE] / gden +
+      0.5 * nden_m2 * Y *
+          (-1.0 * ncrn * ((-H2_1) * rk[C_NCRD2] - H_1 * rk[C_NCRD1]) / (Dn * Dn) -
+           1.0 * rk[C_NCRN] / Dn) +
+      0.5 * 1.0 / nden * ((-H2_1) * H_1 * rk[C_H2MCOOL] + (H_1 * H_1 * H_1) * rk[C_H2MHEAT]);
+  J[9][9] = adhoc * inv_md * Tge;
+}
+
+}  // namespace mvd

@@ -1,0 +1,314 @@
+// Device-side building blocks of the WENO5 flux divergence fe() for CDNA4 (gfx950), FP64.
+//
+// Data layout in HBM (the reference's ManyVector layout, euler3D.hpp:62-65):
+//   fluid fields rho, mx, my, mz, et: 5 separate arrays, cell index i + nx*(j + ny*k);
+//   advected species: one AoS array, species fastest: v + nchem*(i + nx*(j + ny*k)).
+//
+// Two face-flux implementations live here:
+//   face_flux_ref  - the reference recipe (utilities.cpp:270-479) operation by operation,
+//                    with FMA contraction disabled; used by the plain comparator kernels
+//                    and reproducible bit for bit against a non-FMA CPU build.
+//   face_flux_fast - the same mathematics rearranged for the FP64 pipes: per-point
+//                    primitive values computed once, 1/rho instead of repeated divides,
+//                    the left eigenvector product factored into 14 operations, and one
+//                    divide per WENO weight set.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mvd {
+
+constexpr int BC_PERIODIC = 0, BC_NEUMANN = 1, BC_DIRICHLET = 2, BC_REFLECTING = 3;
+constexpr int NCHEM_MAX = 16;
+constexpr int NV_MAX = 5 + NCHEM_MAX;
+
+// Grid and boundary description passed by value to every fe() kernel.
+struct Grid {
+  int nx, ny, nz, nchem;
+  int bc[6];               // xl, xr, yl, yr, zl, zr
+  double gamma, dx, dy, dz;
+};
+
+__host__ __device__ inline int extent(const Grid &g, int d) {
+  return d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
+}
+
+// Source position and sign of extended index e (ghost layer 3 deep) along direction d.
+// Physical boundaries follow the reference's ghost fill (euler3D.hpp:793-1166):
+// low faces mirror (e -> -1-e), high faces copy shifted (e -> e-3); periodic wraps.
+// sgn_all = -1 for Dirichlet (all fields negated); sgn_norm = -1 for the normal momentum
+// under reflecting (and Dirichlet) boundaries.
+struct GhostMap {
+  int src;
+  double sgn_all, sgn_norm;
+};
+
+__device__ inline GhostMap ghost_map(const Grid &g, int d, int e) {
+  const int n = extent(g, d);
+  GhostMap m{e, 1.0, 1.0};
+  if (e < 0 || e >= n) {
+    const bool lo = e < 0;
+    const int bc = g.bc[2 * d + (lo ? 0 : 1)];
+    if (bc == BC_PERIODIC) {
+      m.src = lo ? e + n : e - n;
+    } else {
+      m.src = lo ? -1 - e : e - 3;
+      if (bc == BC_DIRICHLET) { m.sgn_all = -1.0; m.sgn_norm = -1.0; }
+      else if (bc == BC_REFLECTING) { m.sgn_norm = -1.0; }
+    }
+  }
+  return m;
+}
+
+// SUNRsqrt (sundials_math.h): non-positive arguments give 0.
+__device__ inline double sun_sqrt(double x) { return (x <= 0.0) ? 0.0 : sqrt(x); }
+
+// ---------------------------------------------------------------------------------
+// Reference-order face flux (no contraction). s[l][v], l = stencil point, v = field in
+// direction-local order (rho, m_normal, m_t1, m_t2, et, species...).
+// ---------------------------------------------------------------------------------
+template <int NC>
+__device__ inline void face_flux_ref(double (&s)[6][5 + NC], double gamma, double *ff_out) {
+#pragma clang fp contract(off)
+  constexpr int NV = 5 + NC;
+  const double bc = 1.083333333333333333333333333333333333333;
+  const double eps = 1e-6;
+  const double c13 = 0.3333333333333333333333333333333333333333;
+  const double c56 = 0.8333333333333333333333333333333333333333;
+  const double c16 = 0.1666666666666666666666666666666666666667;
+  const double c76 = 1.166666666666666666666666666666666666667;
+  const double c116 = 1.833333333333333333333333333333333333333;
+  double p[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++)
+    p[l] = (gamma - 1.0) * (s[l][4] - (s[l][1] * s[l][1] + s[l][2] * s[l][2] + s[l][3] * s[l][3]) * 0.5 / s[l][0]);
+  const double sL = sun_sqrt(s[2][0]), sR = sun_sqrt(s[3][0]);
+  const double sbar = 0.5 * (sL + sR);
+  const double u = 0.5 * (s[2][1] / sL + s[3][1] / sR) / sbar;
+  const double v = 0.5 * (s[2][2] / sL + s[3][2] / sR) / sbar;
+  const double w = 0.5 * (s[2][3] / sL + s[3][3] / sR) / sbar;
+  const double H = 0.5 * ((p[2] + s[2][4]) / sL + (p[3] + s[3][4]) / sR) / sbar;
+  const double qsq = u * u + v * v + w * w;
+  const double gm = gamma - 1.0;
+  const double cs = gm * (H - 0.5 * qsq);
+  const double ci = 1.0 / cs;
+  const double RV[5][5] = {{1.0, 0.0, 0.0, 1.0, 1.0},
+                           {u - cs, 0.0, 0.0, u, u + cs},
+                           {v, 1.0, 0.0, v, v},
+                           {w, 0.0, 1.0, w, w},
+                           {H - u * cs, v, w, 0.5 * qsq, H + u * cs}};
+  const double LV[5][5] = {
+      {0.5 * ci * (u + 0.5 * gm * qsq), -0.5 * ci * (gm * u + 1.0), -0.5 * v * gm * ci,
+       -0.5 * w * gm * ci, 0.5 * gm * ci},
+      {-v, 0.0, 1.0, 0.0, 0.0},
+      {-w, 0.0, 0.0, 1.0, 0.0},
+      {(-gm) * ci * (qsq - H), u * gm * ci, v * gm * ci, w * gm * ci, (-gm) * ci},
+      {-0.5 * ci * (u - 0.5 * gm * qsq), -0.5 * ci * (gm * u - 1.0), -0.5 * v * gm * ci,
+       -0.5 * w * gm * ci, 0.5 * gm * ci}};
+  double flux[6][NV];
+  double alpha = 0.0;
+#pragma unroll
+  for (int l = 0; l < 6; l++) {
+    const double ul = s[l][1] / s[l][0];
+    flux[l][0] = s[l][1];
+    flux[l][1] = ul * s[l][1] + p[l];
+    flux[l][2] = ul * s[l][2];
+    flux[l][3] = ul * s[l][3];
+    flux[l][4] = ul * (s[l][4] + p[l]);
+#pragma unroll
+    for (int c = 0; c < NC; c++) flux[l][5 + c] = ul * s[l][5 + c];
+    const double cl = sun_sqrt(gamma * p[l] / s[l][0]);
+    const double al = fabs(ul) + cl;
+    alpha = (alpha < al) ? al : alpha;
+  }
+  double ff[NV];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    double fp[5][NV];
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      double fs[NV];
+#pragma unroll
+      for (int c = 0; c < NV; c++)
+        fs[c] = side == 0 ? 0.5 * (flux[l][c] + alpha * s[l][c])
+                          : 0.5 * (flux[l + 1][c] - alpha * s[l + 1][c]);
+#pragma unroll
+      for (int a = 0; a < 5; a++)
+        fp[l][a] = LV[a][0] * fs[0] + LV[a][1] * fs[1] + LV[a][2] * fs[2] + LV[a][3] * fs[3] + LV[a][4] * fs[4];
+#pragma unroll
+      for (int c = 0; c < NC; c++) fp[l][5 + c] = fs[5 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < NV; c++) {
+      const double f0 = fp[0][c], f1 = fp[1][c], f2 = fp[2][c], f3 = fp[3][c], f4 = fp[4][c];
+      const double x1 = f2 - 2.0 * f3 + f4, y1 = 3.0 * f2 - 4.0 * f3 + f4;
+      const double x2 = f1 - 2.0 * f2 + f3, y2 = f1 - f3;
+      const double x3 = f0 - 2.0 * f1 + f2, y3 = f0 - 4.0 * f1 + 3.0 * f2;
+      const double b1 = bc * (x1 * x1) + 0.25 * (y1 * y1);
+      const double b2 = bc * (x2 * x2) + 0.25 * (y2 * y2);
+      const double b3 = bc * (x3 * x3) + 0.25 * (y3 * y3);
+      if (side == 0) {
+        const double w1 = 0.3 / ((eps + b1) * (eps + b1));
+        const double w2 = 0.6 / ((eps + b2) * (eps + b2));
+        const double w3 = 0.1 / ((eps + b3) * (eps + b3));
+        const double q1 = c13 * f2 + c56 * f3 - c16 * f4;
+        const double q2 = (-c16) * f1 + c56 * f2 + c13 * f3;
+        const double q3 = c13 * f0 - c76 * f1 + c116 * f2;
+        ff[c] = (q1 * w1 + q2 * w2 + q3 * w3) / (w1 + w2 + w3);
+      } else {
+        const double w1 = 0.1 / ((eps + b1) * (eps + b1));
+        const double w2 = 0.6 / ((eps + b2) * (eps + b2));
+        const double w3 = 0.3 / ((eps + b3) * (eps + b3));
+        const double q1 = c116 * f2 - c76 * f3 + c13 * f4;
+        const double q2 = c13 * f1 + c56 * f2 - c16 * f3;
+        const double q3 = (-c16) * f0 + c56 * f1 + c13 * f2;
+        ff[c] += (q1 * w1 + q2 * w2 + q3 * w3) / (w1 + w2 + w3);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 5; a++)
+    ff_out[a] = RV[a][0] * ff[0] + RV[a][1] * ff[1] + RV[a][2] * ff[2] + RV[a][3] * ff[3] + RV[a][4] * ff[4];
+#pragma unroll
+  for (int c = 0; c < NC; c++) ff_out[5 + c] = ff[5 + c];
+}
+
+// ---------------------------------------------------------------------------------
+// Fast face flux.
+// ---------------------------------------------------------------------------------
+
+// Per-point values along the sweep direction (direction-local momentum order).
+struct Pt {
+  double r, mn, m1, m2, e;   // conserved: rho, normal momentum, transverse momenta, energy
+  double p, ir, a, sr;       // pressure, 1/rho, |u_n| + c, sqrt(rho)
+};
+
+__device__ inline Pt make_point(double r, double mn, double m1, double m2, double e,
+                                double gamma) {
+  Pt q;
+  q.r = r; q.mn = mn; q.m1 = m1; q.m2 = m2; q.e = e;
+  q.ir = 1.0 / r;
+  q.p = (gamma - 1.0) * (e - 0.5 * (mn * mn + m1 * m1 + m2 * m2) * q.ir);
+  const double un = mn * q.ir;
+  q.a = fabs(un) + sun_sqrt(gamma * q.p * q.ir);
+  q.sr = sun_sqrt(r);
+  return q;
+}
+
+// WENO5-JS reconstruction of one component from the five split-flux values, with the
+// three nonlinear weights d_k/(eps+beta_k)^2 combined over one common denominator:
+//   sum_k q_k d_k/s_k / sum_k d_k/s_k = sum_k q_k d_k t_k / sum_k d_k t_k,  t_k = prod_{m!=k} s_m
+// PLUS selects the f+ stencils/weights (0.3,0.6,0.1), otherwise the f- ones (0.1,0.6,0.3).
+template <bool PLUS>
+__device__ inline double weno5(double f0, double f1, double f2, double f3, double f4) {
+  const double bc = 13.0 / 12.0;
+  const double eps = 1e-6;
+  const double x1 = f2 - 2.0 * f3 + f4, y1 = 3.0 * f2 - 4.0 * f3 + f4;
+  const double x2 = f1 - 2.0 * f2 + f3, y2 = f1 - f3;
+  const double x3 = f0 - 2.0 * f1 + f2, y3 = f0 - 4.0 * f1 + 3.0 * f2;
+  const double b1 = bc * (x1 * x1) + 0.25 * (y1 * y1);
+  const double b2 = bc * (x2 * x2) + 0.25 * (y2 * y2);
+  const double b3 = bc * (x3 * x3) + 0.25 * (y3 * y3);
+  const double s1 = (eps + b1) * (eps + b1);
+  const double s2 = (eps + b2) * (eps + b2);
+  const double s3 = (eps + b3) * (eps + b3);
+  double q1, q2, q3, d1, d3;
+  if (PLUS) {
+    q1 = (2.0 * f2 + 5.0 * f3 - f4) * (1.0 / 6.0);
+    q2 = (-f1 + 5.0 * f2 + 2.0 * f3) * (1.0 / 6.0);
+    q3 = (2.0 * f0 - 7.0 * f1 + 11.0 * f2) * (1.0 / 6.0);
+    d1 = 0.3; d3 = 0.1;
+  } else {
+    q1 = (11.0 * f2 - 7.0 * f3 + 2.0 * f4) * (1.0 / 6.0);
+    q2 = (2.0 * f1 + 5.0 * f2 - f3) * (1.0 / 6.0);
+    q3 = (-f0 + 5.0 * f1 + 2.0 * f2) * (1.0 / 6.0);
+    d1 = 0.1; d3 = 0.3;
+  }
+  const double t1 = d1 * (s2 * s3), t2 = 0.6 * (s1 * s3), t3 = d3 * (s1 * s2);
+  return (q1 * t1 + q2 * t2 + q3 * t3) / (t1 + t2 + t3);
+}
+
+// Face flux from the 6 stencil points P[0..5] (points i-3..i+2 of face i-1/2).
+// Output F[0..4] in direction-local order. Also returns alpha (LF speed) and the
+// point normal velocities, which the species part reuses.
+__device__ inline void face_flux_fast(const Pt (&P)[6], double gamma, double (&F)[5],
+                                      double &alpha_out, double (&un)[6]) {
+  const double gm = gamma - 1.0;
+  // Roe average (utilities.cpp:298-304 rewritten with m/sqrt(rho) = u*sqrt(rho))
+  const Pt &L = P[2], &R = P[3];
+  const double iden = 1.0 / (L.sr + R.sr);
+  const double wl = L.sr * L.ir, wr = R.sr * R.ir;   // = 1/sqrt(rho)
+  const double u = (L.mn * wl + R.mn * wr) * iden;
+  const double v = (L.m1 * wl + R.m1 * wr) * iden;
+  const double w = (L.m2 * wl + R.m2 * wr) * iden;
+  const double H = ((L.p + L.e) * wl + (R.p + R.e) * wr) * iden;
+  const double qsq = u * u + v * v + w * w;
+  const double hq = 0.5 * qsq;
+  const double cs = gm * (H - hq);       // (gamma-1)(H - q^2/2), as in the reference
+  const double ci = 1.0 / cs;
+  // LV applied to a vector x, with the 1/2 of the flux split folded in:
+  //   K  = (gamma-1)(q^2/2 x0 - u x1 - v x2 - w x3 + x4)
+  //   A  = u x0 - x1
+  //   rows: 0: ci/2 (A+K)   1: x2 - v x0   2: x3 - w x0   3: x0 - ci K   4: ci/2 (K-A)
+  const double hci = 0.25 * ci;          // (1/2 split) * (1/2 row factor) * ci
+  double alpha = 0.0;
+#pragma unroll
+  for (int l = 0; l < 6; l++) {
+    un[l] = P[l].mn * P[l].ir;
+    alpha = fmax(alpha, P[l].a);
+  }
+  double ff[5];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    double pr[5][5];
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      const Pt &q = P[l + side];
+      const double ul = un[l + side];
+      const double sa = side == 0 ? alpha : -alpha;
+      const double x0 = q.mn + sa * q.r;
+      const double x1 = ul * q.mn + q.p + sa * q.mn;
+      const double x2 = ul * q.m1 + sa * q.m1;
+      const double x3 = ul * q.m2 + sa * q.m2;
+      const double x4 = ul * (q.e + q.p) + sa * q.e;
+      const double K = gm * (hq * x0 - u * x1 - v * x2 - w * x3 + x4);
+      const double A = u * x0 - x1;
+      pr[l][0] = hci * (A + K);
+      pr[l][1] = 0.5 * (x2 - v * x0);
+      pr[l][2] = 0.5 * (x3 - w * x0);
+      pr[l][3] = 0.5 * x0 - (0.5 * ci) * K;
+      pr[l][4] = hci * (K - A);
+    }
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      const double r = side == 0 ? weno5<true>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c])
+                                 : weno5<false>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c]);
+      ff[c] = side == 0 ? r : ff[c] + r;
+    }
+  }
+  // RV applied to ff (utilities.cpp:470-472), factored
+  const double s04 = ff[0] + ff[4];
+  const double d40 = cs * (ff[4] - ff[0]);
+  F[0] = s04 + ff[3];
+  F[1] = u * F[0] + d40;
+  F[2] = v * F[0] + ff[1];
+  F[3] = w * F[0] + ff[2];
+  F[4] = H * s04 + u * d40 + v * ff[1] + w * ff[2] + hq * ff[3];
+  alpha_out = alpha;
+}
+
+// Species (passive tracer) face flux for one component from its 6 point values c[l],
+// the point normal velocities un[l] and the fluid alpha (utilities.cpp:376-377,395,439,473).
+__device__ inline double tracer_face_flux(const double (&c)[6], const double (&un)[6],
+                                          double alpha) {
+  double fp[5], fm[5];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    fp[l] = 0.5 * (un[l] * c[l] + alpha * c[l]);
+    fm[l] = 0.5 * (un[l + 1] * c[l + 1] - alpha * c[l + 1]);
+  }
+  return weno5<true>(fp[0], fp[1], fp[2], fp[3], fp[4]) +
+         weno5<false>(fm[0], fm[1], fm[2], fm[3], fm[4]);
+}
+
+}  // namespace mvd

@@ -1,0 +1,127 @@
+// Reductions and small elementwise kernels around fe(): the CFL wave-speed reduction
+// (stability, utilities.cpp:483-528), the fslow energy/scaling glue
+// (multirate_chem_hydro_main.cpp:1024-1077) and the Dengo scaling helpers
+// (primordial_blast.cpp:404-458).
+//
+// Reductions are two-pass and deterministic: each block reduces its grid-stride share
+// with 64-lane wavefront shuffles and one LDS exchange, writes one partial; a single
+// block reduces the partials.
+#include "mvd_internal.hpp"
+
+namespace mvd {
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <bool MAX>
+__device__ inline double block_reduce(double v) {
+  __shared__ double part[16];
+  v = MAX ? wave_max(v) : wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) part[wid] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  if (wid == 0) {
+    v = (lane < nw) ? part[lane] : (MAX ? 0.0 : 0.0);
+    v = MAX ? wave_max(v) : wave_sum(v);
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(256)
+wave_speed_partial(Grid g, Fields f, long n, double *partial) {
+  double a = 0.0;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (long)gridDim.x * blockDim.x) {
+    const double rho = f.w[0][c], mx = f.w[1][c], my = f.w[2][c], mz = f.w[3][c], e = f.w[4][c];
+    const double u = fabs(mx / rho);   // the reference uses |mx/rho| for all three directions
+    const double p = (g.gamma - 1.0) * (e - (mx * mx + my * my + mz * mz) * 0.5 / rho);
+    const double cs = sun_sqrt(g.gamma * p / rho);
+    a = fmax(a, fabs(u + cs));
+  }
+  a = block_reduce<true>(a);
+  if (threadIdx.x == 0) partial[blockIdx.x] = a;
+}
+
+__global__ void __launch_bounds__(256) reduce_max_final(const double *partial, int n, double *out) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a = fmax(a, partial[i]);
+  a = block_reduce<true>(a);
+  if (threadIdx.x == 0) out[0] = a;
+}
+
+int max_wave_speed_launch(const Grid &g, const Fields &f, FeScratch &s, hipStream_t st,
+                          double *host_out) {
+  const long n = (long)g.nx * g.ny * g.nz;
+  int nb = (int)std::min<long>((n + 255) / 256, 2048);
+  if (nb < 1) nb = 1;
+  if (s.red_len < nb + 1) return -3;
+  hipLaunchKernelGGL(wave_speed_partial, dim3(nb), dim3(256), 0, st, g, f, n, s.red);
+  hipLaunchKernelGGL(reduce_max_final, dim3(1), dim3(256), 0, st, s.red, nb, s.red + nb);
+  if (hipMemcpyAsync(host_out, s.red + nb, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
+    return -3;
+  if (hipStreamSynchronize(st) != hipSuccess) return -3;
+  return 0;
+}
+
+// fslow, first half: chem *= scale; et <- chem[nchem-1]/EnergyUnits + 0.5/rho |m|^2
+__global__ void __launch_bounds__(256)
+fslow_prepare(long n, int nchem, double *rho, double *mx, double *my, double *mz, double *et,
+              double *chem, const double *scale, double eunit_inv) {
+#pragma clang fp contract(off)
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  for (int v = 0; v < nchem; v++) chem[(long)nchem * c + v] *= scale[(long)nchem * c + v];
+  const double r = rho[c], a = mx[c], b = my[c], d = mz[c];
+  et[c] = chem[(long)nchem * c + nchem - 1] * eunit_inv + 0.5 / r * (a * a + b * b + d * d);
+}
+
+// fslow, second half: chemdot[nchem-1] <- etdot; etdot <- 0
+__global__ void __launch_bounds__(256)
+fslow_finish(long n, int nchem, double *etdot, double *chemdot) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  chemdot[(long)nchem * c + nchem - 1] = etdot[c] * 1.0;
+  etdot[c] = 0.0;
+}
+
+__global__ void __launch_bounds__(256)
+chem_scale(long n, double *chem, const double *scale, int apply) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (apply) chem[i] *= scale[i];
+  else chem[i] /= scale[i];
+}
+
+int fslow_prepare_launch(const Grid &g, double *const *w, const double *scale,
+                         double eunit_inv, hipStream_t st) {
+  const long n = (long)g.nx * g.ny * g.nz;
+  hipLaunchKernelGGL(fslow_prepare, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                     g.nchem, w[0], w[1], w[2], w[3], w[4], w[5], scale, eunit_inv);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int fslow_finish_launch(const Grid &g, double *const *wdot, hipStream_t st) {
+  const long n = (long)g.nx * g.ny * g.nz;
+  hipLaunchKernelGGL(fslow_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                     g.nchem, wdot[4], wdot[5]);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int chem_scale_launch(long n, double *chem, const double *scale, int apply, hipStream_t st) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(chem_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, chem,
+                     scale, apply);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+}  // namespace mvd
