@@ -76,6 +76,14 @@ int mvd_stability(mvd_context *ctx, const double *const *w, double t, double *dt
  * unscales chem. Requires nchem > 0 and mvd_chem_setup. */
 int mvd_fslow(mvd_context *ctx, double t, double *const *w, double *const *wdot);
 
+/* Kernel timing with HIP events on the context's stream (the reference times every RHS
+ * and face flux with its Profile class, profiler.hpp:23-206). mvd_profile(ctx,1) resets
+ * and enables; mvd_profile_read returns the number of slots and fills per-slot total
+ * milliseconds and launch counts. Slots: 0 fe_xface, 1 fe_yface, 2 fe_zface_div,
+ * 3 fe_plain_faces, 4 fe_plain_div. */
+int mvd_profile(mvd_context *ctx, int on);
+int mvd_profile_read(mvd_context *ctx, double *ms, long *count, int nslot);
+
 /* ---- primordial chemistry (raja_primordial_network.cpp) ---- */
 
 /* cvklu_setup_data (raja_primordial_network.cpp:45-254): tables_host points to the 50

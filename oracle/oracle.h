@@ -13,6 +13,8 @@ typedef struct {
   double gamma, dx, dy, dz;
 } orc_grid;
 
+void orc_set_threads(int n);
+int orc_max_threads(void);
 int orc_face_flux(const double *stencil, int dir, int nchem, double gamma, double *fface);
 int orc_fe(const orc_grid *g, const double *const *w, double *const *wdot);
 double orc_max_wave_speed(const orc_grid *g, const double *const *w);

@@ -3,6 +3,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "orc_fe.hpp"
 #include "oracle.h"
 
@@ -52,6 +55,22 @@ static void gather(const orc_grid *g, const double *const *w, int d, long i, lon
 
 using namespace orc;
 
+extern "C" void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+extern "C" int orc_max_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 extern "C" int orc_face_flux(const double *stencil, int dir, int nchem, double gamma,
                              double *fface) {
   double s[6][NVMAX];
@@ -60,6 +79,17 @@ extern "C" int orc_face_flux(const double *stencil, int dir, int nchem, double g
     for (int v = 0; v < nv; v++) s[l][v] = stencil[l * nv + v];
   face_flux(s, dir, fface, gamma, nchem);
   return 0;
+}
+
+// The 6 x (5+nchem) stencil the oracle feeds face_flux for the lower face of (i,j,k)
+// along direction dir (row-major [point][field], global momentum order).
+extern "C" void orc_gather(const orc_grid *g, const double *const *w, int dir, long i, long j,
+                           long k, double *out) {
+  double s[6][NVMAX];
+  gather(g, w, dir, i, j, k, s);
+  const int nv = 5 + g->nchem;
+  for (int l = 0; l < 6; l++)
+    for (int v = 0; v < nv; v++) out[l * nv + v] = s[l][v];
 }
 
 // fEuler (utilities.cpp:17-253) with zero external forcing (the forcing used by every
@@ -78,10 +108,13 @@ extern "C" int orc_fe(const orc_grid *g, const double *const *w, double *const *
 
   std::vector<double> xf((size_t)nv * (nx + 1) * ny * nz), yf((size_t)nv * nx * (ny + 1) * nz),
       zf((size_t)nv * nx * ny * (nz + 1));
-  double s[6][NVMAX];
+  // Each face flux is a pure function of its stencil, so the cells may be visited in any
+  // order; OpenMP threads only change the wall time, never the result.
+#pragma omp parallel for collapse(2) schedule(static)
   for (long k = 0; k < nz; k++)
     for (long j = 0; j < ny; j++)
       for (long i = 0; i < nx; i++) {
+        double s[6][NVMAX];
         gather(g, w, 0, i, j, k, s);
         face_flux(s, 0, &xf[(size_t)nv * (i + (nx + 1) * (j + ny * k))], g->gamma, nchem);
         if (i == nx - 1) {
@@ -103,6 +136,7 @@ extern "C" int orc_fe(const orc_grid *g, const double *const *w, double *const *
       }
 
   // divergence (utilities.cpp:198-245): wdot -= (dFx/dx + dFy/dy) + dFz/dz
+#pragma omp parallel for collapse(2) schedule(static)
   for (long k = 0; k < nz; k++)
     for (long j = 0; j < ny; j++)
       for (long i = 0; i < nx; i++) {

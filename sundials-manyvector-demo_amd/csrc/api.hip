@@ -1,6 +1,7 @@
 // C-ABI layer (include/mvd_rhs.h): context lifetime, argument checks, launches.
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <cmath>
 #include "mvd_internal.hpp"
 #include "mvd_chem.hpp"
@@ -58,6 +59,7 @@ int fe_common(mvd_context *ctx, const double *const *w, double *const *wdot, boo
                              hipMemcpyDeviceToHost, ctx->stream)))
     return MVD_DEVICE_ERROR;
   if (!dev_ok(hipStreamSynchronize(ctx->stream))) return MVD_DEVICE_ERROR;
+  ctx->fe.timer.collect();
   return (*ctx->host_flag != 0u) ? MVD_ILLEGAL_STATE : MVD_SUCCESS;
 }
 
@@ -126,6 +128,8 @@ void mvd_destroy(mvd_context *c) {
   (void)hipFree(c->fe.Fz);
   (void)hipFree(c->fe.flag);
   (void)hipFree(c->fe.red);
+  if (c->fe.timer.on)
+    for (int i = 0; i < 2 * mvd::PROF_NSLOT; i++) (void)hipEventDestroy(c->fe.timer.ev[i]);
   if (c->host_flag) (void)hipHostFree(c->host_flag);
   mvd::chem_free(c->chem);
   delete c;
@@ -155,7 +159,36 @@ int mvd_last_state_flags(mvd_context *ctx) {
                              hipMemcpyDeviceToHost, ctx->stream)) ||
       !dev_ok(hipStreamSynchronize(ctx->stream)))
     return MVD_DEVICE_ERROR;
+  ctx->fe.timer.collect();
   return (int)*ctx->host_flag;
+}
+
+int mvd_profile(mvd_context *ctx, int on) {
+  if (!ctx) return MVD_BAD_ARGUMENT;
+  mvd::KernelTimer &t = ctx->fe.timer;
+  if (on && !t.on) {
+    for (int i = 0; i < 2 * mvd::PROF_NSLOT; i++)
+      if (!dev_ok(hipEventCreate(&t.ev[i]))) return MVD_DEVICE_ERROR;
+  }
+  if (!on && t.on) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 2 * mvd::PROF_NSLOT; i++) (void)hipEventDestroy(t.ev[i]);
+  }
+  t.on = on != 0;
+  for (int s = 0; s < mvd::PROF_NSLOT; s++) { t.ms[s] = 0.0; t.count[s] = 0; t.pending[s] = false; }
+  return MVD_SUCCESS;
+}
+
+int mvd_profile_read(mvd_context *ctx, double *ms, long *count, int nslot) {
+  if (!ctx || !ms || !count || nslot <= 0) return MVD_BAD_ARGUMENT;
+  mvd::KernelTimer &t = ctx->fe.timer;
+  if (!dev_ok(hipStreamSynchronize(ctx->stream))) return MVD_DEVICE_ERROR;
+  t.collect();
+  for (int s = 0; s < nslot; s++) {
+    ms[s] = s < mvd::PROF_NSLOT ? t.ms[s] : 0.0;
+    count[s] = s < mvd::PROF_NSLOT ? t.count[s] : 0;
+  }
+  return mvd::PROF_NSLOT;
 }
 
 int mvd_max_wave_speed(mvd_context *ctx, const double *const *w, double *alpha) {

@@ -8,10 +8,11 @@
 //   face_flux_ref  - the reference recipe (utilities.cpp:270-479) operation by operation,
 //                    with FMA contraction disabled; used by the plain comparator kernels
 //                    and reproducible bit for bit against a non-FMA CPU build.
-//   face_flux_fast - the same mathematics rearranged for the FP64 pipes: per-point
-//                    primitive values computed once, 1/rho instead of repeated divides,
-//                    the left eigenvector product factored into 14 operations, and one
-//                    divide per WENO weight set.
+//   face_flux_lds  - the same mathematics rearranged for the FP64 pipes: per-point
+//                    primitive values computed once per point (staged in LDS), 1/rho
+//                    instead of repeated divides, the left eigenvector product factored
+//                    into a handful of FMAs, the WENO weights over one common
+//                    denominator and one divide per component for both split sides.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -62,6 +63,23 @@ __device__ inline GhostMap ghost_map(const Grid &g, int d, int e) {
 
 // SUNRsqrt (sundials_math.h): non-positive arguments give 0.
 __device__ inline double sun_sqrt(double x) { return (x <= 0.0) ? 0.0 : sqrt(x); }
+
+// Reciprocal from the hardware estimate (v_rcp_f64) refined by two Newton steps, and a
+// divide built on it with one residual correction; within 1-2 ulp of the IEEE result at
+// roughly half the instruction count of the correctly rounded divide sequence.
+__device__ inline double fast_rcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ inline double fast_div(double n, double d) {
+  const double r = fast_rcp(d);
+  const double q = n * r;
+  return fma(fma(-d, q, n), r, q);
+}
 
 // ---------------------------------------------------------------------------------
 // Reference-order face flux (no contraction). s[l][v], l = stencil point, v = field in
@@ -187,7 +205,7 @@ __device__ inline Pt make_point(double r, double mn, double m1, double m2, doubl
                                 double gamma) {
   Pt q;
   q.r = r; q.mn = mn; q.m1 = m1; q.m2 = m2; q.e = e;
-  q.ir = 1.0 / r;
+  q.ir = fast_rcp(r);
   q.p = (gamma - 1.0) * (e - 0.5 * (mn * mn + m1 * m1 + m2 * m2) * q.ir);
   const double un = mn * q.ir;
   q.a = fabs(un) + sun_sqrt(gamma * q.p * q.ir);
@@ -195,98 +213,118 @@ __device__ inline Pt make_point(double r, double mn, double m1, double m2, doubl
   return q;
 }
 
-// WENO5-JS reconstruction of one component from the five split-flux values, with the
-// three nonlinear weights d_k/(eps+beta_k)^2 combined over one common denominator:
-//   sum_k q_k d_k/s_k / sum_k d_k/s_k = sum_k q_k d_k t_k / sum_k d_k t_k,  t_k = prod_{m!=k} s_m
-// PLUS selects the f+ stencils/weights (0.3,0.6,0.1), otherwise the f- ones (0.1,0.6,0.3).
+// WENO5-JS numerator/denominator pair for one component, with two algebraic savings:
+//  * beta_k = (13/12) x_k^2 + (1/4) y_k^2 = (13/12) (x_k^2 + (3/13) y_k^2); the common factor
+//    (13/12)^2 of every (eps + beta_k)^2 cancels in the weight ratio, so the weights use
+//    (eps*12/13 + x_k^2 + (3/13) y_k^2)^2;
+//  * the 1/6 of the candidate stencils is applied once to the quotient by the caller.
+// Returns num/den = 6 * (reconstructed value) split as (num, den).
 template <bool PLUS>
-__device__ inline double weno5(double f0, double f1, double f2, double f3, double f4) {
-  const double bc = 13.0 / 12.0;
-  const double eps = 1e-6;
-  const double x1 = f2 - 2.0 * f3 + f4, y1 = 3.0 * f2 - 4.0 * f3 + f4;
-  const double x2 = f1 - 2.0 * f2 + f3, y2 = f1 - f3;
-  const double x3 = f0 - 2.0 * f1 + f2, y3 = f0 - 4.0 * f1 + 3.0 * f2;
-  const double b1 = bc * (x1 * x1) + 0.25 * (y1 * y1);
-  const double b2 = bc * (x2 * x2) + 0.25 * (y2 * y2);
-  const double b3 = bc * (x3 * x3) + 0.25 * (y3 * y3);
-  const double s1 = (eps + b1) * (eps + b1);
-  const double s2 = (eps + b2) * (eps + b2);
-  const double s3 = (eps + b3) * (eps + b3);
+__device__ inline void weno5_nd(double f0, double f1, double f2, double f3, double f4,
+                                double &num, double &den) {
+  const double epsb = 1e-6 * 12.0 / 13.0;
+  const double c = 3.0 / 13.0;
+  const double d01 = f1 - f0, d12 = f2 - f1, d23 = f3 - f2, d34 = f4 - f3;
+  const double x1 = d34 - d23, x2 = d23 - d12, x3 = d12 - d01;
+  const double y1 = x1 - 2.0 * d23;         // 3 f2 - 4 f3 + f4
+  const double y2 = d12 + d23;              // -(f1 - f3)
+  const double y3 = x3 + 2.0 * d12;         // f0 - 4 f1 + 3 f2
+  const double b1 = epsb + fma(x1, x1, (c * y1) * y1);
+  const double b2 = epsb + fma(x2, x2, (c * y2) * y2);
+  const double b3 = epsb + fma(x3, x3, (c * y3) * y3);
+  const double s1 = b1 * b1, s2 = b2 * b2, s3 = b3 * b3;
   double q1, q2, q3, d1, d3;
   if (PLUS) {
-    q1 = (2.0 * f2 + 5.0 * f3 - f4) * (1.0 / 6.0);
-    q2 = (-f1 + 5.0 * f2 + 2.0 * f3) * (1.0 / 6.0);
-    q3 = (2.0 * f0 - 7.0 * f1 + 11.0 * f2) * (1.0 / 6.0);
+    q1 = 2.0 * f2 + 5.0 * f3 - f4;
+    q2 = -f1 + 5.0 * f2 + 2.0 * f3;
+    q3 = 2.0 * f0 - 7.0 * f1 + 11.0 * f2;
     d1 = 0.3; d3 = 0.1;
   } else {
-    q1 = (11.0 * f2 - 7.0 * f3 + 2.0 * f4) * (1.0 / 6.0);
-    q2 = (2.0 * f1 + 5.0 * f2 - f3) * (1.0 / 6.0);
-    q3 = (-f0 + 5.0 * f1 + 2.0 * f2) * (1.0 / 6.0);
+    q1 = 11.0 * f2 - 7.0 * f3 + 2.0 * f4;
+    q2 = 2.0 * f1 + 5.0 * f2 - f3;
+    q3 = -f0 + 5.0 * f1 + 2.0 * f2;
     d1 = 0.1; d3 = 0.3;
   }
   const double t1 = d1 * (s2 * s3), t2 = 0.6 * (s1 * s3), t3 = d3 * (s1 * s2);
-  return (q1 * t1 + q2 * t2 + q3 * t3) / (t1 + t2 + t3);
+  num = q1 * t1 + q2 * t2 + q3 * t3;
+  den = t1 + t2 + t3;
 }
 
-// Face flux from the 6 stencil points P[0..5] (points i-3..i+2 of face i-1/2).
-// Output F[0..4] in direction-local order. Also returns alpha (LF speed) and the
-// point normal velocities, which the species part reuses.
-__device__ inline void face_flux_fast(const Pt (&P)[6], double gamma, double (&F)[5],
-                                      double &alpha_out, double (&un)[6]) {
+// Sum of the f+ and f- reconstructions of one component with a single divide:
+//   n+/(6 d+) + n-/(6 d-) = (n+ d- + n- d+) / (6 d+ d-)
+// (d+ d- is a product of four (eps+beta)^2 factors: finite for |beta| < 1e38).
+__device__ inline double weno5_pair(const double (&fp)[5], const double (&fm)[5]) {
+  double np, dp, nm, dm;
+  weno5_nd<true>(fp[0], fp[1], fp[2], fp[3], fp[4], np, dp);
+  weno5_nd<false>(fm[0], fm[1], fm[2], fm[3], fm[4], nm, dm);
+  return fast_div(fma(np, dm, nm * dp), 6.0 * (dp * dm));
+}
+
+// Face flux with the 6-point stencil read from LDS as it is needed (point l, field q at
+// base[q*FS + l*LS], fields in Pt order r, mn, m1, m2, e, p, ir, a, sr), one split side at a
+// time, so only one side's 25 projected values are live.
+__device__ inline void face_flux_lds(const double *base, int FS, int LS, double gamma,
+                                     double (&F)[5], double &alpha_out) {
   const double gm = gamma - 1.0;
-  // Roe average (utilities.cpp:298-304 rewritten with m/sqrt(rho) = u*sqrt(rho))
-  const Pt &L = P[2], &R = P[3];
-  const double iden = 1.0 / (L.sr + R.sr);
-  const double wl = L.sr * L.ir, wr = R.sr * R.ir;   // = 1/sqrt(rho)
-  const double u = (L.mn * wl + R.mn * wr) * iden;
-  const double v = (L.m1 * wl + R.m1 * wr) * iden;
-  const double w = (L.m2 * wl + R.m2 * wr) * iden;
-  const double H = ((L.p + L.e) * wl + (R.p + R.e) * wr) * iden;
-  const double qsq = u * u + v * v + w * w;
-  const double hq = 0.5 * qsq;
-  const double cs = gm * (H - hq);       // (gamma-1)(H - q^2/2), as in the reference
-  const double ci = 1.0 / cs;
-  // LV applied to a vector x, with the 1/2 of the flux split folded in:
-  //   K  = (gamma-1)(q^2/2 x0 - u x1 - v x2 - w x3 + x4)
-  //   A  = u x0 - x1
-  //   rows: 0: ci/2 (A+K)   1: x2 - v x0   2: x3 - w x0   3: x0 - ci K   4: ci/2 (K-A)
-  const double hci = 0.25 * ci;          // (1/2 split) * (1/2 row factor) * ci
   double alpha = 0.0;
 #pragma unroll
-  for (int l = 0; l < 6; l++) {
-    un[l] = P[l].mn * P[l].ir;
-    alpha = fmax(alpha, P[l].a);
+  for (int l = 0; l < 6; l++) alpha = fmax(alpha, base[7 * FS + l * LS]);
+  double u, v, w, H;
+  {
+    const double *L = base + 2 * LS, *R = base + 3 * LS;
+    const double Lsr = L[8 * FS], Rsr = R[8 * FS];
+    const double iden = fast_rcp(Lsr + Rsr);
+    const double wl = Lsr * L[6 * FS], wr = Rsr * R[6 * FS];
+    u = (L[1 * FS] * wl + R[1 * FS] * wr) * iden;
+    v = (L[2 * FS] * wl + R[2 * FS] * wr) * iden;
+    w = (L[3 * FS] * wl + R[3 * FS] * wr) * iden;
+    H = ((L[5 * FS] + L[4 * FS]) * wl + (R[5 * FS] + R[4 * FS]) * wr) * iden;
   }
-  double ff[5];
+  const double qsq = u * u + v * v + w * w;
+  const double hq = 0.5 * qsq;
+  const double cs = gm * (H - hq);
+  const double ci = fast_rcp(cs);
+  const double hci = 0.25 * ci, hci2 = 0.5 * ci;
+  // K = (gamma-1)(q^2/2 x0 - u x1 - v x2 - w x3 + x4) with the factor folded in
+  const double gh = gm * hq, gu = gm * u, gv = gm * v, gw = gm * w;
+  double nps[5], dps[5], ff[5];
 #pragma unroll
   for (int side = 0; side < 2; side++) {
+    __builtin_amdgcn_sched_barrier(0);
     double pr[5][5];
 #pragma unroll
     for (int l = 0; l < 5; l++) {
-      const Pt &q = P[l + side];
-      const double ul = un[l + side];
+      const double *q = base + (l + side) * LS;
+      const double r = q[0], mn = q[FS], m1 = q[2 * FS], m2 = q[3 * FS], e = q[4 * FS];
+      const double p = q[5 * FS];
+      const double ul = mn * q[6 * FS];
       const double sa = side == 0 ? alpha : -alpha;
-      const double x0 = q.mn + sa * q.r;
-      const double x1 = ul * q.mn + q.p + sa * q.mn;
-      const double x2 = ul * q.m1 + sa * q.m1;
-      const double x3 = ul * q.m2 + sa * q.m2;
-      const double x4 = ul * (q.e + q.p) + sa * q.e;
-      const double K = gm * (hq * x0 - u * x1 - v * x2 - w * x3 + x4);
-      const double A = u * x0 - x1;
+      const double us = ul + sa;
+      const double x0 = fma(sa, r, mn);
+      const double x1 = fma(us, mn, p);
+      const double x2 = us * m1;
+      const double x3 = us * m2;
+      const double x4 = fma(us, e, ul * p);
+      const double K = fma(gm, x4, fma(-gw, x3, fma(-gv, x2, fma(-gu, x1, gh * x0))));
+      const double A = fma(u, x0, -x1);
       pr[l][0] = hci * (A + K);
-      pr[l][1] = 0.5 * (x2 - v * x0);
-      pr[l][2] = 0.5 * (x3 - w * x0);
-      pr[l][3] = 0.5 * x0 - (0.5 * ci) * K;
+      pr[l][1] = 0.5 * fma(-v, x0, x2);
+      pr[l][2] = 0.5 * fma(-w, x0, x3);
+      pr[l][3] = fma(-hci2, K, 0.5 * x0);
       pr[l][4] = hci * (K - A);
     }
 #pragma unroll
     for (int c = 0; c < 5; c++) {
-      const double r = side == 0 ? weno5<true>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c])
-                                 : weno5<false>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c]);
-      ff[c] = side == 0 ? r : ff[c] + r;
+      if (side == 0) {
+        weno5_nd<true>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c], nps[c], dps[c]);
+      } else {
+        double nm, dm;
+        weno5_nd<false>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c], nm, dm);
+        ff[c] = fast_div(fma(nps[c], dm, nm * dps[c]), 6.0 * (dps[c] * dm));
+      }
     }
   }
-  // RV applied to ff (utilities.cpp:470-472), factored
+  __builtin_amdgcn_sched_barrier(0);
   const double s04 = ff[0] + ff[4];
   const double d40 = cs * (ff[4] - ff[0]);
   F[0] = s04 + ff[3];
@@ -297,8 +335,15 @@ __device__ inline void face_flux_fast(const Pt (&P)[6], double gamma, double (&F
   alpha_out = alpha;
 }
 
+// Normal velocities of the 6 LDS stencil points (for the species part).
+__device__ inline void lds_velocities(const double *base, int FS, int LS, double (&un)[6]) {
+#pragma unroll
+  for (int l = 0; l < 6; l++) un[l] = base[FS + l * LS] * base[6 * FS + l * LS];
+}
+
 // Species (passive tracer) face flux for one component from its 6 point values c[l],
-// the point normal velocities un[l] and the fluid alpha (utilities.cpp:376-377,395,439,473).
+// the point normal velocities un[l] and the fluid alpha (utilities.cpp:376-377,395,439,473):
+// component-wise WENO5 of the split fluxes, no characteristic projection.
 __device__ inline double tracer_face_flux(const double (&c)[6], const double (&un)[6],
                                           double alpha) {
   double fp[5], fm[5];
@@ -307,8 +352,7 @@ __device__ inline double tracer_face_flux(const double (&c)[6], const double (&u
     fp[l] = 0.5 * (un[l] * c[l] + alpha * c[l]);
     fm[l] = 0.5 * (un[l + 1] * c[l + 1] - alpha * c[l + 1]);
   }
-  return weno5<true>(fp[0], fp[1], fp[2], fp[3], fp[4]) +
-         weno5<false>(fm[0], fm[1], fm[2], fm[3], fm[4]);
+  return weno5_pair(fp, fm);
 }
 
 }  // namespace mvd

@@ -15,6 +15,36 @@ struct FieldsOut {
   double *chem;
 };
 
+// Kernel timing slots (HIP events recorded on the launch stream around each kernel).
+enum { PROF_XFACE = 0, PROF_YFACE, PROF_ZFACE_DIV, PROF_PLAIN_FACES, PROF_PLAIN_DIV, PROF_NSLOT };
+
+struct KernelTimer {
+  bool on = false;
+  hipEvent_t ev[2 * PROF_NSLOT] = {};
+  bool pending[PROF_NSLOT] = {};
+  double ms[PROF_NSLOT] = {};
+  long count[PROF_NSLOT] = {};
+  void begin(int slot, hipStream_t st) {
+    if (on) (void)hipEventRecord(ev[2 * slot], st);
+  }
+  void end(int slot, hipStream_t st) {
+    if (on) { (void)hipEventRecord(ev[2 * slot + 1], st); pending[slot] = true; }
+  }
+  // after the stream has been synchronized
+  void collect() {
+    if (!on) return;
+    for (int s = 0; s < PROF_NSLOT; s++) {
+      if (!pending[s]) continue;
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, ev[2 * s], ev[2 * s + 1]) == hipSuccess) {
+        ms[s] += t;
+        count[s] += 1;
+      }
+      pending[s] = false;
+    }
+  }
+};
+
 // Device scratch owned by the context (EulerData keeps xflux/yflux/zflux for the same
 // purpose, euler3D.hpp:496-499).
 struct FeScratch {
@@ -22,6 +52,7 @@ struct FeScratch {
   unsigned *flag = nullptr;                             // legal-state bits of the last call
   double *red = nullptr;                                // reduction partials
   long red_len = 0;
+  KernelTimer timer;
 };
 
 int fe_launch(const Grid &g, const Fields &f, const FieldsOut &o, FeScratch &s,

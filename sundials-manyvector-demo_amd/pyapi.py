@@ -54,6 +54,8 @@ def lib():
         "mvd_max_wave_speed": ([vp, _PP, _P], C.c_int),
         "mvd_stability": ([vp, _PP, C.c_double, _P], C.c_int),
         "mvd_fslow": ([vp, C.c_double, _PP, _PP], C.c_int),
+        "mvd_profile": ([vp, C.c_int], C.c_int),
+        "mvd_profile_read": ([vp, _P, C.POINTER(C.c_long), C.c_int], C.c_int),
         "mvd_chem_setup": ([vp, _P, C.c_long, C.c_double], C.c_int),
         "mvd_chem_prepare": ([vp, _P], C.c_int),
         "mvd_chem_apply_scaling": ([vp, _P], C.c_int),
@@ -154,6 +156,20 @@ class Context:
 
     def fslow(self, w, wdot, t=0.0):
         return _check("mvd_fslow", lib().mvd_fslow(self.h, t, dptrs(w), dptrs(wdot)))
+
+    PROFILE_SLOTS = ("fe_xface", "fe_yface", "fe_zface_div", "fe_plain_faces", "fe_plain_div")
+
+    def profile(self, on=True):
+        _check("mvd_profile", lib().mvd_profile(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        n = len(self.PROFILE_SLOTS)
+        ms = (C.c_double * n)()
+        cnt = (C.c_long * n)()
+        rc = lib().mvd_profile_read(self.h, ms, cnt, n)
+        if rc < 0:
+            raise MvdError("mvd_profile_read", rc)
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(self.PROFILE_SLOTS)}
 
     # chemistry ---------------------------------------------------------------------------
     def chem_setup(self, tables, ncell, current_z=-1.0):

@@ -42,9 +42,12 @@ def lib():
         L = C.CDLL(LIB_PATH)
         P = C.POINTER(C.c_double)
         PP = C.POINTER(P)
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_max_threads.restype = C.c_int
         L.orc_fe.argtypes = [C.POINTER(Grid), PP, PP]
         L.orc_fe.restype = C.c_int
         L.orc_face_flux.argtypes = [P, C.c_int, C.c_int, C.c_double, P]
+        L.orc_gather.argtypes = [C.POINTER(Grid), PP, C.c_int, C.c_long, C.c_long, C.c_long, P]
         L.orc_max_wave_speed.argtypes = [C.POINTER(Grid), PP]
         L.orc_max_wave_speed.restype = C.c_double
         L.orc_stability_dt.argtypes = [C.POINTER(Grid), PP, C.c_double]
@@ -87,11 +90,26 @@ def new_state(g, nchem=None):
     return [np.zeros(n) for _ in range(5)] + [np.zeros(max(nc, 1) * n)]
 
 
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return lib().orc_max_threads()
+
+
 def fe(g, w):
     """fEuler on the CPU oracle; w = [rho, mx, my, mz, et, chem]. Returns (flag, wdot)."""
     wdot = new_state(g)
     flag = lib().orc_fe(C.byref(g), _pp(w), _pp(wdot))
     return flag, wdot
+
+
+def gather(g, w, direction, i, j, k):
+    nv = 5 + g.nchem
+    out = np.zeros(6 * nv)
+    lib().orc_gather(C.byref(g), _pp(w), direction, i, j, k, _p(out))
+    return out.reshape(6, nv)
 
 
 def face_flux(stencil, direction, gamma=1.4):
