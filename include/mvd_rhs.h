@@ -107,6 +107,44 @@ int mvd_chem_jac(mvd_context *ctx, double t, const double *y, double *J);
  * 4 mdensity, 5 inv_mdensity, 6 cie_optical_depth, 7 h2_optical_depth */
 double *mvd_chem_state(mvd_context *ctx, int which);
 
+/* ---- implicit chemistry step (the fast stage; replaces ARKStep DIRK + SUNDIALS Newton +
+ * SUNLinSol_MagmaDense/KLU, primordial_ode_main.cpp:422-536, and the block-diagonal
+ * SUNLinSol_RankLocalLS, multirate_chem_hydro_main.cpp:1312-1411) ---- */
+
+/* SDIRK table (equal diagonal entries, stiffly accurate) and Newton controls */
+typedef struct {
+  int nstage;              /* 1..4 */
+  double A[16];            /* row-major 4x4 Butcher matrix */
+  double c[4];
+  double rtol, atol;       /* WRMS weights 1/(rtol|z|+atol) of the Newton correction */
+  double newton_tol;       /* converged when the WRMS norm of the correction <= newton_tol */
+  int max_iters;           /* Newton iterations before the Jacobian is re-evaluated (1..100) */
+  int max_setups;          /* Jacobian re-evaluations per stage before a cell fails (0..20) */
+  double rhs_scale;        /* 1 for frhs, TimeUnits for ffast/fimpl */
+} mvd_dirk_opts;
+
+typedef struct {
+  long newton_iters;       /* over all cells */
+  int max_cell_iters;      /* largest per-cell Newton count */
+  long failed_cells;       /* cells that did not converge (recoverable failure) */
+  long setups;             /* Jacobian evaluations + LU factorisations */
+  long hist[16];           /* cells by Newton count (last bin: >= 15) */
+  double ms;               /* device time of the step */
+} mvd_dirk_stats;
+
+/* Alexander's 3-stage, 3rd-order, L-stable SDIRK and default Newton controls */
+int mvd_dirk_sdirk3(mvd_dirk_opts *o);
+/* One step of size h of every cell, y (ncell x 10, scaled, device) overwritten with
+ * y_{n+1}. plain = 0: the batched persistent kernel; plain = 1: the reference-structured
+ * multi-kernel path. Returns 0, or the number of failed cells (> 0, recoverable). */
+int mvd_chem_dirk_step(mvd_context *ctx, double t, double h, const mvd_dirk_opts *o, double *y,
+                       int plain, mvd_dirk_stats *stats);
+/* Autotune the batched kernel's launch shape (threads per block, resident waves per CU) by
+ * timing one step on a scratch copy of y; the per-cell temperatures are restored. The
+ * choice is kept in the context; returns the best time in ms through best_ms. */
+int mvd_chem_dirk_tune(mvd_context *ctx, double h, const mvd_dirk_opts *o, const double *y,
+                       double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,20 @@ typedef struct {
   double gamma, dx, dy, dz;
 } orc_grid;
 
+/* one DIRK step of the chemistry (orc_dirk.cpp) */
+typedef struct {
+  int nstage;
+  double A[16];          /* 4x4 row-major Butcher matrix (lower triangular) */
+  double c[4];
+  double rtol, atol, newton_tol;
+  int max_iters, max_setups;
+  double rhs_scale;      /* 1 (frhs) or TimeUnits (ffast / fimpl) */
+} orc_dirk;
+int orc_chem_dirk_step(void *chem, const orc_dirk *opt, double h, double *y, int *iters);
+int orc_lu(const double *M, double *LU, int *piv, const double *b, double *x);
+/* bitwise check of the early exit of the device temperature solve (orc_chem.cpp) */
+long orc_temperature_exit_check(void *chem, const double *y, const double *Ts0, int *steps);
+
 void orc_set_threads(int n);
 int orc_max_threads(void);
 int orc_face_flux(const double *stencil, int dir, int nchem, double gamma, double *fface);

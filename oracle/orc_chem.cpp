@@ -7,16 +7,7 @@
 
 using namespace orc;
 
-namespace {
-struct Chem {
-  long n;
-  std::vector<double> tab;
-  TableGrid g;
-  double z;
-  std::vector<double> scale, inv_scale, Ts, dTs_ge, mdensity, inv_mdensity, cie_oda, h2_oda,
-      rs, drs;
-};
-}  // namespace
+using Chem = orc::ChemState;
 
 // cvklu_setup_data (raja_primordial_network.cpp:45-254) with the tables given as the
 // committed blob: Ts = 1000 K, bounds [1,1e5], nbins = 1023,
@@ -214,4 +205,48 @@ extern "C" void orc_chem_opcount(void *h, unsigned long long *out) {
   jac_physical(y, rs, drs, Ts, Cnt(c->z), dTs, aux_of(c, 0), J, nzm);
   out[6] = OpCounter::add; out[7] = OpCounter::mul; out[8] = OpCounter::div;
   out[9] = OpCounter::sqrt_; out[10] = OpCounter::cmp; out[11] = OpCounter::trans;
+}
+
+// The early exit of the device temperature solve, restated: the step map depends only on its
+// input, so a fixed point (x' == x) or a 2-cycle (x' == previous input) ends the ten steps
+// with the values they would have produced (parity of the step picks the member of the
+// cycle). Runs both forms from the warm starts Ts0 for every cell of y (scaled) and returns
+// the number of cells whose (T, dT/dge) differ bitwise; steps[i] gets the steps the early
+// form took (may be null).
+extern "C" long orc_temperature_exit_check(void *h, const double *y, const double *Ts0,
+                                           int *steps) {
+  Chem *c = (Chem *)h;
+  long bad = 0;
+  for (long i = 0; i < c->n; i++) {
+    double ya[NSP];
+    for (int s = 0; s < NSP; s++) ya[s] = y[NSP * i + s] * c->scale[NSP * i + s];
+    double T10 = Ts0[i], d10 = 0.0;
+    temperature(c->g, ya, T10, d10);
+    const double density = temperature_density(ya);
+    double Ts = Ts0[i], Tnew = Ts * 1.1, dge_dT = 0.0, x_prev = 0.0, d_prev = 0.0;
+    int it = 0;
+    for (; it < 10; it++) {
+      const double T = Ts;
+      Tnew = temperature_step(c->g, ya, density, T, dge_dT);
+      Ts = Tnew;
+      if (std::memcmp(&Tnew, &T, sizeof(double)) == 0) break;
+      if (it > 0 && std::memcmp(&Tnew, &x_prev, sizeof(double)) == 0) {
+        if ((it & 1) == 0) {
+          Tnew = T;
+          dge_dT = d_prev;
+        }
+        break;
+      }
+      x_prev = T;
+      d_prev = dge_dT;
+    }
+    Ts = Tnew;
+    if (Ts < c->g.tmin) Ts = c->g.tmin;
+    else if (Ts > c->g.tmax) Ts = c->g.tmax;
+    const double dTs = 1.0 / dge_dT;
+    if (std::memcmp(&Ts, &T10, sizeof(double)) != 0 || std::memcmp(&dTs, &d10, sizeof(double)) != 0)
+      bad++;
+    if (steps) steps[i] = it < 10 ? it + 1 : 10;
+  }
+  return bad;
 }

@@ -5,6 +5,7 @@
 // (src/raja_primordial_network.cpp) so that the oracle reproduces its arithmetic.
 // pow(x,2) is written x*x: GCC folds pow(x,2.0) to x*x in the reference build.
 #pragma once
+#include <vector>
 #include "orc_common.hpp"
 
 namespace orc {
@@ -46,43 +47,56 @@ inline int table_bin(const TableGrid &g, R logT) {
 
 // cvklu_calculate_temperature (raja_primordial_network.cpp:908-1011): exactly 10 Newton
 // iterations warm-started from Ts, then clamped to [1,1e5]; dTs_ge = 1/dge_dT.
+// One Newton step of the temperature solve from the input T: returns the next iterate and
+// dge_dT at T (raja_primordial_network.cpp:943-990).
 template <class R>
-void temperature(const TableGrid &g, const R (&y)[NSP], R &Ts, R &dTs_ge) {
+R temperature_step(const TableGrid &g, const R (&y)[NSP], R density, R T, R &dge_dT) {
   const R kb = R(1.3806504e-16), mh = R(1.67e-24);
   const R gamma = R(5.e0 / 3.e0);
   const R gm1 = R(1.0) / (gamma - R(1));
   const R H2_1 = y[0], H2_2 = y[1], H_1 = y[2], H_2 = y[3], H_m0 = y[4], He_1 = y[5],
           He_2 = y[6], He_3 = y[7], de = y[8], ge = y[9];
-  const R density = R(2.0) * H2_1 + R(2.0) * H2_2 + R(1.0079400000000001) * H_1
-                  + R(1.0079400000000001) * H_2 + R(1.0079400000000001) * H_m0
-                  + R(4.0026020000000004) * He_1 + R(4.0026020000000004) * He_2
-                  + R(4.0026020000000004) * He_3;
-  R T = Ts;
-  R Tnew = T * R(1.1);
-  R dge_dT = R(0);
   const double *t = g.tab;
+  const int b = table_bin(g, log(T));
+  const R t1 = R(g.lb) + R(double(b)) * R(g.dbin);
+  const R t2 = R(g.lb) + R(double(b + 1)) * R(g.dbin);
+  const R Tdef = (log(T) - t1) / (t2 - t1);
+  auto lin = [&](int tab) {
+    const double *a = t + tab * TLEN;
+    return R(a[b]) + Tdef * (R(a[b + 1]) - R(a[b]));
+  };
+  const R gH2_2 = lin(G_GH2_2);
+  const R dgH2_2 = lin(G_DGH2_2);
+  const R gH2_1 = lin(G_GH2_1);
+  const R dgH2_1 = lin(G_DGH2_1);
+  const R g1 = R(1.0) / (gH2_1 - R(1.0));
+  const R g2 = R(1.0) / (gH2_2 - R(1.0));
+  const R nsum = H2_1 * g1 + H2_2 * g2 + H_1 * gm1 + H_2 * gm1 + H_m0 * gm1 + He_1 * gm1
+               + He_2 * gm1 + He_3 * gm1 + gm1 * de;
+  dge_dT = T * kb * ((-H2_1) * g1 * g1 * dgH2_1 - H2_2 * g2 * g2 * dgH2_2) / (density * mh)
+         + kb * nsum / (density * mh);
+  const R dge = T * kb * nsum / (density * mh) - ge;
+  return T - dge / dge_dT;
+}
+
+template <class R>
+R temperature_density(const R (&y)[NSP]) {
+  return R(2.0) * y[0] + R(2.0) * y[1] + R(1.0079400000000001) * y[2]
+       + R(1.0079400000000001) * y[3] + R(1.0079400000000001) * y[4]
+       + R(4.0026020000000004) * y[5] + R(4.0026020000000004) * y[6]
+       + R(4.0026020000000004) * y[7];
+}
+
+// cvklu_calculate_temperature (raja_primordial_network.cpp:908-1011): ten Newton steps from
+// the warm start Ts, clamp to [tmin, tmax], dTs_ge = 1/dge_dT of the last step.
+template <class R>
+void temperature(const TableGrid &g, const R (&y)[NSP], R &Ts, R &dTs_ge) {
+  const R density = temperature_density(y);
+  R Tnew = Ts * R(1.1);
+  R dge_dT = R(0);
   for (int it = 0; it < 10; it++) {
-    T = Ts;
-    const int b = table_bin(g, log(Ts));
-    const R t1 = R(g.lb) + R(double(b)) * R(g.dbin);
-    const R t2 = R(g.lb) + R(double(b + 1)) * R(g.dbin);
-    const R Tdef = (log(Ts) - t1) / (t2 - t1);
-    auto lin = [&](int tab) {
-      const double *a = t + tab * TLEN;
-      return R(a[b]) + Tdef * (R(a[b + 1]) - R(a[b]));
-    };
-    const R gH2_2 = lin(G_GH2_2);
-    const R dgH2_2 = lin(G_DGH2_2);
-    const R gH2_1 = lin(G_GH2_1);
-    const R dgH2_1 = lin(G_DGH2_1);
-    const R g1 = R(1.0) / (gH2_1 - R(1.0));
-    const R g2 = R(1.0) / (gH2_2 - R(1.0));
-    const R nsum = H2_1 * g1 + H2_2 * g2 + H_1 * gm1 + H_2 * gm1 + H_m0 * gm1 + He_1 * gm1
-                 + He_2 * gm1 + He_3 * gm1 + gm1 * de;
-    dge_dT = T * kb * ((-H2_1) * g1 * g1 * dgH2_1 - H2_2 * g2 * g2 * dgH2_2) / (density * mh)
-           + kb * nsum / (density * mh);
-    const R dge = T * kb * nsum / (density * mh) - ge;
-    Tnew = T - dge / dge_dT;
+    const R T = Ts;
+    Tnew = temperature_step(g, y, density, T, dge_dT);
     Ts = Tnew;
   }
   Ts = Tnew;
@@ -111,6 +125,17 @@ void interp_rates(const TableGrid &g, R T, R (&rs)[NRATE], R (&drs)[NRATE]) {
 
 struct CellAux {      // per-cell constants from setting_up_extra_variables
   double mdensity, inv_mdensity, cie_oda, h2_oda;
+};
+
+// State of one cvklu_data for n cells (raja_primordial_network.hpp:104-276): tables,
+// scaling, temperature warm start and the per-cell rate cache.
+struct ChemState {
+  long n;
+  std::vector<double> tab;
+  TableGrid g;
+  double z;
+  std::vector<double> scale, inv_scale, Ts, dTs_ge, mdensity, inv_mdensity, cie_oda, h2_oda,
+      rs, drs;
 };
 
 // Species and energy RHS in physical units (before inv_scale), raja_primordial_network.cpp:1302-1371.

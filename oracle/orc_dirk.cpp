@@ -1,0 +1,191 @@
+// Oracle: one DIRK step of the per-cell primordial chemistry -- TEST INFRASTRUCTURE ONLY
+// (see orc_common.hpp).
+//
+// The reference advances the chemistry with ARKStep's DIRK stages, SUNDIALS' Newton solver
+// and a block-diagonal direct linear solver (primordial_ode_main.cpp:422-536,
+// multirate_chem_hydro_main.cpp:898-994,1312-1411). SUNDIALS is not available here, so
+// this restates the stage solve the GPU solver implements, cell by cell:
+//   * y' = s_rhs * f(y), f = calculate_rhs_cvklu in scaled variables (s_rhs = 1 is frhs,
+//     s_rhs = TimeUnits is ffast/fimpl without forcing);
+//   * stage i solves z - h a_ii f(z) = r_i,  r_i = y_n + h sum_{j<i} a_ij F_j;
+//   * Newton with M = I - h a_ii s_rhs J, J = calculate_jacobian_cvklu at y_n (after f(y_n)
+//     refreshed the temperature), factored once per step by 10x10 LU with partial pivoting
+//     and reused (modified Newton); if a stage reaches max_iters, J is re-evaluated at the
+//     current iterate, refactored and the stage restarted from it (up to max_setups times);
+//   * predictor: the previous stage value (y_n for the first stage);
+//   * convergence: WRMS norm of the correction, weights 1/(rtol|z|+atol), <= newton_tol;
+//   * F_i = (z_i - r_i)/(h a_ii) (ARKODE's implicit-stage derivative), y_{n+1} = z_s.
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "orc_chem.hpp"
+#include "oracle.h"
+
+using namespace orc;
+
+namespace {
+
+void lu_factor(double (&a)[NSP][NSP], int (&piv)[NSP]) {
+  for (int k = 0; k < NSP; k++) {
+    int p = k;
+    double best = std::fabs(a[k][k]);
+    for (int r = k + 1; r < NSP; r++)
+      if (std::fabs(a[r][k]) > best) { best = std::fabs(a[r][k]); p = r; }
+    piv[k] = p;
+    if (p != k)
+      for (int c = 0; c < NSP; c++) std::swap(a[k][c], a[p][c]);
+    const double ik = 1.0 / a[k][k];
+    for (int r = k + 1; r < NSP; r++) {
+      const double l = a[r][k] * ik;
+      a[r][k] = l;
+      for (int c = k + 1; c < NSP; c++) a[r][c] -= l * a[k][c];
+    }
+  }
+}
+
+void lu_solve(const double (&a)[NSP][NSP], const int (&piv)[NSP], double (&b)[NSP]) {
+  for (int k = 0; k < NSP; k++)
+    if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+  for (int r = 1; r < NSP; r++)
+    for (int c = 0; c < r; c++) b[r] -= a[r][c] * b[c];
+  for (int r = NSP - 1; r >= 0; r--) {
+    for (int c = r + 1; c < NSP; c++) b[r] -= a[r][c] * b[c];
+    b[r] /= a[r][r];
+  }
+}
+
+struct Cell {
+  const ChemState *c;
+  long i;
+  double Ts, dTs;
+  double sc[NSP], isc[NSP];
+  CellAux ax;
+};
+
+// calculate_rhs_cvklu for one cell (scaled), updating the cell's temperature
+void cell_rhs(Cell &cl, const double (&y)[NSP], double srhs, double (&f)[NSP]) {
+  double ya[NSP], rs[NRATE], drs[NRATE];
+  for (int s = 0; s < NSP; s++) ya[s] = y[s] * cl.sc[s];
+  temperature(cl.c->g, ya, cl.Ts, cl.dTs);
+  interp_rates(cl.c->g, cl.Ts, rs, drs);
+  double fp[NSP];
+  rhs_physical(ya, rs, cl.Ts, cl.c->z, cl.ax, fp);
+  for (int s = 0; s < NSP; s++) {
+    f[s] = fp[s];
+    f[s] *= cl.isc[s];
+  }
+  f[9] *= cl.ax.inv_mdensity;
+  for (int s = 0; s < NSP; s++) f[s] *= srhs;
+}
+
+// calculate_jacobian_cvklu for one cell from the current temperature; M = I - hg*srhs*J
+void cell_matrix(const Cell &cl, const double (&y)[NSP], double hg, double srhs,
+                 double (&M)[NSP][NSP]) {
+  double ya[NSP], rs[NRATE], drs[NRATE];
+  for (int s = 0; s < NSP; s++) ya[s] = y[s] * cl.sc[s];
+  interp_rates(cl.c->g, cl.Ts, rs, drs);
+  double J[NSP][NSP];
+  bool nzm[NSP][NSP];
+  jac_physical(ya, rs, drs, cl.Ts, cl.c->z, cl.dTs, cl.ax, J, nzm);
+  for (int r = 0; r < NSP; r++)
+    for (int c = 0; c < NSP; c++) {
+      double v = 0.0;
+      if (nzm[r][c]) {
+        v = J[r][c];
+        v *= (cl.isc[r] * cl.sc[c]);
+      }
+      M[r][c] = (r == c ? 1.0 : 0.0) - hg * (srhs * v);
+    }
+}
+
+double wrms(const double (&d)[NSP], const double (&z)[NSP], double rtol, double atol) {
+  double s = 0.0;
+  for (int k = 0; k < NSP; k++) {
+    const double w = d[k] / (rtol * std::fabs(z[k]) + atol);
+    s += w * w;
+  }
+  return std::sqrt(s / NSP);
+}
+
+}  // namespace
+
+extern "C" int orc_lu(const double *Min, double *LUout, int *piv, const double *b, double *x) {
+  double a[NSP][NSP];
+  int p[NSP];
+  for (int r = 0; r < NSP; r++)
+    for (int c = 0; c < NSP; c++) a[r][c] = Min[r * NSP + c];
+  lu_factor(a, p);
+  double bb[NSP];
+  for (int r = 0; r < NSP; r++) bb[r] = b[r];
+  lu_solve(a, p, bb);
+  for (int r = 0; r < NSP; r++) {
+    x[r] = bb[r];
+    piv[r] = p[r];
+    for (int c = 0; c < NSP; c++) LUout[r * NSP + c] = a[r][c];
+  }
+  return 0;
+}
+
+// One DIRK step of every cell; y (n x 10, scaled) is overwritten with y_{n+1}; Ts is left
+// at the temperature of the last RHS evaluation of each cell. iters[i] receives the total
+// Newton iterations of cell i (may be null). Returns the number of failed cells.
+extern "C" int orc_chem_dirk_step(void *h, const orc_dirk *opt, double hstep, double *y,
+                                  int *iters) {
+  ChemState *c = (ChemState *)h;
+  const int ns = opt->nstage;
+  int failed = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : failed)
+  for (long i = 0; i < c->n; i++) {
+    Cell cl{c, i, c->Ts[i], c->dTs_ge[i], {}, {}, {c->mdensity[i], c->inv_mdensity[i],
+                                                  c->cie_oda[i], c->h2_oda[i]}};
+    for (int s = 0; s < NSP; s++) {
+      cl.sc[s] = c->scale[NSP * i + s];
+      cl.isc[s] = c->inv_scale[NSP * i + s];
+    }
+    double yn[NSP], z[NSP], F[4][NSP], f[NSP];
+    for (int s = 0; s < NSP; s++) yn[s] = z[s] = y[NSP * i + s];
+    cell_rhs(cl, yn, opt->rhs_scale, f);               // refresh Ts at y_n for J
+    double M[NSP][NSP];
+    int piv[NSP];
+    bool ok_cell = true;
+    int nit = 0;
+    for (int st = 0; st < ns && ok_cell; st++) {
+      const double hg = hstep * opt->A[st * 4 + st];
+      if (st == 0) {
+        cell_matrix(cl, yn, hg, opt->rhs_scale, M);
+        lu_factor(M, piv);
+      }
+      double r[NSP];
+      for (int s = 0; s < NSP; s++) {
+        double acc = 0.0;
+        for (int j = 0; j < st; j++) acc += opt->A[st * 4 + j] * F[j][s];
+        r[s] = yn[s] + hstep * acc;
+      }
+      int setups = 0;
+      bool conv = false;
+      while (!conv) {
+        for (int it = 0; it < opt->max_iters; it++) {
+          cell_rhs(cl, z, opt->rhs_scale, f);
+          double d[NSP];
+          for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
+          lu_solve(M, piv, d);
+          for (int s = 0; s < NSP; s++) z[s] += d[s];
+          nit++;
+          if (wrms(d, z, opt->rtol, opt->atol) <= opt->newton_tol) { conv = true; break; }
+        }
+        if (conv) break;
+        if (++setups > opt->max_setups) { ok_cell = false; break; }
+        cell_rhs(cl, z, opt->rhs_scale, f);             // temperature at the iterate
+        cell_matrix(cl, z, hg, opt->rhs_scale, M);
+        lu_factor(M, piv);
+      }
+      for (int s = 0; s < NSP; s++) F[st][s] = (z[s] - r[s]) / hg;
+    }
+    if (!ok_cell) failed++;
+    for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
+    c->Ts[i] = cl.Ts;
+    c->dTs_ge[i] = cl.dTs;
+    if (iters) iters[i] = ok_cell ? nit : -1;
+  }
+  return failed;
+}

@@ -5,6 +5,7 @@
 #include <cmath>
 #include "mvd_internal.hpp"
 #include "mvd_chem.hpp"
+#include "mvd_dirk.hpp"
 #include "../../include/mvd_rhs.h"
 
 struct mvd_context {
@@ -15,6 +16,8 @@ struct mvd_context {
   unsigned *host_flag = nullptr;      // pinned
   double energy_units = 1.0;
   mvd::ChemState chem;
+  int dirk_block = 256;   // batched DIRK launch shape (set by mvd_chem_dirk_tune)
+  int dirk_waves = 0;     // resident waves of the persistent grid (0: one per SIMD)
 };
 
 namespace {
@@ -277,6 +280,157 @@ int mvd_chem_jac(mvd_context *ctx, double, const double *y, double *J) {
 double *mvd_chem_state(mvd_context *ctx, int which) {
   if (!ctx || !ctx->chem.ready) return nullptr;
   return mvd::chem_state_ptr(ctx->chem, which);
+}
+
+// ---- implicit chemistry step ----
+
+int mvd_dirk_sdirk3(mvd_dirk_opts *o) {
+  if (!o) return MVD_BAD_ARGUMENT;
+  std::memset(o, 0, sizeof(*o));
+  // gamma: root of x^3 - 3x^2 + 3x/2 - 1/6 in (1/6, 1/2) (Alexander 1977), by Newton
+  double g = 0.4358665215;
+  for (int it = 0; it < 8; it++)
+    g -= (((g - 3.0) * g + 1.5) * g - 1.0 / 6.0) / ((3.0 * g - 6.0) * g + 1.5);
+  const double t2 = 0.5 * (1.0 + g);
+  const double b1 = -(6.0 * g * g - 16.0 * g + 1.0) / 4.0;
+  const double b2 = (6.0 * g * g - 20.0 * g + 5.0) / 4.0;
+  o->nstage = 3;
+  o->A[0] = g;
+  o->A[4] = t2 - g;  o->A[5] = g;
+  o->A[8] = b1;      o->A[9] = b2;  o->A[10] = g;
+  o->c[0] = g; o->c[1] = t2; o->c[2] = 1.0;
+  o->rtol = 1e-5;            // the reference's chemistry test input (input_primordial_ode.txt)
+  o->atol = 1e-9;
+  o->newton_tol = 0.1;
+  o->max_iters = 5;
+  o->max_setups = 2;
+  o->rhs_scale = 1.0;
+  return MVD_SUCCESS;
+}
+
+namespace {
+
+int dirk_check(const mvd_dirk_opts *o, mvd::DirkDev &d) {
+  if (!o || o->nstage < 1 || o->nstage > mvd::MAXSTAGE) return MVD_BAD_ARGUMENT;
+  if (o->max_iters < 1 || o->max_iters > 100 || o->max_setups < 0 || o->max_setups > 20 ||
+      !(o->newton_tol > 0) || !(o->rtol >= 0) || !(o->atol > 0) || !(o->rhs_scale > 0))
+    return MVD_BAD_ARGUMENT;
+  for (int i = 0; i < o->nstage; i++) {
+    if (!(o->A[i * 4 + i] > 0) || o->A[i * 4 + i] != o->A[0]) return MVD_UNSUPPORTED;  // SDIRK
+    for (int j = i + 1; j < 4; j++)
+      if (o->A[i * 4 + j] != 0.0) return MVD_BAD_ARGUMENT;
+  }
+  d.nstage = o->nstage;
+  for (int k = 0; k < 16; k++) d.A[k] = o->A[k];
+  d.rtol = o->rtol; d.atol = o->atol; d.ntol = o->newton_tol;
+  d.max_iters = o->max_iters; d.max_setups = o->max_setups;
+  d.rhs_scale = o->rhs_scale;
+  return MVD_SUCCESS;
+}
+
+int default_waves(mvd_context *ctx) {
+  if (ctx->dirk_waves > 0) return ctx->dirk_waves;
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) == hipSuccess) ncu = pr.multiProcessorCount;
+  }
+  return ncu * 4;                 // the batched kernel runs one wave per SIMD
+}
+
+}  // namespace
+
+int mvd_chem_dirk_step(mvd_context *ctx, double, double h, const mvd_dirk_opts *o, double *y,
+                       int plain, mvd_dirk_stats *stats) {
+  if (!ctx || !y || !(h > 0)) return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  mvd::DirkDev d;
+  int rc = dirk_check(o, d);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  if (!dev_ok(hipEventCreate(&e0)) || !dev_ok(hipEventCreate(&e1))) return MVD_DEVICE_ERROR;
+  (void)hipEventRecord(e0, ctx->stream);
+  rc = plain ? mvd::dirk_plain_run(ctx->chem, d, h, y, ctx->stream)
+             : mvd::dirk_batched_launch(ctx->chem, d, h, y, ctx->stream, default_waves(ctx),
+                                        ctx->dirk_block);
+  (void)hipEventRecord(e1, ctx->stream);
+  if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return MVD_DEVICE_ERROR; }
+  int *hc = ctx->chem.host_counters;
+  const bool ok = dev_ok(hipMemcpyAsync(hc, ctx->chem.counters + 1, sizeof(int) * mvd::NSTAT,
+                                        hipMemcpyDeviceToHost, ctx->stream)) &&
+                  dev_ok(hipStreamSynchronize(ctx->stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!ok) return MVD_DEVICE_ERROR;
+  if (stats) {
+    stats->newton_iters = hc[mvd::STAT_ITERS];
+    stats->max_cell_iters = hc[mvd::STAT_MAXIT];
+    stats->failed_cells = hc[mvd::STAT_FAILED];
+    stats->setups = hc[mvd::STAT_SETUPS];
+    for (int b = 0; b < 16; b++) stats->hist[b] = hc[mvd::STAT_HIST + b];
+    stats->ms = ms;
+  }
+  return hc[mvd::STAT_FAILED];
+}
+
+int mvd_chem_dirk_tune(mvd_context *ctx, double h, const mvd_dirk_opts *o, const double *y,
+                       double *best_ms) {
+  if (!ctx || !y || !(h > 0)) return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  mvd::DirkDev d;
+  int rc = dirk_check(o, d);
+  if (rc) return rc;
+  const long n = ctx->chem.n;
+  double *ys = nullptr, *Ts0 = nullptr;
+  if (!dev_ok(hipMalloc(&ys, sizeof(double) * n * mvd::NSP)) ||
+      !dev_ok(hipMalloc(&Ts0, sizeof(double) * 2 * n))) {
+    (void)hipFree(ys);
+    return MVD_DEVICE_ERROR;
+  }
+  (void)hipMemcpyAsync(Ts0, ctx->chem.Ts, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream);
+  (void)hipMemcpyAsync(Ts0 + n, ctx->chem.dTs_ge, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                       ctx->stream);
+  int ncu = 256, dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) == hipSuccess) ncu = pr.multiProcessorCount;
+  }
+  const int blocks[] = {64, 128, 256};
+  const int wpc[] = {1, 2, 4, 8, 16};
+  double best = 1e300;
+  int bb = 256, bw = ncu * 4;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  for (int b : blocks)
+    for (int w : wpc) {
+      (void)hipMemcpyAsync(ys, y, sizeof(double) * n * mvd::NSP, hipMemcpyDeviceToDevice, ctx->stream);
+      (void)hipMemcpyAsync(ctx->chem.Ts, Ts0, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream);
+      (void)hipMemcpyAsync(ctx->chem.dTs_ge, Ts0 + n, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                           ctx->stream);
+      (void)hipEventRecord(e0, ctx->stream);
+      if (mvd::dirk_batched_launch(ctx->chem, d, h, ys, ctx->stream, ncu * w, b)) { rc = MVD_DEVICE_ERROR; break; }
+      (void)hipEventRecord(e1, ctx->stream);
+      if (!dev_ok(hipStreamSynchronize(ctx->stream))) { rc = MVD_DEVICE_ERROR; break; }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) { best = ms; bb = b; bw = ncu * w; }
+    }
+  (void)hipMemcpyAsync(ctx->chem.Ts, Ts0, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream);
+  (void)hipMemcpyAsync(ctx->chem.dTs_ge, Ts0 + n, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                       ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(ys);
+  (void)hipFree(Ts0);
+  if (rc) return rc;
+  ctx->dirk_block = bb;
+  ctx->dirk_waves = bw;
+  if (best_ms) *best_ms = best;
+  return MVD_SUCCESS;
 }
 
 }  // extern "C"

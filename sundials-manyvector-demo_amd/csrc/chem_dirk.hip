@@ -1,0 +1,689 @@
+// One diagonally implicit RK step of the per-cell primordial chemistry (the fast implicit
+// stage of the reacting-flow demo), two ways:
+//
+//   batched  - one persistent launch. Each wave pulls 64-cell chunks from an atomic work
+//              queue (work stealing across waves and CUs: cells need 3 to 9+ Newton
+//              iterations) and each lane runs all stages and Newton iterations of its cell:
+//              RHS with the temperature solve, analytic Jacobian, M = I - h a_ii s J, 10x10
+//              LU with partial pivoting, triangular solves, WRMS convergence test, re-setup
+//              after max_iters. The iterate, the stage right-hand side and the LU factors
+//              stay in the register file for the whole step (one wave per SIMD: 512 VGPR +
+//              AGPR); only the stage derivatives, read once per stage, go to a per-chunk
+//              scratch block. Statistics are accumulated per block in LDS.
+//   plain    - the reference's structure (primordial_ode_main.cpp with SUNDIALS + MAGMA):
+//              one-thread-per-cell kernels for the RHS, the Jacobian into column-major 10x10
+//              blocks, the batched LU factorisation, and the Newton update with the
+//              triangular solves, the iterate and stage values in global memory, and a host
+//              round trip per Newton iteration to test convergence.
+//
+// Both follow the same per-cell algorithm (stage predictor, modified Newton with the
+// Jacobian from y_n, re-setup after max_iters) and the same operation order, so they agree
+// with each other bitwise and with the CPU oracle iteration for iteration.
+#include "mvd_chem.hpp"
+#include "mvd_dirk.hpp"
+#include "../../include/mvd_rhs.h"
+
+namespace mvd {
+
+// Per-cell inputs: scale factors at sc[s*st] / isc[s*st] (st = 1 for the per-cell AoS
+// arrays, 64 for the element-major chunk block) and the cell's auxiliary values.
+struct CellIn {
+  const double *sc, *isc;
+  long st;
+  CellAux ax;
+  __device__ double s(int k) const { return sc[k * st]; }
+  __device__ double is(int k) const { return isc[k * st]; }
+};
+
+__device__ inline CellIn load_cell(const ChemPtrsD &p, long i) {
+  return CellIn{p.scale + NSP * i, p.iscale + NSP * i, 1,
+                CellAux{p.md[i], p.imd[i], p.cie[i], p.h2o[i]}};
+}
+
+// calculate_rhs_cvklu for one cell in scaled variables, times srhs; updates Ts/dTs
+__device__ inline void cell_rhs(const ChemConst &cc, const CellIn &c, double &Ts, double &dTs,
+                                const double (&y)[NSP], double srhs, double (&f)[NSP]) {
+#pragma clang fp contract(off)
+  double ya[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) ya[s] = y[s] * c.s(s);
+  chem_temperature(cc, ya, Ts, dTs);
+  // species rows from the reaction tables, then the energy row from the cooling tables;
+  // the fence keeps the two groups of table loads from being scheduled together
+  double k[NRATE];
+#pragma unroll
+  for (int r = 0; r < NRATE; r++) k[r] = 0.0;
+  chem_rates_only_range<0, C_BREM>(cc, Ts, k);
+  chem_rhs_species(ya, k, f);
+  __asm__ volatile("" ::: "memory");
+  chem_rates_only_range<C_BREM, NRATE>(cc, Ts, k);
+  f[9] = chem_rhs_energy(ya, k, Ts, cc.z, c.ax);
+#pragma unroll
+  for (int s = 0; s < NSP; s++) f[s] *= c.is(s);
+  f[9] *= c.ax.inv_mdensity;
+#pragma unroll
+  for (int s = 0; s < NSP; s++) f[s] *= srhs;
+}
+
+// A cell's 10x10 matrix in memory: element (r, c) at b[r*rs + c*cs]. The batched solver
+// uses its element-major chunk block (rs = 640, cs = 64); the plain solver the reference's
+// column-major blocks (rs = 1, cs = 10).
+struct Mat {
+  double *b;
+  long rs, cs;
+  __device__ double &operator()(int r, int c) const { return b[r * rs + c * cs]; }
+};
+
+// M = I - hg * srhs * J(y), J = calculate_jacobian_cvklu from the current temperature.
+// The species rows (reaction tables) are formed and stored before the gas-energy row
+// (cooling tables) is evaluated; the fence keeps the two groups of table loads apart so that
+// both never occupy registers at once.
+__device__ inline void cell_matrix(const ChemConst &cc, const CellIn &c, double Ts, double dTs,
+                                   const double (&y)[NSP], double hg, double srhs,
+                                   const Mat &m) {
+#pragma clang fp contract(off)
+  double ya[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) ya[s] = y[s] * c.s(s);
+  double k[NRATE], dk[NRATE], J[NSP][NSP];
+#pragma unroll
+  for (int r = 0; r < NRATE; r++) k[r] = dk[r] = 0.0;
+#pragma unroll
+  for (int r = 0; r < NSP; r++)
+#pragma unroll
+    for (int q = 0; q < NSP; q++) J[r][q] = 0.0;
+  chem_rates_range<0, C_BREM>(cc, Ts, k, dk);
+  chem_jac_species(ya, k, dk, dTs, J);
+#pragma unroll
+  for (int r = 0; r < NSP - 1; r++)
+#pragma unroll
+    for (int q = 0; q < NSP; q++)
+      m(r, q) = (r == q ? 1.0 : 0.0) - hg * (srhs * (J[r][q] * (c.is(r) * c.s(q))));
+  __asm__ volatile("" ::: "memory");
+  chem_rates_range<C_BREM, NRATE>(cc, Ts, k, dk);
+  chem_jac_energy(ya, k, dk, Ts, cc.z, dTs, c.ax, J);
+#pragma unroll
+  for (int q = 0; q < NSP; q++)
+    m(NSP - 1, q) = (NSP - 1 == q ? 1.0 : 0.0) -
+                    hg * (srhs * (J[NSP - 1][q] * (c.is(NSP - 1) * c.s(q))));
+}
+
+// LU with partial pivoting (first maximum wins, LAPACK getrf row interchanges) with the
+// whole matrix in registers (right-looking, predicated row exchanges so that the runtime
+// pivot never indexes a register array). Returns the pivots packed 4 bits each.
+__device__ inline unsigned long long lu_factor_regs(double (&a)[NSP][NSP]) {
+#pragma clang fp contract(off)
+  unsigned long long pv = 0;
+#pragma unroll
+  for (int k = 0; k < NSP; k++) {
+    int p = k;
+    double best = fabs(a[k][k]);
+#pragma unroll
+    for (int r = k + 1; r < NSP; r++) {
+      const double v = fabs(a[r][k]);
+      if (v > best) { best = v; p = r; }
+    }
+    pv |= (unsigned long long)p << (4 * k);
+#pragma unroll
+    for (int r = k + 1; r < NSP; r++) {
+      const bool sw = (r == p);
+#pragma unroll
+      for (int q = 0; q < NSP; q++) {
+        const double x = a[k][q], y = a[r][q];
+        a[k][q] = sw ? y : x;
+        a[r][q] = sw ? x : y;
+      }
+    }
+    const double ik = 1.0 / a[k][k];
+#pragma unroll
+    for (int r = k + 1; r < NSP; r++) {
+      const double l = a[r][k] * ik;
+      a[r][k] = l;
+#pragma unroll
+      for (int q = k + 1; q < NSP; q++) a[r][q] -= l * a[k][q];
+    }
+  }
+  return pv;
+}
+
+__device__ inline void lu_solve(const Mat &a, unsigned long long pv, double (&b)[NSP]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int k = 0; k < NSP; k++) {
+    const int p = (int)((pv >> (4 * k)) & 15);
+#pragma unroll
+    for (int r = k + 1; r < NSP; r++) {
+      const bool sw = (r == p);
+      const double bk = b[k], br = b[r];
+      b[k] = sw ? br : bk;
+      b[r] = sw ? bk : br;
+    }
+  }
+#pragma unroll
+  for (int r = 1; r < NSP; r++)
+#pragma unroll
+    for (int q = 0; q < r; q++) b[r] -= a(r, q) * b[q];
+#pragma unroll
+  for (int r = NSP - 1; r >= 0; r--) {
+#pragma unroll
+    for (int q = r + 1; q < NSP; q++) b[r] -= a(r, q) * b[q];
+    b[r] /= a(r, r);
+  }
+}
+
+__device__ inline double wrms_norm(const double (&d)[NSP], const double (&z)[NSP],
+                                   double rtol, double atol) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < NSP; k++) {
+    const double w = d[k] / (rtol * fabs(z[k]) + atol);
+    s += w * w;
+  }
+  return sqrt(s / NSP);
+}
+
+// a_{st,j} with constant indices only (no dynamic indexing of the by-value argument)
+__device__ inline double dirk_coef(const DirkDev &o, int st, int j) {
+  double v = 0.0;
+#pragma unroll
+  for (int q = 1; q < MAXSTAGE; q++)
+    if (q == st) v = o.A[q * 4 + j];
+  return v;
+}
+
+// ------------------------------------------------------------------------------------
+// statistics: per-wave partial rows, summed by one block at the end (no same-address
+// atomics on the hot path)
+// ------------------------------------------------------------------------------------
+
+// Per-lane accumulators and the wave's histogram (uniform) of one wave.
+struct WaveStats {
+  int iters = 0, maxit = 0, failed = 0, setups = 0;
+  int hist[NHIST] = {};
+  // called by the lanes that finished a cell (the others inactive)
+  __device__ void add(int nit, int nsetup, bool fail) {
+    iters += nit;
+    maxit = nit > maxit ? nit : maxit;
+    failed += fail ? 1 : 0;
+    setups += nsetup;
+    const int bin = nit < NHIST - 1 ? nit : NHIST - 1;
+#pragma unroll
+    for (int b = 0; b < NHIST; b++) hist[b] += (int)__popcll(__ballot(bin == b));
+  }
+  // called by the whole wave: lane 0 writes the wave's row
+  __device__ void flush(int *row) const {
+    int it = iters, mx = maxit, fl = failed, su = setups;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      it += __shfl_xor(it, off, 64);
+      fl += __shfl_xor(fl, off, 64);
+      su += __shfl_xor(su, off, 64);
+      const int m2 = __shfl_xor(mx, off, 64);
+      mx = m2 > mx ? m2 : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      row[STAT_ITERS] = it;
+      row[STAT_MAXIT] = mx;
+      row[STAT_FAILED] = fl;
+      row[STAT_SETUPS] = su;
+#pragma unroll
+      for (int b = 0; b < NHIST; b++) row[STAT_HIST + b] = hist[b];
+    }
+  }
+};
+
+// sums nrow rows of NSTAT partials into out (zeroed beforehand): LDS accumulation per
+// block, then one global atomic per statistic and block
+__global__ void __launch_bounds__(256)
+stats_reduce(const int *rows, long nrow, int *out) {
+  __shared__ int acc[NSTAT];
+  const int t = threadIdx.x;
+  if (t < NSTAT) acc[t] = 0;
+  __syncthreads();
+  const long m = nrow * NSTAT;
+  for (long e = (long)blockIdx.x * blockDim.x + t; e < m; e += (long)gridDim.x * blockDim.x) {
+    const int s = (int)(e % NSTAT);
+    if (s == STAT_MAXIT) atomicMax(&acc[s], rows[e]);
+    else atomicAdd(&acc[s], rows[e]);
+  }
+  __syncthreads();
+  if (t < NSTAT) {
+    if (t == STAT_MAXIT) atomicMax(&out[t], acc[t]);
+    else atomicAdd(&out[t], acc[t]);
+  }
+}
+
+static inline unsigned stats_blocks(long nrow) {
+  const long b = (nrow * NSTAT + 2047) / 2048;
+  return (unsigned)(b < 1 ? 1 : (b > 64 ? 64 : b));
+}
+
+// ------------------------------------------------------------------------------------
+// batched path
+// ------------------------------------------------------------------------------------
+
+// M = I - hg * srhs * J into registers (same expressions and order as cell_matrix)
+__device__ inline void cell_matrix_regs(const ChemConst &cc, const CellIn &c, double Ts,
+                                        double dTs, const double (&y)[NSP], double hg,
+                                        double srhs, double (&M)[NSP][NSP]) {
+#pragma clang fp contract(off)
+  double ya[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) ya[s] = y[s] * c.s(s);
+  double k[NRATE], dk[NRATE];
+#pragma unroll
+  for (int r = 0; r < NRATE; r++) k[r] = dk[r] = 0.0;
+#pragma unroll
+  for (int r = 0; r < NSP; r++)
+#pragma unroll
+    for (int q = 0; q < NSP; q++) M[r][q] = 0.0;
+  chem_rates_range<0, C_BREM>(cc, Ts, k, dk);
+  chem_jac_species(ya, k, dk, dTs, M);
+#pragma unroll
+  for (int r = 0; r < NSP - 1; r++)
+#pragma unroll
+    for (int q = 0; q < NSP; q++)
+      M[r][q] = (r == q ? 1.0 : 0.0) - hg * (srhs * (M[r][q] * (c.is(r) * c.s(q))));
+  chem_rates_range<C_BREM, NRATE>(cc, Ts, k, dk);
+  chem_jac_energy(ya, k, dk, Ts, cc.z, dTs, c.ax, M);
+#pragma unroll
+  for (int q = 0; q < NSP; q++)
+    M[NSP - 1][q] = (NSP - 1 == q ? 1.0 : 0.0) -
+                    hg * (srhs * (M[NSP - 1][q] * (c.is(NSP - 1) * c.s(q))));
+}
+
+__device__ inline void lu_solve_regs(const double (&a)[NSP][NSP], unsigned long long pv,
+                                     double (&b)[NSP]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int k = 0; k < NSP; k++) {
+    const int p = (int)((pv >> (4 * k)) & 15);
+#pragma unroll
+    for (int r = k + 1; r < NSP; r++) {
+      const bool sw = (r == p);
+      const double bk = b[k], br = b[r];
+      b[k] = sw ? br : bk;
+      b[r] = sw ? bk : br;
+    }
+  }
+#pragma unroll
+  for (int r = 1; r < NSP; r++)
+#pragma unroll
+    for (int q = 0; q < r; q++) b[r] -= a[r][q] * b[q];
+#pragma unroll
+  for (int r = NSP - 1; r >= 0; r--) {
+#pragma unroll
+    for (int q = r + 1; q < NSP; q++) b[r] -= a[r][q] * b[q];
+    b[r] /= a[r][r];
+  }
+}
+
+// Stage derivatives of a 64-cell chunk, element-major (F_j[s] of lane l at (j*10+s)*64 + l).
+constexpr int STAGE_WORDS = (MAXSTAGE - 1) * NSP;
+
+__global__ void __launch_bounds__(256)
+chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double *y,
+                  double *work, int *queue, int *rows) {
+#pragma clang fp contract(off)
+  // block statistics in LDS (the per-cell updates are LDS atomics; one row per block)
+  __shared__ int bs[NSTAT];
+  if (threadIdx.x < NSTAT) bs[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(queue, 64);
+    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+    if ((long)base >= n) break;
+    if ((long)base + lane >= n) continue;
+    // per-lane state for the whole step: iterate, stage right-hand side, LU factors
+    double Ts, dTs, z[NSP], r[NSP], lu[NSP][NSP];
+    {
+      const long i = (long)base + lane;
+      Ts = p.Ts[i];
+      dTs = p.dTs[i];
+#pragma unroll
+      for (int s = 0; s < NSP; s++) { z[s] = y[NSP * i + s]; r[s] = z[s] + h * 0.0; }
+    }
+    // One loop, one RHS evaluation per pass; the sequence of evaluations (and so of the
+    // temperature warm starts) is that of the stage/Newton/re-setup loops:
+    //   RHS(y_n), setup; per stage: RHS(z), Newton update, ... ; after max_iters without
+    //   convergence: RHS(z), setup, and the stage's Newton loop restarts.
+    bool setup = true, ok = true;
+    int st = 0, it = 0, setups = 0, nit = 0, nsetup = 0;
+    const double hg = h * o.A[0];                          // SDIRK: one diagonal value
+    unsigned long long pv = 0;
+    for (;;) {
+      // The lane offset is made opaque on every pass so that the per-lane addresses of the
+      // cell's inputs are formed where they are used instead of being held in registers.
+      long lo = lane;
+      __asm__ volatile("" : "+v"(lo));
+      const long i = (long)base + lo;
+      const CellIn c = load_cell(p, i);
+      double *F = work + (long)STAGE_WORDS * base + lo;     // F_j[s] at (j*10 + s)*64
+      double f[NSP];
+      cell_rhs(cc, c, Ts, dTs, z, o.rhs_scale, f);
+      if (setup) {
+        cell_matrix_regs(cc, c, Ts, dTs, z, hg, o.rhs_scale, lu);
+        pv = lu_factor_regs(lu);
+        nsetup++;
+        setup = false;
+        it = 0;
+        continue;
+      }
+      double d[NSP];
+#pragma unroll
+      for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
+      lu_solve_regs(lu, pv, d);
+#pragma unroll
+      for (int s = 0; s < NSP; s++) z[s] += d[s];
+      nit++;
+      it++;
+      if (wrms_norm(d, z, o.rtol, o.atol) <= o.ntol) {
+        if (++st == o.nstage) break;
+        // F_{st-1} = (z - r)/hg (ARKODE's implicit-stage derivative); the next stage's r
+#pragma unroll
+        for (int s = 0; s < NSP; s++) F[((st - 1) * NSP + s) * 64] = (z[s] - r[s]) / hg;
+#pragma unroll
+        for (int s = 0; s < NSP; s++) {
+          double a = 0.0;
+#pragma unroll
+          for (int j = 0; j < MAXSTAGE - 1; j++)
+            if (j < st) a += dirk_coef(o, st, j) * F[(j * NSP + s) * 64];
+          r[s] = y[NSP * i + s] + h * a;                   // y still holds y_n
+        }
+        it = 0;
+        setups = 0;
+      } else if (it == o.max_iters) {
+        if (++setups > o.max_setups) { ok = false; break; }
+        setup = true;
+      }
+    }
+    {
+      const long i = (long)base + lane;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
+      p.Ts[i] = Ts;
+      p.dTs[i] = dTs;
+    }
+    atomicAdd(&bs[STAT_ITERS], nit);
+    atomicMax(&bs[STAT_MAXIT], nit);
+    atomicAdd(&bs[STAT_SETUPS], nsetup);
+    if (!ok) atomicAdd(&bs[STAT_FAILED], 1);
+    atomicAdd(&bs[STAT_HIST + (nit < NHIST - 1 ? nit : NHIST - 1)], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < NSTAT) rows[(long)blockIdx.x * NSTAT + threadIdx.x] = bs[threadIdx.x];
+}
+
+// ------------------------------------------------------------------------------------
+// plain (reference-structured) kernels
+// ------------------------------------------------------------------------------------
+
+// f = s * rhs(z) for the active cells (updates Ts/dTs of those cells)
+__global__ void __launch_bounds__(256)
+plain_rhs(long n, ChemConst cc, ChemPtrsD p, double srhs, const double *z, double *f,
+          const int *active) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !active[i]) return;
+  const CellIn c = load_cell(p, i);
+  double Ts = p.Ts[i], dTs = p.dTs[i];
+  double zz[NSP], ff[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) zz[s] = z[NSP * i + s];
+  cell_rhs(cc, c, Ts, dTs, zz, srhs, ff);
+  p.Ts[i] = Ts;
+  p.dTs[i] = dTs;
+#pragma unroll
+  for (int s = 0; s < NSP; s++) f[NSP * i + s] = ff[s];
+}
+
+// calculate_jacobian_cvklu at z from the current temperature and M = I - hg s J into the
+// column-major 10x10 blocks (SUNMatrix_MagmaDenseBlock) of the cells flagged in `todo`
+__global__ void __launch_bounds__(256)
+plain_jac(long n, ChemConst cc, ChemPtrsD p, double hg, double srhs, const double *z,
+          double *LU, const int *todo) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !todo[i]) return;
+  const CellIn c = load_cell(p, i);
+  double zz[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) zz[s] = z[NSP * i + s];
+  cell_matrix(cc, c, p.Ts[i], p.dTs[i], zz, hg, srhs, Mat{LU + (long)NSP * NSP * i, 1, NSP});
+}
+
+// batched getrf: each thread factors its cell's block in registers
+__global__ void __launch_bounds__(256)
+plain_getrf(long n, double *LU, unsigned long long *piv, const int *todo) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !todo[i]) return;
+  double *blk = LU + (long)NSP * NSP * i;
+  double a[NSP][NSP];
+#pragma unroll
+  for (int q = 0; q < NSP; q++)
+#pragma unroll
+    for (int r = 0; r < NSP; r++) a[r][q] = blk[NSP * q + r];
+  piv[i] = lu_factor_regs(a);
+#pragma unroll
+  for (int q = 0; q < NSP; q++)
+#pragma unroll
+    for (int r = 0; r < NSP; r++) blk[NSP * q + r] = a[r][q];
+}
+
+// stage start: r = y_n + h sum_{j<st} a_st,j F_j; activate the cells that have not failed
+__global__ void __launch_bounds__(256)
+plain_stage_init(long n, int st, double h, DirkDev o, const double *yn, const double *F,
+                 double *r, int *active, const int *failed) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int s = 0; s < NSP; s++) {
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAXSTAGE - 1; j++)
+      if (j < st) a += dirk_coef(o, st, j) * F[(long)j * n * NSP + NSP * i + s];
+    r[NSP * i + s] = yn[NSP * i + s] + h * a;
+  }
+  active[i] = failed[i] ? 0 : 1;
+}
+
+// Newton update (getrs + z += d + WRMS test) of the active cells; converged cells are
+// deactivated, the others counted (one atomic per wave: uniform address)
+__global__ void __launch_bounds__(256)
+plain_newton(long n, double hg, DirkDev o, double *LU, const unsigned long long *piv,
+             const double *f, const double *r, double *z, int *active, int *cell_iters,
+             int *nactive) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !active[i]) return;
+  double d[NSP], zz[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) {
+    zz[s] = z[NSP * i + s];
+    d[s] = -(zz[s] - hg * f[NSP * i + s] - r[NSP * i + s]);
+  }
+  lu_solve(Mat{LU + (long)NSP * NSP * i, 1, NSP}, piv[i], d);
+#pragma unroll
+  for (int s = 0; s < NSP; s++) {
+    zz[s] += d[s];
+    z[NSP * i + s] = zz[s];
+  }
+  cell_iters[i] += 1;
+  if (wrms_norm(d, zz, o.rtol, o.atol) <= o.ntol) active[i] = 0;
+  else atomicAdd(nactive, 1);
+}
+
+// stage end: F_st = (z - r)/hg
+__global__ void __launch_bounds__(256)
+plain_stage_finish(long n, int st, double hg, const double *z, const double *r, double *F) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int s = 0; s < NSP; s++)
+    F[(long)st * n * NSP + NSP * i + s] = (z[NSP * i + s] - r[NSP * i + s]) / hg;
+}
+
+// per-wave statistics rows of the plain path
+__global__ void __launch_bounds__(256)
+plain_finish(long n, const int *cell_iters, const int *cell_setups, const int *failed,
+             int *rows) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  WaveStats ws;
+  if (i < n) ws.add(cell_iters[i], cell_setups[i], failed[i] != 0);
+  if (((long)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) < n) ws.flush(rows + (i >> 6) * NSTAT);
+}
+
+__global__ void __launch_bounds__(256)
+fill_int(long n, int *a, int v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void __launch_bounds__(256)
+copy_double(long n, const double *a, double *b) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// the cells still active after the last allowed setup have failed
+__global__ void __launch_bounds__(256)
+mark_failed(long n, const int *active, int *failed) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && active[i]) failed[i] = 1;
+}
+
+__global__ void __launch_bounds__(256)
+count_setups(long n, const int *active, int *cell_setups) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && active[i]) cell_setups[i] += 1;
+}
+
+// ------------------------------------------------------------------------------------
+// host drivers
+// ------------------------------------------------------------------------------------
+
+static inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
+
+static ChemPtrsD ptrs_d(ChemState &s) {
+  return ChemPtrsD{s.scale, s.inv_scale, s.mdensity, s.inv_mdensity, s.cie_oda, s.h2_oda,
+                   s.Ts, s.dTs_ge};
+}
+
+static int ensure_work(ChemState &s, size_t bytes) {
+  if (s.work_bytes >= bytes) return 0;
+  if (s.work) (void)hipFree(s.work);
+  s.work = nullptr;
+  s.work_bytes = 0;
+  if (hipMalloc(&s.work, bytes) != hipSuccess) return -1;
+  s.work_bytes = bytes;
+  return 0;
+}
+
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
+                        int grid_waves, int block) {
+  const long n = s.n;
+  if (n <= 0 || n > (1L << 30)) return -1;
+  if (block != 64 && block != 128 && block != 256) return -1;
+  const int wpb = block / 64;
+  const size_t chunks = (size_t)(n + 63) / 64;
+  long waves = grid_waves > 0 ? grid_waves : 1;
+  if (waves > (long)chunks) waves = (long)chunks;
+  const long grid = (waves + wpb - 1) / wpb;
+  const long nrow = grid;                                  // one statistics row per block
+  // work: stage-derivative blocks | per-block statistics rows
+  const size_t b_blk = align256(chunks * 64 * STAGE_WORDS * sizeof(double));
+  const size_t b_rows = align256((size_t)nrow * NSTAT * sizeof(int));
+  if (ensure_work(s, b_blk + b_rows)) return -1;
+  char *w = (char *)s.work;
+  int *rows = (int *)(w + b_blk);
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
+  hipLaunchKernelGGL(chem_dirk_batched, dim3((unsigned)grid), dim3(block), 0, st, n, s.cc,
+                     ptrs_d(s), o, h, y, (double *)w, s.counters, rows);
+  hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(nrow)), dim3(256), 0, st,
+                     (const int *)rows, nrow, s.counters + 1);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st) {
+  const long n = s.n;
+  if (n <= 0 || n > (1L << 30)) return -1;
+  // scratch: yn, z, f, r, F[MAXSTAGE-1], LU blocks | pivots | active, iters, setups,
+  // failed, count | per-wave statistics rows
+  const size_t b_dbl = align256(((size_t)n * NSP * (4 + (MAXSTAGE - 1)) + (size_t)n * NSP * NSP) *
+                                sizeof(double));
+  const size_t b_piv = align256((size_t)n * sizeof(unsigned long long));
+  const size_t b_int = align256(((size_t)n * 4 + 64) * sizeof(int));
+  const long nrow = (n + 63) / 64;
+  const size_t b_rows = align256((size_t)nrow * NSTAT * sizeof(int));
+  if (ensure_work(s, b_dbl + b_piv + b_int + b_rows)) return -1;
+  char *w = (char *)s.work;
+  double *yn = (double *)w, *z = yn + n * NSP, *f = z + n * NSP, *r = f + n * NSP;
+  double *acc = r + n * NSP, *LU = acc + (long)(MAXSTAGE - 1) * n * NSP;
+  unsigned long long *piv = (unsigned long long *)(w + b_dbl);
+  int *active = (int *)(w + b_dbl + b_piv), *iters = active + n, *csetups = iters + n,
+      *failed = csetups + n, *nact = failed + n;
+  int *rows = (int *)(w + b_dbl + b_piv + b_int);
+  const ChemPtrsD p = ptrs_d(s);
+  hipLaunchKernelGGL(copy_double, dim3(nb(n * NSP)), dim3(256), 0, st, n * NSP, y, yn);
+  hipLaunchKernelGGL(copy_double, dim3(nb(n * NSP)), dim3(256), 0, st, n * NSP, y, z);
+  // iters, setups, failed are contiguous
+  hipLaunchKernelGGL(fill_int, dim3(nb(3 * n)), dim3(256), 0, st, 3 * n, iters, 0);
+  hipLaunchKernelGGL(fill_int, dim3(nb(n)), dim3(256), 0, st, n, active, 1);
+  // temperature at y_n, Jacobian at y_n, factor (the first stage's a_ii)
+  hipLaunchKernelGGL(plain_rhs, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, o.rhs_scale, z, f,
+                     active);
+  hipLaunchKernelGGL(plain_jac, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, h * o.A[0],
+                     o.rhs_scale, z, LU, active);
+  hipLaunchKernelGGL(plain_getrf, dim3(nb(n)), dim3(256), 0, st, n, LU, piv, active);
+  hipLaunchKernelGGL(count_setups, dim3(nb(n)), dim3(256), 0, st, n, active, csetups);
+  int *hcnt = s.host_counters;
+  for (int stg = 0; stg < o.nstage; stg++) {
+    const double hg = h * o.A[stg * 4 + stg];
+    hipLaunchKernelGGL(plain_stage_init, dim3(nb(n)), dim3(256), 0, st, n, stg, h, o, yn, acc, r,
+                       active, failed);
+    int setups = 0;
+    for (;;) {
+      bool all_done = false;
+      for (int it = 0; it < o.max_iters; it++) {
+        hipLaunchKernelGGL(plain_rhs, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, o.rhs_scale, z,
+                           f, active);
+        hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, 1L, nact, 0);
+        hipLaunchKernelGGL(plain_newton, dim3(nb(n)), dim3(256), 0, st, n, hg, o, LU, piv, f, r,
+                           z, active, iters, nact);
+        // convergence test on the host, as the reference's Newton loop does
+        if (hipMemcpyAsync(hcnt, nact, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+          return -1;
+        if (*hcnt == 0) { all_done = true; break; }
+      }
+      if (all_done) break;
+      if (++setups > o.max_setups) {
+        hipLaunchKernelGGL(mark_failed, dim3(nb(n)), dim3(256), 0, st, n, active, failed);
+        break;
+      }
+      hipLaunchKernelGGL(plain_rhs, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, o.rhs_scale, z, f,
+                         active);
+      hipLaunchKernelGGL(plain_jac, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, hg, o.rhs_scale,
+                         z, LU, active);
+      hipLaunchKernelGGL(plain_getrf, dim3(nb(n)), dim3(256), 0, st, n, LU, piv, active);
+      hipLaunchKernelGGL(count_setups, dim3(nb(n)), dim3(256), 0, st, n, active, csetups);
+    }
+    if (stg + 1 < o.nstage)
+      hipLaunchKernelGGL(plain_stage_finish, dim3(nb(n)), dim3(256), 0, st, n, stg, hg, z, r,
+                         acc);
+  }
+  hipLaunchKernelGGL(copy_double, dim3(nb(n * NSP)), dim3(256), 0, st, n * NSP, z, y);
+  hipLaunchKernelGGL(plain_finish, dim3(nb(n)), dim3(256), 0, st, n, (const int *)iters,
+                     (const int *)csetups, (const int *)failed, rows);
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
+  hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(nrow)), dim3(256), 0, st,
+                     (const int *)rows, nrow, s.counters + 1);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace mvd

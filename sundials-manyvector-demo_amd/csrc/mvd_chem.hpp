@@ -70,6 +70,13 @@ __device__ inline int chem_bin(const ChemConst &c, double logT) {
 
 // cvklu_calculate_temperature (raja_primordial_network.cpp:908-1011): 10 Newton steps
 // from the warm start Ts, clamp to [1,1e5], dTs_ge = 1/dge_dT.
+//
+// The step map x -> x' (and its dge_dT) depends only on the input x and the cell, so once
+// an iterate repeats bitwise the rest of the ten steps is known: a fixed point (x' == x)
+// keeps x and its dge_dT; a 2-cycle (x' == the previous input) alternates, and the tenth
+// output and ninth dge_dT follow from the parity of the step. Leaving the loop there gives
+// bitwise the values of all ten steps, with fewer evaluations (most warm-started solves
+// settle in 2-4 steps).
 __device__ inline void chem_temperature(const ChemConst &c, const double (&y)[NSP], double &Ts,
                                         double &dTs_ge) {
 #pragma clang fp contract(off)
@@ -84,6 +91,7 @@ __device__ inline void chem_temperature(const ChemConst &c, const double (&y)[NS
   double T = Ts, Tnew = T * 1.1, dge_dT = 0.0;
   const double *g1t = c.tab + G_GH2_1 * TLEN, *dg1t = c.tab + G_DGH2_1 * TLEN;
   const double *g2t = c.tab + G_GH2_2 * TLEN, *dg2t = c.tab + G_DGH2_2 * TLEN;
+  double x_prev = 0.0, d_prev = 0.0;    // input and dge_dT of the previous step
   for (int it = 0; it < 10; it++) {
     T = Ts;
     const double lT = log(Ts);
@@ -103,6 +111,17 @@ __device__ inline void chem_temperature(const ChemConst &c, const double (&y)[NS
     const double dge = T * kb * nsum / dm - y[9];
     Tnew = T - dge / dge_dT;
     Ts = Tnew;
+    const long long bn = __double_as_longlong(Tnew);
+    if (bn == __double_as_longlong(T)) break;                  // fixed point
+    if (it > 0 && bn == __double_as_longlong(x_prev)) {        // 2-cycle x_{j+1} == x_{j-1}
+      if ((it & 1) == 0) {                                     // j even: x_10 = x_j, d_9 = d_{j-1}
+        Tnew = T;
+        dge_dT = d_prev;
+      }                                                        // j odd: x_10 = x_{j+1}, d_9 = d_j
+      break;
+    }
+    x_prev = T;
+    d_prev = dge_dT;
   }
   Ts = Tnew;
   if (Ts < c.tmin) Ts = c.tmin;
@@ -126,6 +145,41 @@ __device__ inline void chem_rates(const ChemConst &c, double T, double (&k)[NRAT
     const double a0 = c.tab[r * TLEN + b], a1 = c.tab[r * TLEN + b + 1];
     k[r] = a0 + Tdef * (a1 - a0);
     dk[r] = (a1 - a0) * Tfactor;
+  }
+}
+
+// Rates and derivatives of tables [r0, r1) only.
+template <int R0, int R1>
+__device__ inline void chem_rates_range(const ChemConst &c, double T, double (&k)[NRATE],
+                                        double (&dk)[NRATE]) {
+#pragma clang fp contract(off)
+  const double lT = log(T);
+  const int b = chem_bin(c, lT);
+  const double t1 = c.lb + (double)b * c.dbin;
+  const double t2 = c.lb + (double)(b + 1) * c.dbin;
+  const double Tdef = (lT - t1) / (t2 - t1);
+  const double Tfactor = (1.0 / T) / (t2 - t1);
+#pragma unroll
+  for (int r = R0; r < R1; r++) {
+    const double a0 = c.tab[r * TLEN + b], a1 = c.tab[r * TLEN + b + 1];
+    k[r] = a0 + Tdef * (a1 - a0);
+    dk[r] = (a1 - a0) * Tfactor;
+  }
+}
+
+// Rate values of tables [R0, R1) only.
+template <int R0, int R1>
+__device__ inline void chem_rates_only_range(const ChemConst &c, double T, double (&k)[NRATE]) {
+#pragma clang fp contract(off)
+  const double lT = log(T);
+  const int b = chem_bin(c, lT);
+  const double t1 = c.lb + (double)b * c.dbin;
+  const double t2 = c.lb + (double)(b + 1) * c.dbin;
+  const double Tdef = (lT - t1) / (t2 - t1);
+#pragma unroll
+  for (int r = R0; r < R1; r++) {
+    const double a0 = c.tab[r * TLEN + b], a1 = c.tab[r * TLEN + b + 1];
+    k[r] = a0 + Tdef * (a1 - a0);
   }
 }
 
@@ -187,9 +241,11 @@ __device__ inline Reactions chem_reactions(const double (&y)[NSP], const double 
 
 // Species and gas-energy RHS in physical units (raja_primordial_network.cpp:1302-1371,
 // before the inv_scale / inv_mdensity factors).
-__device__ inline void chem_rhs_physical(const double (&y)[NSP], const double (&k)[NRATE],
-                                         double T, double z, const CellAux &ax,
-                                         double (&f)[NSP]) {
+//
+// Split in two like the Jacobian: the species rows read only the 21 reaction rates, the
+// gas-energy row only the 25 cooling rates.
+__device__ inline void chem_rhs_species(const double (&y)[NSP], const double (&k)[NRATE],
+                                        double (&f)[NSP]) {
 #pragma clang fp contract(off)
   const Reactions R = chem_reactions(y, k);
   f[0] = R.r08 + R.r10 - R.r11 - R.r12 - R.r13 + R.r19 + R.r21 + R.r22;
@@ -203,6 +259,11 @@ __device__ inline void chem_rhs_physical(const double (&y)[NSP], const double (&
   f[7] = R.r05 - R.r06;
   f[8] = R.r01 - R.r02 + R.r03 - R.r04 + R.r05 - R.r06 - R.r07 + R.r08 + R.r14 + R.r15 + R.r17 -
          R.r18;
+}
+
+__device__ inline double chem_rhs_energy(const double (&y)[NSP], const double (&k)[NRATE],
+                                         double T, double z, const CellAux &ax) {
+#pragma clang fp contract(off)
   const double H2_1 = y[0], H_1 = y[2], H_2 = y[3], He_1 = y[5], He_2 = y[6], He_3 = y[7],
                de = y[8];
   const double cie = ax.cie, h2o = ax.h2o;
@@ -211,7 +272,7 @@ __device__ inline void chem_rhs_physical(const double (&y)[NSP], const double (&
   const double ncrD = H2_1 * k[C_NCRD2] + H_1 * k[C_NCRD1];
   const double zp1 = z + 1.0;
   const double zp4 = (zp1 * zp1) * (zp1 * zp1);
-  f[9] = -2.0158800000000001 * H2_1 * k[C_CIECO] * cie * ax.mdensity -
+  return -2.0158800000000001 * H2_1 * k[C_CIECO] * cie * ax.mdensity -
          H2_1 * cie * k[C_H2LTE] * h2o / (k[C_H2LTE] / gS + 1.0) - H_1 * k[C_CEHI] * cie * de -
          H_1 * k[C_CIHI] * cie * de - H_2 * cie * de * k[C_REHII] -
          He_1 * k[C_CIHEI] * cie * de - He_2 * k[C_CEHEII] * cie * de -
@@ -224,20 +285,26 @@ __device__ inline void chem_rhs_physical(const double (&y)[NSP], const double (&
              ((-H2_1) * H_1 * k[C_H2MCOOL] + (H_1 * H_1 * H_1) * k[C_H2MHEAT]);
 }
 
+__device__ inline void chem_rhs_physical(const double (&y)[NSP], const double (&k)[NRATE],
+                                         double T, double z, const CellAux &ax,
+                                         double (&f)[NSP]) {
+  chem_rhs_species(y, k, f);
+  f[9] = chem_rhs_energy(y, k, T, z, ax);
+}
+
 // Analytic Jacobian, physical units, row-major J[r][c] with the reference's extra
 // factors folded in (Tge on the ge column, 1/mdensity on the ge row, the replaced ge-ge
 // entry), raja_primordial_network.cpp:1518-1867. Structural zeros are left at 0.
-__device__ inline void chem_jac_physical(const double (&y)[NSP], const double (&k)[NRATE],
-                                         const double (&rk)[NRATE], double T, double z,
-                                         double Tge, const CellAux &ax,
-                                         double (&J)[NSP][NSP]) {
+//
+// Split in two: the species rows 0-8 read only the 21 reaction rates (and their
+// T-derivatives), the gas-energy row 9 only the 25 cooling rates, so a caller short of
+// registers can interpolate and consume one group at a time.
+__device__ inline void chem_jac_species(const double (&y)[NSP], const double (&k)[NRATE],
+                                        const double (&rk)[NRATE], double Tge,
+                                        double (&J)[NSP][NSP]) {
 #pragma clang fp contract(off)
   const double H2_1 = y[0], H2_2 = y[1], H_1 = y[2], H_2 = y[3], H_m0 = y[4], He_1 = y[5],
                He_2 = y[6], He_3 = y[7], de = y[8];
-#pragma unroll
-  for (int a = 0; a < NSP; a++)
-#pragma unroll
-    for (int b = 0; b < NSP; b++) J[a][b] = 0.0;
   const double k01 = k[K01], k02 = k[K02], k03 = k[K03], k04 = k[K04], k05 = k[K05],
                k06 = k[K06], k07 = k[K07], k08 = k[K08], k09 = k[K09], k10 = k[K10],
                k11 = k[K11], k12 = k[K12], k13 = k[K13], k14 = k[K14], k15 = k[K15],
@@ -320,8 +387,15 @@ __device__ inline void chem_jac_physical(const double (&y)[NSP], const double (&
             k07 * H_1 + k14 * H_m0 - k18 * H2_2;
   J[8][9] = (D.r01 - D.r02 + D.r03 - D.r04 + D.r05 - D.r06 - D.r07 + D.r08 + D.r14 + D.r15 +
              D.r17 - D.r18) * Tge;
+}
 
-  // gas-energy row
+__device__ inline void chem_jac_energy(const double (&y)[NSP], const double (&k)[NRATE],
+                                       const double (&rk)[NRATE], double T, double z,
+                                       double Tge, const CellAux &ax, double (&J)[NSP][NSP]) {
+#pragma clang fp contract(off)
+  const double H2_1 = y[0], H_1 = y[2], H_2 = y[3], He_1 = y[5], He_2 = y[6], He_3 = y[7],
+               de = y[8];
+  const double H1sq = H_1 * H_1;
   const double brem = k[C_BREM], ceHeI = k[C_CEHEI], ceHeII = k[C_CEHEII], ceHI = k[C_CEHI],
                cieco = k[C_CIECO], ciHeI = k[C_CIHEI], ciHeII = k[C_CIHEII],
                ciHeIS = k[C_CIHEIS], ciHI = k[C_CIHI], compton = k[C_COMPTON],
@@ -366,6 +440,18 @@ __device__ inline void chem_jac_physical(const double (&y)[NSP], const double (&
            1.0 * rk[C_NCRN] / Dn) +
       0.5 * 1.0 / nden * ((-H2_1) * H_1 * rk[C_H2MCOOL] + (H_1 * H_1 * H_1) * rk[C_H2MHEAT]);
   J[9][9] = adhoc * inv_md * Tge;
+}
+
+__device__ inline void chem_jac_physical(const double (&y)[NSP], const double (&k)[NRATE],
+                                         const double (&rk)[NRATE], double T, double z,
+                                         double Tge, const CellAux &ax,
+                                         double (&J)[NSP][NSP]) {
+#pragma unroll
+  for (int a = 0; a < NSP; a++)
+#pragma unroll
+    for (int b = 0; b < NSP; b++) J[a][b] = 0.0;
+  chem_jac_species(y, k, rk, Tge, J);
+  chem_jac_energy(y, k, rk, T, z, Tge, ax, J);
 }
 
 }  // namespace mvd

@@ -1,0 +1,32 @@
+// Shared declarations of the per-cell chemistry DIRK solvers (batched and plain).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "mvd_chem.hpp"
+
+namespace mvd {
+
+constexpr int MAXSTAGE = 4;   // stages of the DIRK tables supported (A is 4x4)
+constexpr int NHIST = 16;     // Newton-iteration histogram bins (last bin: >= 15)
+
+// statistics words in ChemState::counters (after the work-queue counter)
+enum { STAT_ITERS = 0, STAT_MAXIT, STAT_FAILED, STAT_SETUPS, STAT_HIST, NSTAT = STAT_HIST + NHIST };
+
+struct ChemPtrsD {
+  const double *scale, *iscale, *md, *imd, *cie, *h2o;
+  double *Ts, *dTs;
+};
+
+// Method and Newton controls, by value in kernel arguments.
+struct DirkDev {
+  int nstage;
+  double A[16];
+  double rtol, atol, ntol;
+  int max_iters, max_setups;
+  double rhs_scale;
+};
+
+int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
+                        int grid_waves, int block);
+int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st);
+
+}  // namespace mvd

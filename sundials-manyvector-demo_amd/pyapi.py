@@ -29,6 +29,24 @@ class Params(C.Structure):
                 ("length_units", C.c_double), ("time_units", C.c_double)]
 
 
+class DirkOpts(C.Structure):
+    """mvd_dirk_opts: SDIRK table (row-major 4x4 A) and Newton controls."""
+    _fields_ = [("nstage", C.c_int), ("A", C.c_double * 16), ("c", C.c_double * 4),
+                ("rtol", C.c_double), ("atol", C.c_double), ("newton_tol", C.c_double),
+                ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double)]
+
+
+class DirkStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_long), ("max_cell_iters", C.c_int),
+                ("failed_cells", C.c_long), ("setups", C.c_long), ("hist", C.c_long * 16),
+                ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {"newton_iters": self.newton_iters, "max_cell_iters": self.max_cell_iters,
+                "failed_cells": self.failed_cells, "setups": self.setups,
+                "hist": list(self.hist), "ms": self.ms}
+
+
 _P = C.POINTER(C.c_double)
 _PP = C.POINTER(_P)
 _lib = None
@@ -63,6 +81,10 @@ def lib():
         "mvd_chem_rhs": ([vp, C.c_double, _P, _P], C.c_int),
         "mvd_chem_jac": ([vp, C.c_double, _P, _P], C.c_int),
         "mvd_chem_state": ([vp, C.c_int], _P),
+        "mvd_dirk_sdirk3": ([C.POINTER(DirkOpts)], C.c_int),
+        "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
+                                C.POINTER(DirkStats)], C.c_int),
+        "mvd_chem_dirk_tune": ([vp, C.c_double, C.POINTER(DirkOpts), _P, _P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -130,7 +152,10 @@ class Context:
             self.h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:   # interpreter shutdown: the library may already be unloaded
+            pass
 
     # fe() family ------------------------------------------------------------------------
     def fe(self, w, wdot, t=0.0, plain=False, allow_illegal=False):
@@ -208,6 +233,34 @@ class Context:
         torch_mod.cuda.synchronize()
         _memcpy_d2d(out.data_ptr(), C.cast(ptr, C.c_void_p).value, n * 8)
         return out
+
+
+    # implicit chemistry step ------------------------------------------------------------
+    def dirk_step(self, y, h, opts=None, plain=False, t=0.0):
+        """One SDIRK step of every cell (y scaled, overwritten). Returns (failed, stats)."""
+        o = opts if opts is not None else sdirk3()
+        st = DirkStats()
+        rc = lib().mvd_chem_dirk_step(self.h, t, h, C.byref(o), dptr(y), 1 if plain else 0,
+                                      C.byref(st))
+        if rc < 0:
+            raise MvdError("mvd_chem_dirk_step", rc)
+        return rc, st.as_dict()
+
+    def dirk_tune(self, y, h, opts=None):
+        o = opts if opts is not None else sdirk3()
+        ms = C.c_double()
+        _check("mvd_chem_dirk_tune", lib().mvd_chem_dirk_tune(self.h, h, C.byref(o), dptr(y),
+                                                              C.byref(ms)))
+        return ms.value
+
+
+def sdirk3(**kw):
+    """Alexander's 3-stage SDIRK with the default Newton controls; keyword overrides."""
+    o = DirkOpts()
+    _check("mvd_dirk_sdirk3", lib().mvd_dirk_sdirk3(C.byref(o)))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
 
 
 def _memcpy_d2d(dst, src, nbytes):

@@ -31,6 +31,58 @@ def make_grid(nx, ny, nz, nchem=0, bc=(0,) * 6, gamma=1.4, extent=(1.0, 1.0, 1.0
     return g
 
 
+class Dirk(C.Structure):
+    """orc_dirk (same layout as the library's mvd_dirk_opts)."""
+    _fields_ = [("nstage", C.c_int), ("A", C.c_double * 16), ("c", C.c_double * 4),
+                ("rtol", C.c_double), ("atol", C.c_double), ("newton_tol", C.c_double),
+                ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double)]
+
+
+def sdirk3_table():
+    """Alexander's SDIRK3 from its defining equations, independently of the library:
+    gamma is the root in (1/6, 1/2) of x^3 - 3x^2 + 3x/2 - 1/6; c2 = (1 + gamma)/2;
+    b1 = -(6 gamma^2 - 16 gamma + 1)/4, b2 = (6 gamma^2 - 20 gamma + 5)/4, b3 = gamma."""
+    r = np.roots([1.0, -3.0, 1.5, -1.0 / 6.0])
+    g = float([x.real for x in r if abs(x.imag) < 1e-12 and 1 / 6 < x.real < 0.5][0])
+    c2 = 0.5 * (1.0 + g)
+    b1 = -(6 * g * g - 16 * g + 1) / 4
+    b2 = (6 * g * g - 20 * g + 5) / 4
+    A = np.zeros((4, 4))
+    A[0, 0] = g
+    A[1, 0], A[1, 1] = c2 - g, g
+    A[2, 0], A[2, 1], A[2, 2] = b1, b2, g
+    return A, np.array([g, c2, 1.0, 0.0])
+
+
+def dirk_opts(A=None, c=None, nstage=3, rtol=1e-5, atol=1e-9, newton_tol=0.1, max_iters=5,
+              max_setups=2, rhs_scale=1.0):
+    if A is None:
+        A, c = sdirk3_table()
+    o = Dirk()
+    o.nstage = nstage
+    for k, v in enumerate(np.asarray(A, dtype=np.float64).reshape(-1)):
+        o.A[k] = v
+    for k, v in enumerate(c):
+        o.c[k] = v
+    o.rtol, o.atol, o.newton_tol = rtol, atol, newton_tol
+    o.max_iters, o.max_setups, o.rhs_scale = max_iters, max_setups, rhs_scale
+    return o
+
+
+def copy_opts(src, cls):
+    """Field-by-field copy between the two identical ctypes layouts."""
+    o = cls()
+    for name, _ in Dirk._fields_:
+        v = getattr(src, name)
+        if name in ("A", "c"):
+            arr = getattr(o, name)
+            for k in range(len(arr)):
+                arr[k] = v[k]
+        else:
+            setattr(o, name, v)
+    return o
+
+
 _lib = None
 
 
@@ -70,6 +122,13 @@ def lib():
         L.orc_chem_opcount.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong)]
         L.orc_fslow.argtypes = [C.POINTER(Grid), C.c_void_p, C.c_double, PP, PP]
         L.orc_fslow.restype = C.c_int
+        L.orc_chem_dirk_step.argtypes = [C.c_void_p, C.POINTER(Dirk), C.c_double, P,
+                                         C.POINTER(C.c_int)]
+        L.orc_chem_dirk_step.restype = C.c_int
+        L.orc_lu.argtypes = [P, P, C.POINTER(C.c_int), P, P]
+        L.orc_lu.restype = C.c_int
+        L.orc_temperature_exit_check.argtypes = [C.c_void_p, P, P, C.POINTER(C.c_int)]
+        L.orc_temperature_exit_check.restype = C.c_long
         _lib = L
     return _lib
 
@@ -185,11 +244,39 @@ class Chem:
                                    _p(y), _p(ydot))
         return ydot
 
+    def temperature_exit_check(self, y, Ts0):
+        """(cells whose early-exit temperature differs bitwise from ten steps, steps taken)"""
+        steps = np.zeros(self.n, dtype=np.int32)
+        bad = lib().orc_temperature_exit_check(
+            self.h, _p(y), _p(np.ascontiguousarray(Ts0, dtype=np.float64)),
+            steps.ctypes.data_as(C.POINTER(C.c_int)))
+        return bad, steps
+
     def opcount(self):
         out = (C.c_ulonglong * 12)()
         lib().orc_chem_opcount(self.h, out)
         keys = ("add", "mul", "div", "sqrt", "cmp", "trans")
         return dict(zip(keys, list(out)[:6])), dict(zip(keys, list(out)[6:]))
+
+
+def dirk_step(chem, opts, h, y):
+    """One oracle DIRK step of every cell; y (n*10, scaled) is updated in place.
+    Returns (failed cells, Newton iterations per cell (-1 for failed cells))."""
+    iters = np.zeros(chem.n, dtype=np.int32)
+    failed = lib().orc_chem_dirk_step(chem.h, C.byref(opts), h, _p(y),
+                                      iters.ctypes.data_as(C.POINTER(C.c_int)))
+    return failed, iters
+
+
+def lu(M, b):
+    """Oracle 10x10 LU with partial pivoting: returns (LU row-major, piv, x)."""
+    M = np.ascontiguousarray(M, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    LU = np.zeros(100)
+    x = np.zeros(10)
+    piv = np.zeros(10, dtype=np.int32)
+    lib().orc_lu(_p(M), _p(LU), piv.ctypes.data_as(C.POINTER(C.c_int)), _p(b), _p(x))
+    return LU.reshape(10, 10), piv, x
 
 
 def fslow(g, chem, energy_units, w):

@@ -1,0 +1,120 @@
+"""Batched and plain GPU SDIRK chemistry steps through the C ABI against the CPU oracle.
+
+Both GPU paths and the oracle run the same per-cell algorithm (same RHS evaluation sequence,
+same Newton updates and convergence test, same LU operation order, no FMA contraction), so
+they agree far inside the 1e-8 relative bar for integrated states: the tolerance below is
+1e-10, per species, relative to the largest magnitude of that species over the batch, on
+the cells that converged. Failed cells (Newton budget exhausted; the integrator would reject
+the step) must be the same number on every path.
+"""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from conftest import PKG, normwise_err
+
+spec = importlib.util.spec_from_file_location("mvd_problems", os.path.join(PKG, "problems.py"))
+problems = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(problems)
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def setup(mvd, torch, n, seed):
+    y, _ = problems.chem_cells(n, seed=seed)
+    oc = orc.Chem(n)
+    yo = y.copy()
+    oc.prepare(yo)
+    oc.extra()
+    ctx = mvd.Context(mvd.make_params(4, 4, 4, 10))
+    ctx.chem_setup(orc.load_tables(), n)
+    yd = torch.from_numpy(y.copy()).to("cuda")
+    ctx.chem_prepare(yd)
+    return oc, yo, ctx, yd
+
+
+def compare(got, ref, ok):
+    g = got.reshape(-1, 10)[ok]
+    r = ref.reshape(-1, 10)[ok]
+    return max(normwise_err(g[:, s], r[:, s]) for s in range(10))
+
+
+@pytest.mark.parametrize("n,seed,h", [(4096, 1, 1e3), (1000, 2, 1e3), (1, 3, 1e3),
+                                      (4096, 4, 1e5), (2048, 5, 1e8)])
+def test_dirk_batched_and_plain_match_oracle(mvd, torch_gpu, n, seed, h):
+    torch = torch_gpu
+    print(f"[dirk parity] n={n} seed={seed} h={h}", flush=True)
+    oc, yo, ctx, yd = setup(mvd, torch, n, seed)
+    _, _, ctx_p, y_plain = setup(mvd, torch, n, seed)      # same start for the plain path
+    failed_o, iters_o = orc.dirk_step(oc, orc.dirk_opts(), h, yo)
+    ok = iters_o >= 0
+    failed_b, st_b = ctx.dirk_step(yd, h)
+    Ts_b = ctx.chem_state("Ts", torch).cpu().numpy()
+    failed_p, st_p = ctx_p.dirk_step(y_plain, h, plain=True)
+    assert failed_b == failed_o == failed_p
+    assert st_b["failed_cells"] == failed_o
+    yb = yd.cpu().numpy()
+    yp = y_plain.cpu().numpy()
+    assert compare(yb, yo, ok) <= TOL
+    assert compare(yp, yo, ok) <= TOL
+    assert np.max(np.abs(Ts_b[ok] - oc.array("Ts")[ok]) / oc.array("Ts")[ok]) <= TOL
+    # the Newton work on the converged cells is the oracle's (iteration histogram)
+    hist_o = np.bincount(np.minimum(iters_o[ok], 15), minlength=16)
+    if failed_o == 0:
+        assert st_b["newton_iters"] == int(iters_o.sum())
+        assert st_p["newton_iters"] == int(iters_o.sum())
+        assert list(hist_o) == st_b["hist"] == st_p["hist"]
+    assert st_b["setups"] == st_p["setups"] >= n
+    assert st_b["max_cell_iters"] == st_p["max_cell_iters"]
+
+
+def test_dirk_reports_newton_failure(mvd, torch_gpu):
+    torch = torch_gpu
+    n = 777
+    print(f"[dirk failure] n={n}", flush=True)
+    _, _, ctx, yd = setup(mvd, torch, n, 6)
+    o = mvd.sdirk3(max_iters=1, max_setups=0, newton_tol=1e-30)
+    for plain in (False, True):
+        y = yd.clone()
+        failed, st = ctx.dirk_step(y, 1e3, opts=o, plain=plain)
+        assert failed == n and st["failed_cells"] == n
+        assert st["newton_iters"] == n and st["setups"] == n
+
+
+def test_dirk_rejects_bad_options(mvd, torch_gpu):
+    torch = torch_gpu
+    _, _, ctx, yd = setup(mvd, torch, 64, 7)
+    o = mvd.sdirk3()
+    o.A[5] = 0.3                                      # unequal diagonal: not SDIRK
+    with pytest.raises(mvd.MvdError) as e:
+        ctx.dirk_step(yd, 1e3, opts=o)
+    assert e.value.code == mvd.MVD_UNSUPPORTED
+    o = mvd.sdirk3()
+    o.A[1] = 0.1                                      # above the diagonal
+    with pytest.raises(mvd.MvdError) as e:
+        ctx.dirk_step(yd, 1e3, opts=o)
+    assert e.value.code == mvd.MVD_BAD_ARGUMENT
+    with pytest.raises(mvd.MvdError) as e:
+        ctx.dirk_step(yd, -1.0)
+    assert e.value.code == mvd.MVD_BAD_ARGUMENT
+
+
+def test_dirk_tune_keeps_results(mvd, torch_gpu):
+    """The autotuner times launch shapes on a scratch copy and restores the temperatures:
+    a step after tuning equals a step without it."""
+    torch = torch_gpu
+    n = 20000
+    print(f"[dirk tune] n={n}", flush=True)
+    _, _, ctx, yd = setup(mvd, torch, n, 8)
+    _, _, ctx2, yd2 = setup(mvd, torch, n, 8)
+    ms = ctx.dirk_tune(yd, 1e3)
+    assert ms > 0
+    f1, s1 = ctx.dirk_step(yd, 1e3)
+    f2, s2 = ctx2.dirk_step(yd2, 1e3)
+    assert f1 == f2 == 0
+    assert torch.equal(yd, yd2)
+    assert s1["newton_iters"] == s2["newton_iters"]

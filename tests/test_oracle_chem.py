@@ -213,3 +213,27 @@ def test_stateful_warm_start():
     T2 = c.array("Ts").copy()
     assert np.allclose(T1, T2, rtol=1e-10)
     assert not np.all(T1 == 1000.0)
+
+
+def test_temperature_early_exit_is_exact():
+    """The device temperature solve leaves its ten Newton steps at a bitwise fixed point or
+    2-cycle of the step map. Restated in the oracle, that early form must give bitwise the
+    (T, dT/dge) of the ten steps, from cold starts, converged warm starts (where 2-cycles
+    appear) and slightly perturbed warm starts."""
+    n = 4096
+    y, T_true = problems.chem_cells(n, seed=21)
+    ch = orc.Chem(n)
+    ys = y.copy()
+    ch.prepare(ys)
+    ch.extra()
+    starts = [np.full(n, 1000.0), T_true]
+    ch.rhs(ys)
+    Tw = ch.array("Ts").copy()
+    starts += [Tw, Tw * (1 + 1e-12), Tw * (1 - 1e-9), Tw * 1.01]
+    mean_steps = []
+    for Ts0 in starts:
+        bad, steps = ch.temperature_exit_check(ys, Ts0)
+        assert bad == 0
+        mean_steps.append(steps.mean())
+    # converged warm starts settle within a few steps
+    assert mean_steps[2] < 4.0, mean_steps
