@@ -49,13 +49,7 @@ def kernel_flops(name, n, nchem=0):
     nv = 5 + nchem
     faces = (n + 1) * n * n
     cells = n ** 3
-    if name in ("fe_xface",):
-        return faces * ff
-    if name == "fe_march_y":
-        return faces * ff + cells * nv * 6        # x and y terms of the divergence
-    if name == "fe_march_z":
-        return faces * ff + cells * nv * 3
-    if name == "fe_yface":
+    if name in ("fe_xface", "fe_yface"):
         return faces * ff
     if name == "fe_zface_div":
         return faces * ff + cells * nv * DIV_FLOPS_PER_VAR
@@ -94,6 +88,105 @@ def cpu_baseline(n_sample, threads):
     return n_sample ** 3 / dt / 1e9, dt
 
 
+def chem_opcounts():
+    """FP64 operations of one cell's chemistry RHS and Jacobian, counted by the oracle's
+    op-counting scalar (additions, multiplications, divisions, square roots; comparisons
+    and transcendental calls are reported apart and not counted as flops)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    problems = _load("mvd_problems", os.path.join(PKG, "problems.py"))
+    y, _ = problems.chem_cells(1, seed=5)
+    ch = orc.Chem(1)
+    ys = y.copy()
+    ch.prepare(ys)
+    ch.extra()
+    rhs, jac = ch.opcount()
+    fl = lambda d: d["add"] + d["mul"] + d["div"] + d["sqrt"]
+    return fl(rhs), fl(jac), rhs["trans"]
+
+
+def bench_chem(api, problems, torch, n, seed, steps, h=1e3):
+    """Config-3 batched chemistry: one SDIRK3 step of dt = 1e3 s over n independent cells,
+    batched persistent kernel vs the plain one-cell-per-thread kernels. Every timed step
+    starts from the same state (y and the temperature warm start are restored outside the
+    timed region); the device time of the step comes from HIP events around it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    y, _ = problems.chem_cells(n, seed=seed)
+    ctx = api.Context(api.make_params(4, 4, 4, 10))
+    ctx.chem_setup(orc.load_tables(), n)
+    y0 = torch.from_numpy(y).to("cuda")
+    ctx.chem_prepare(y0)
+    Ts0 = ctx.chem_state("Ts", torch)
+    dT0 = ctx.chem_state("dTs_ge", torch)
+    opts = api.sdirk3()
+    yy = y0.clone()
+    tuned_ms = ctx.dirk_tune(yy, h, opts)          # launch-shape autotuner (untimed)
+
+    def run(plain, k):
+        ms, st = [], None
+        for _ in range(k):
+            ctx.chem_state_set("Ts", Ts0)
+            ctx.chem_state_set("dTs_ge", dT0)
+            yy.copy_(y0)
+            torch.cuda.synchronize()
+            failed, st = ctx.dirk_step(yy, h, opts, plain=plain)
+            assert failed == 0
+            ms.append(st["ms"])
+        return float(np.mean(ms)), st
+
+    run(False, 2)
+    run(True, 1)
+    b_ms, b_st = run(False, steps)
+    p_ms, p_st = run(True, max(2, steps // 2))
+    assert b_st["newton_iters"] == p_st["newton_iters"] and b_st["hist"] == p_st["hist"]
+    rhs_fl, jac_fl, rhs_trans = chem_opcounts()
+    # RHS evaluations: one per Newton iteration plus one before each Jacobian setup
+    nrhs = b_st["newton_iters"] + b_st["setups"]
+    lu_fl = 2 * 10 ** 3 // 3 + 10 ** 2            # factor, per setup (approx.)
+    solve_fl = 2 * 10 ** 2                        # triangular solves, per iteration
+    flops = nrhs * rhs_fl + b_st["setups"] * (jac_fl + lu_fl) + b_st["newton_iters"] * solve_fl
+    out = {
+        "workload": f"{round(n ** (1 / 3))}^3 = {n} independent primordial-chemistry cells, "
+                    f"one SDIRK3 step dt={h:g} s, seed {seed}",
+        "batched_ms": b_ms, "plain_ms": p_ms,
+        "batched_cells_per_s": n / (b_ms * 1e-3), "plain_cells_per_s": n / (p_ms * 1e-3),
+        "speedup_batched_vs_plain": p_ms / b_ms,
+        "newton_iters": b_st["newton_iters"], "max_cell_iters": b_st["max_cell_iters"],
+        "setups": b_st["setups"], "newton_hist": b_st["hist"],
+        "rhs_evals": nrhs, "fp64_flops": flops,
+        "batched_fp64_tflops": flops / (b_ms * 1e-3) / 1e12,
+        "batched_fp64_frac": flops / (b_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+        "tuned_best_ms": tuned_ms,
+        "op_count_per_cell": {"rhs_flops": rhs_fl, "rhs_transcendentals": rhs_trans,
+                              "jac_flops": jac_fl},
+    }
+    ctx.close()
+    return out
+
+
+def cpu_chem_baseline(n_sample, seed, threads, reps, h=1e3):
+    """The CPU oracle's per-cell SDIRK3 step on config-3 cells, `reps` times from the same
+    initial state; returns cells/s and the total seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    problems = _load("mvd_problems", os.path.join(PKG, "problems.py"))
+    y, _ = problems.chem_cells(n_sample, seed=seed)
+    orc.set_threads(threads)
+    total = 0.0
+    for _ in range(reps):
+        ch = orc.Chem(n_sample)
+        ys = y.copy()
+        ch.prepare(ys)
+        ch.extra()
+        t0 = time.perf_counter()
+        failed, _ = orc.dirk_step(ch, orc.dirk_opts(), h, ys)
+        total += time.perf_counter() - t0
+        assert failed == 0
+        del ch
+    return n_sample * reps / total, total
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,8 +195,17 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--plain-steps", type=int, default=5)
-    ap.add_argument("--cpu-n", type=int, default=160)
+    ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--chem-n", type=int, default=64 ** 3)
+    ap.add_argument("--chem-steps", type=int, default=10)
+    ap.add_argument("--chem-cpu-n", type=int, default=64 ** 3)
+    ap.add_argument("--chem-cpu-reps", type=int, default=8)
+    ap.add_argument("--no-chem", action="store_true")
+    ap.add_argument("--profile-fe", action="store_true",
+                    help="for hardware-counter runs: --steps fe() evaluations, then two "
+                         "copies of a 1 GiB device buffer (the known-size calibration "
+                         "kernel), and nothing else")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -122,6 +224,19 @@ def main():
     w = [torch.from_numpy(a).to("cuda") for a in problems.blast3d(n, seed=args.seed)] + [None]
     wdot = [torch.empty_like(a) for a in w[:5]] + [None]
     ctx = api.Context(api.make_params(n, n, n, 0, (0,) * 6, 1.4))
+
+    if args.profile_fe:
+        for _ in range(args.steps):
+            ctx.fe(w, wdot)
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_fe": args.steps, "copy_bytes": a.numel() * 8}))
+        ctx.close()
+        return
 
     for _ in range(args.warmup):
         ctx.fe(w, wdot)
@@ -180,19 +295,33 @@ def main():
         "speedup_vs_plain_gpu": plain_ms / ms_per_step,
     }
 
+    if not args.no_chem:
+        extra["chem"] = bench_chem(api, problems, torch, args.chem_n, 4321, args.chem_steps)
+
     cpu = None
     if not args.no_cpu:
         v1, dt1 = cpu_baseline(args.cpu_n, 1)
         cpu = {"value": v1, "unit": "Gcell/s", "cores": 1, "kind": "port",
                "sample": f"one fe() of the blast IC at {args.cpu_n}^3 (periodic, nchem=0), "
-                         f"oracle restatement of fEuler, -O2 no FMA, {dt1:.1f} s"}
+                         f"oracle restatement of fEuler (serial, no profiler calls), -O2 "
+                         f"without FMA contraction, {dt1:.1f} s"}
         nthr = min(16, os.cpu_count() or 1)
         vN, dtN = cpu_baseline(args.cpu_n, nthr)
         extra["cpu_openmp"] = {"value": vN, "unit": "Gcell/s", "cores": nthr, "seconds": dtN}
         extra["speedup_vs_cpu_1core"] = gcells / v1
+        if not args.no_chem:
+            c1, cdt = cpu_chem_baseline(args.chem_cpu_n, 4321, 1, args.chem_cpu_reps)
+            extra["chem"]["cpu_baseline"] = {
+                "value": c1, "unit": "cells/s", "cores": 1, "kind": "port",
+                "sample": f"oracle SDIRK3 step of the {args.chem_cpu_n} config-3 cells, "
+                          f"{args.chem_cpu_reps} times, {cdt:.1f} s"}
+            cN, cdtN = cpu_chem_baseline(args.chem_cpu_n, 4321, nthr, 1)
+            extra["chem"]["cpu_openmp"] = {"value": cN, "unit": "cells/s", "cores": nthr,
+                                           "seconds": cdtN}
 
     line = {
-        "metric": "fe() Gcell/s @256^3 FP64",
+        "metric": "fe() Gcell/s @256\u00b3 FP64; batched chem cells/s vs per-cell; "
+                  "VJP/fwd cost ratio",
         "value": gcells,
         "unit": "Gcell/s",
         "n_gpus": 1,

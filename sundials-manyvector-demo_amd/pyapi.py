@@ -235,6 +235,18 @@ class Context:
         return out
 
 
+    def chem_state_set(self, which, src):
+        """Overwrite a per-cell state array (see chem_state) from a torch device tensor."""
+        names = {"scale": 0, "inv_scale": 1, "Ts": 2, "dTs_ge": 3, "mdensity": 4,
+                 "inv_mdensity": 5, "cie_oda": 6, "h2_oda": 7}
+        w = names[which]
+        n = self.ncell * (10 if w in (0, 1) else 1)
+        assert src.numel() == n and src.dtype.itemsize == 8 and src.is_contiguous()
+        ptr = lib().mvd_chem_state(self.h, w)
+        if not ptr:
+            raise MvdError("mvd_chem_state", MVD_NOT_READY)
+        _memcpy_d2d(C.cast(ptr, C.c_void_p).value, src.data_ptr(), n * 8)
+
     # implicit chemistry step ------------------------------------------------------------
     def dirk_step(self, y, h, opts=None, plain=False, t=0.0):
         """One SDIRK step of every cell (y scaled, overwritten). Returns (failed, stats)."""
