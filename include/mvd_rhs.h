@@ -103,6 +103,14 @@ int mvd_chem_rhs(mvd_context *ctx, double t, const double *y, double *ydot);
  * 10x10 blocks J[100*cell + 10*col + row] (DENSEIDX, line 40), evaluated from the
  * temperature of the last mvd_chem_rhs call. */
 int mvd_chem_jac(mvd_context *ctx, double t, const double *y, double *J);
+/* Vector-Jacobian product of calculate_rhs_cvklu (no counterpart in the reference, which
+ * has only the forward Jacobian callback; the adjoint callers are sensitivity and
+ * checkpointed-adjoint drivers): ybar = (d ydot / d y)^T v in the scaled variables, for the
+ * exact derivative of the RHS as evaluated (temperature = root of the temperature
+ * equation, so dT/dy enters through dT/dy = -(dg/dy)/(dg/dT); rates piecewise linear in
+ * log T). Updates Ts/dTs like mvd_chem_rhs; ydot (may be NULL) receives the RHS itself. */
+int mvd_chem_vjp(mvd_context *ctx, double t, const double *y, const double *v, double *ybar,
+                 double *ydot);
 /* device pointers of per-cell chemistry state: 0 scale, 1 inv_scale, 2 Ts, 3 dTs_ge,
  * 4 mdensity, 5 inv_mdensity, 6 cie_optical_depth, 7 h2_optical_depth */
 double *mvd_chem_state(mvd_context *ctx, int which);

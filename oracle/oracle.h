@@ -26,6 +26,8 @@ int orc_chem_dirk_step(void *chem, const orc_dirk *opt, double h, double *y, int
 int orc_lu(const double *M, double *LU, int *piv, const double *b, double *x);
 /* bitwise check of the early exit of the device temperature solve (orc_chem.cpp) */
 long orc_temperature_exit_check(void *chem, const double *y, const double *Ts0, int *steps);
+/* vector-Jacobian product of the chemistry RHS (orc_chem.cpp) */
+int orc_chem_vjp(void *chem, const double *y, const double *v, double *ybar, int fixed_T);
 
 void orc_set_threads(int n);
 int orc_max_threads(void);

@@ -250,3 +250,115 @@ extern "C" long orc_temperature_exit_check(void *h, const double *y, const doubl
   }
   return bad;
 }
+
+// ---- vector-Jacobian product of the RHS (forward mode, the full Jacobian per cell) ----
+namespace {
+
+constexpr int ND = NSP + 1;   // tangents: the 10 physical inputs, then T
+
+struct DualN {
+  double v;
+  double d[ND];
+  DualN() : v(0) { for (double &x : d) x = 0; }
+  DualN(double x) : v(x) { for (double &q : d) q = 0; }
+};
+inline DualN operator+(const DualN &a, const DualN &b) {
+  DualN r(a.v + b.v);
+  for (int i = 0; i < ND; i++) r.d[i] = a.d[i] + b.d[i];
+  return r;
+}
+inline DualN operator-(const DualN &a, const DualN &b) {
+  DualN r(a.v - b.v);
+  for (int i = 0; i < ND; i++) r.d[i] = a.d[i] - b.d[i];
+  return r;
+}
+inline DualN operator-(const DualN &a) {
+  DualN r(-a.v);
+  for (int i = 0; i < ND; i++) r.d[i] = -a.d[i];
+  return r;
+}
+inline DualN operator*(const DualN &a, const DualN &b) {
+  DualN r(a.v * b.v);
+  for (int i = 0; i < ND; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+  return r;
+}
+inline DualN operator/(const DualN &a, const DualN &b) {
+  DualN r(a.v / b.v);
+  for (int i = 0; i < ND; i++) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v);
+  return r;
+}
+inline DualN pow(const DualN &a, double e) {
+  DualN r(std::pow(a.v, e));
+  const double s = e * std::pow(a.v, e - 1.0);
+  for (int i = 0; i < ND; i++) r.d[i] = s * a.d[i];
+  return r;
+}
+inline DualN log(const DualN &a) {
+  DualN r(std::log(a.v));
+  for (int i = 0; i < ND; i++) r.d[i] = a.d[i] / a.v;
+  return r;
+}
+
+}  // namespace
+
+// ybar = (d ydot/d y)^T v for calculate_rhs_cvklu in the scaled variables (the function
+// mvd_chem_rhs evaluates), per cell: the physical RHS in forward mode with tangents along
+// the ten inputs and T, the temperature equation g(T, y) = T kb nsum(T, y)/(density mh) - ge
+// likewise (its T-slope from the interpolated gamma tables), dT/dy = -(dg/dy)/(dg/dT).
+// fixed_T != 0 drops the temperature path (the derivative at fixed rates). Temperatures are
+// solved from the warm start and stored, as orc_chem_rhs does.
+extern "C" int orc_chem_vjp(void *h, const double *y, const double *v, double *ybar,
+                            int fixed_T) {
+  Chem *c = (Chem *)h;
+  const TableGrid &g = c->g;
+  for (long i = 0; i < c->n; i++) {
+    const long j = NSP * i;
+    double ya[NSP];
+    for (int s = 0; s < NSP; s++) ya[s] = y[j + s] * c->scale[j + s];
+    double T = c->Ts[i], dT = 0.0;
+    temperature(g, ya, T, dT);
+    c->Ts[i] = T;
+    c->dTs_ge[i] = dT;
+    double rs[NRATE], drs[NRATE];
+    interp_rates(g, T, rs, drs);
+    DualN yd[NSP], kd[NRATE], f[NSP];
+    for (int s = 0; s < NSP; s++) { yd[s] = DualN(ya[s]); yd[s].d[s] = 1.0; }
+    for (int r = 0; r < NRATE; r++) { kd[r] = DualN(rs[r]); kd[r].d[NSP] = drs[r]; }
+    DualN Td(T);
+    Td.d[NSP] = 1.0;
+    rhs_physical(yd, kd, Td, DualN(c->z), aux_of(c, i), f);
+    double dTdy[NSP] = {};
+    const bool clamped = !(T > g.tmin && T < g.tmax);
+    if (!fixed_T && !clamped) {
+      // g(T, y) with the interpolated gamma_H2 tables
+      const double kb = 1.3806504e-16, mh = 1.67e-24, gm1 = 1.0 / (5.0 / 3.0 - 1.0);
+      const int b = table_bin(g, std::log(T));
+      const double t1 = g.lb + double(b) * g.dbin, t2 = g.lb + double(b + 1) * g.dbin;
+      const DualN Tdef = (log(Td) - DualN(t1)) / DualN(t2 - t1);
+      auto lin = [&](int tab) {
+        const double *a = g.tab + tab * TLEN;
+        return DualN(a[b]) + Tdef * DualN(a[b + 1] - a[b]);
+      };
+      const DualN g1 = DualN(1.0) / (lin(G_GH2_1) - DualN(1.0));
+      const DualN g2 = DualN(1.0) / (lin(G_GH2_2) - DualN(1.0));
+      const DualN nsum = yd[0] * g1 + yd[1] * g2 +
+                         DualN(gm1) * (yd[2] + yd[3] + yd[4] + yd[5] + yd[6] + yd[7] + yd[8]);
+      const DualN density = DualN(2.0) * (yd[0] + yd[1]) +
+                            DualN(1.0079400000000001) * (yd[2] + yd[3] + yd[4]) +
+                            DualN(4.0026020000000004) * (yd[5] + yd[6] + yd[7]);
+      const DualN gg = Td * DualN(kb) * nsum / (density * DualN(mh)) - yd[9];
+      for (int q = 0; q < NSP; q++) dTdy[q] = -gg.d[q] / gg.d[NSP];
+    }
+    for (int q = 0; q < NSP; q++) {
+      double acc = 0.0;
+      for (int s = 0; s < NSP; s++) {
+        const double Jsq = f[s].d[q] + f[s].d[NSP] * dTdy[q];     // physical d f_s / d y_q
+        double w = v[j + s] * c->inv_scale[j + s];
+        if (s == NSP - 1) w *= c->inv_mdensity[i];
+        acc += w * Jsq;
+      }
+      ybar[j + q] = acc * c->scale[j + q];
+    }
+  }
+  return 0;
+}

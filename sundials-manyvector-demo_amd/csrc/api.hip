@@ -271,6 +271,13 @@ int mvd_chem_rhs(mvd_context *ctx, double, const double *y, double *ydot) {
   return mvd::chem_rhs(ctx->chem, y, ydot, ctx->stream, true);
 }
 
+int mvd_chem_vjp(mvd_context *ctx, double, const double *y, const double *v, double *ybar,
+                 double *ydot) {
+  if (!ctx || !y || !v || !ybar) return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  return mvd::chem_vjp(ctx->chem, y, v, ybar, ydot, ctx->stream, true);
+}
+
 int mvd_chem_jac(mvd_context *ctx, double, const double *y, double *J) {
   if (!ctx || !y || !J) return MVD_BAD_ARGUMENT;
   if (!ctx->chem.ready) return MVD_NOT_READY;

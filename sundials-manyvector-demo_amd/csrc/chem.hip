@@ -82,6 +82,43 @@ chem_rhs_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *ydot
   for (int v = 0; v < NSP; v++) ydot[NSP * i + v] = f[v];
 }
 
+// Vector-Jacobian product of calculate_rhs_cvklu in the scaled variables:
+// ybar = (d ydot / d y)^T v, with the temperature a function of y (solved from the warm start
+// like the RHS, which it also updates); optionally ydot itself.
+__global__ void __launch_bounds__(256)
+chem_vjp_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, const double *v,
+                double *ybar, double *ydot) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double y[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) y[s] = yd[NSP * i + s] * p.scale[NSP * i + s];
+  double Ts = p.Ts[i], dTs;
+  chem_temperature(cc, y, Ts, dTs);
+  p.Ts[i] = Ts;
+  p.dTs[i] = dTs;
+  const CellAux ax{p.md[i], p.imd[i], p.cie[i], p.h2o[i]};
+  if (ydot) {
+    double k[NRATE], f[NSP];
+    chem_rates_only(cc, Ts, k);
+    chem_rhs_physical(y, k, Ts, cc.z, ax, f);
+#pragma unroll
+    for (int s = 0; s < NSP; s++) f[s] *= p.iscale[NSP * i + s];
+    f[9] *= ax.inv_mdensity;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) ydot[NSP * i + s] = f[s];
+  }
+  double vp[NSP], yb[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) vp[s] = v[NSP * i + s] * p.iscale[NSP * i + s];
+  vp[9] *= ax.inv_mdensity;
+  const bool clamped = !(Ts > cc.tmin && Ts < cc.tmax);
+  chem_vjp_physical(cc, y, Ts, clamped, ax, vp, yb);
+#pragma unroll
+  for (int s = 0; s < NSP; s++) ybar[NSP * i + s] = yb[s] * p.scale[NSP * i + s];
+}
+
 // calculate_jacobian_cvklu (raja_primordial_network.cpp:1393-1965): dense column-major
 // blocks, each entry times inv_scale[row]*scale[col].
 __global__ void __launch_bounds__(256)
@@ -186,6 +223,16 @@ int chem_jac(ChemState &s, const double *y, double *J, hipStream_t st, bool sync
   if (hipGetLastError() != hipSuccess) return MVD_DEVICE_ERROR;
   if (!sync) return MVD_SUCCESS;
   return hipStreamSynchronize(st) == hipSuccess ? MVD_SUCCESS : MVD_DEVICE_ERROR;
+}
+
+int chem_vjp(ChemState &s, const double *y, const double *v, double *ybar, double *ydot,
+             hipStream_t st, bool sync) {
+  if (!s.ready) return MVD_NOT_READY;
+  hipLaunchKernelGGL(chem_vjp_kernel, dim3(nb(s.n)), dim3(256), 0, st, s.n, s.cc, ptrs_of(s), y,
+                     v, ybar, ydot);
+  if (hipGetLastError() != hipSuccess) return MVD_DEVICE_ERROR;
+  if (sync && hipStreamSynchronize(st) != hipSuccess) return MVD_DEVICE_ERROR;
+  return MVD_SUCCESS;
 }
 
 double *chem_state_ptr(ChemState &s, int which) {

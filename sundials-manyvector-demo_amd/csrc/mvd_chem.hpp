@@ -52,6 +52,8 @@ int chem_setup(ChemState &s, const double *tables_host, long n, double z, hipStr
 int chem_prepare(ChemState &s, double *chem, hipStream_t st);
 int chem_rhs(ChemState &s, const double *y, double *ydot, hipStream_t st, bool sync);
 int chem_jac(ChemState &s, const double *y, double *J, hipStream_t st, bool sync);
+int chem_vjp(ChemState &s, const double *y, const double *v, double *ybar, double *ydot,
+             hipStream_t st, bool sync);
 double *chem_state_ptr(ChemState &s, int which);
 void chem_free(ChemState &s);
 
@@ -261,15 +263,57 @@ __device__ inline void chem_rhs_species(const double (&y)[NSP], const double (&k
          R.r18;
 }
 
-__device__ inline double chem_rhs_energy(const double (&y)[NSP], const double (&k)[NRATE],
-                                         double T, double z, const CellAux &ax) {
+// Scalar with one tangent (forward-mode derivative), for the gradient of the gas-energy
+// row in the vector-Jacobian product; all operations without FMA contraction.
+struct Dual1 {
+  double v, d;
+};
+__device__ inline Dual1 operator+(Dual1 a, Dual1 b) {
 #pragma clang fp contract(off)
-  const double H2_1 = y[0], H_1 = y[2], H_2 = y[3], He_1 = y[5], He_2 = y[6], He_3 = y[7],
-               de = y[8];
+  return {a.v + b.v, a.d + b.d};
+}
+__device__ inline Dual1 operator-(Dual1 a, Dual1 b) {
+#pragma clang fp contract(off)
+  return {a.v - b.v, a.d - b.d};
+}
+__device__ inline Dual1 operator-(Dual1 a) { return {-a.v, -a.d}; }
+__device__ inline Dual1 operator*(Dual1 a, Dual1 b) {
+#pragma clang fp contract(off)
+  return {a.v * b.v, a.d * b.v + a.v * b.d};
+}
+__device__ inline Dual1 operator/(Dual1 a, Dual1 b) {
+#pragma clang fp contract(off)
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+__device__ inline Dual1 operator+(double a, Dual1 b) { return {a + b.v, b.d}; }
+__device__ inline Dual1 operator+(Dual1 a, double b) { return {a.v + b, a.d}; }
+__device__ inline Dual1 operator-(double a, Dual1 b) { return {a - b.v, -b.d}; }
+__device__ inline Dual1 operator-(Dual1 a, double b) { return {a.v - b, a.d}; }
+__device__ inline Dual1 operator*(double a, Dual1 b) {
+#pragma clang fp contract(off)
+  return {a * b.v, a * b.d};
+}
+__device__ inline Dual1 operator*(Dual1 a, double b) {
+#pragma clang fp contract(off)
+  return {a.v * b, a.d * b};
+}
+__device__ inline Dual1 operator/(double a, Dual1 b) {
+#pragma clang fp contract(off)
+  const double q = a / b.v;
+  return {q, (-q * b.d) / b.v};
+}
+
+template <class R>
+__device__ inline R chem_rhs_energy_t(const R (&y)[NSP], const R (&k)[NRATE], R T, double z,
+                                      const CellAux &ax) {
+#pragma clang fp contract(off)
+  const R H2_1 = y[0], H_1 = y[2], H_2 = y[3], He_1 = y[5], He_2 = y[6], He_3 = y[7],
+          de = y[8];
   const double cie = ax.cie, h2o = ax.h2o;
-  const double gS = H2_1 * k[C_GAH2] + H_1 * k[C_GAHI] + H_2 * k[C_GAHP] + He_1 * k[C_GAHE] +
-                    de * k[C_GAEL];
-  const double ncrD = H2_1 * k[C_NCRD2] + H_1 * k[C_NCRD1];
+  const R gS = H2_1 * k[C_GAH2] + H_1 * k[C_GAHI] + H_2 * k[C_GAHP] + He_1 * k[C_GAHE] +
+               de * k[C_GAEL];
+  const R ncrD = H2_1 * k[C_NCRD2] + H_1 * k[C_NCRD1];
   const double zp1 = z + 1.0;
   const double zp4 = (zp1 * zp1) * (zp1 * zp1);
   return -2.0158800000000001 * H2_1 * k[C_CIECO] * cie * ax.mdensity -
@@ -283,6 +327,117 @@ __device__ inline double chem_rhs_energy(const double (&y)[NSP], const double (&
          cie * k[C_COMPTON] * de * zp4 * (T - 2.73 * z - 2.73) +
          0.5 * 1.0 / (k[C_NCRN] / ncrD + 1.0) *
              ((-H2_1) * H_1 * k[C_H2MCOOL] + (H_1 * H_1 * H_1) * k[C_H2MHEAT]);
+}
+
+__device__ inline double chem_rhs_energy(const double (&y)[NSP], const double (&k)[NRATE],
+                                         double T, double z, const CellAux &ax) {
+  return chem_rhs_energy_t<double>(y, k, T, z, ax);
+}
+
+// ---- vector-Jacobian product of the RHS --------------------------------------------
+
+// Reactions in the order of Reactions (r01..r19, r21, r22): rate index (the same position in
+// the rate tables) and reactant species (-1: none); stoichiometry of the nine species rows
+// (f_s = sum_j CHEM_STO[s][j] r_j), read off chem_rhs_species.
+constexpr int NREACT = 21;
+__device__ constexpr signed char CHEM_REACTANTS[NREACT][3] = {
+    {2, 8, -1}, {3, 8, -1}, {5, 8, -1}, {6, 8, -1}, {6, 8, -1}, {7, 8, -1}, {2, 8, -1},
+    {2, 4, -1}, {2, 3, -1}, {1, 2, -1}, {0, 3, -1}, {0, 8, -1}, {0, 2, -1}, {4, 8, -1},
+    {2, 4, -1}, {3, 4, -1}, {3, 4, -1}, {1, 8, -1}, {1, 4, -1}, {0, 2, 2}, {2, 2, 2}};
+__device__ constexpr signed char CHEM_STO[NSP - 1][NREACT] = {
+    // r01 02 03 04 05 06 07 08 09 10 11 12 13 14 15 16 17 18 19 21 22
+    {0, 0, 0, 0, 0, 0, 0, 1, 0, 1, -1, -1, -1, 0, 0, 0, 0, 0, 1, 1, 1},      // H2_1
+    {0, 0, 0, 0, 0, 0, 0, 0, 1, -1, 1, 0, 0, 0, 0, 0, 1, -1, -1, 0, 0},      // H2_2
+    {-1, 1, 0, 0, 0, 0, -1, -1, -1, -1, 1, 2, 2, 1, 1, 2, 0, 2, 1, -2, -2},  // H_1
+    {1, -1, 0, 0, 0, 0, 0, 0, -1, 1, -1, 0, 0, 0, 0, -1, -1, 0, 0, 0, 0},    // H_2
+    {0, 0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 0, -1, -1, -1, -1, 0, -1, 0, 0},   // H_m0
+    {0, 0, -1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},        // He_1
+    {0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},       // He_2
+    {0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},        // He_3
+    {1, -1, 1, -1, 1, -1, -1, 1, 0, 0, 0, 0, 0, 1, 1, 0, 1, -1, 0, 0, 0}};   // de
+
+// Gradient of the physical RHS contracted with vp: yab = (df/dy)^T vp for y (physical
+// units) with the temperature T = T(y), the root of the temperature equation
+// g(T, y) = T kb nsum(T, y) / (density(y) mh) - ge = 0 (so dT/dy = -(dg/dy) / (dg/dT), with
+// dg/dT from the slopes of the interpolated gamma tables). The species rows are
+// differentiated in reverse through the reactions, the gas-energy row in forward mode
+// along each species it depends on and along T. `clamped`: T sits on a clamp bound, so it
+// does not depend on y.
+__device__ inline void chem_vjp_physical(const ChemConst &c, const double (&y)[NSP], double T,
+                                         bool clamped, const CellAux &ax,
+                                         const double (&vp)[NSP], double (&yab)[NSP]) {
+#pragma clang fp contract(off)
+  double k[NRATE], dk[NRATE];
+  chem_rates(c, T, k, dk);
+#pragma unroll
+  for (int s = 0; s < NSP; s++) yab[s] = 0.0;
+  double Tb = 0.0;
+  // species rows: reverse through r_j = k_j * prod(reactants)
+#pragma unroll
+  for (int j = 0; j < NREACT; j++) {
+    double rb = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP - 1; s++)
+      if (CHEM_STO[s][j] != 0) rb += (double)CHEM_STO[s][j] * vp[s];
+    const int a = CHEM_REACTANTS[j][0], b = CHEM_REACTANTS[j][1], cc = CHEM_REACTANTS[j][2];
+    const double ya = y[a], yb = y[b], yc = cc >= 0 ? y[cc] : 1.0;
+    yab[a] += rb * (k[j] * yb * yc);
+    yab[b] += rb * (k[j] * ya * yc);
+    if (cc >= 0) yab[cc] += rb * (k[j] * ya * yb);
+    Tb += rb * (dk[j] * ya * yb * yc);
+  }
+  // gas-energy row: forward tangents along H2_1, H_1, H_2, He_1, He_2, He_3, de and T
+  {
+    Dual1 yd[NSP], kd[NRATE];
+    constexpr int dirs[8] = {0, 2, 3, 5, 6, 7, 8, NSP};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int s = dirs[q];
+#pragma unroll
+      for (int u = 0; u < NSP; u++) yd[u] = Dual1{y[u], u == s ? 1.0 : 0.0};
+#pragma unroll
+      for (int r = 0; r < NRATE; r++) kd[r] = Dual1{k[r], s == NSP ? dk[r] : 0.0};
+      const Dual1 f9 = chem_rhs_energy_t<Dual1>(yd, kd, Dual1{T, s == NSP ? 1.0 : 0.0},
+                                                c.z, ax);
+      if (s == NSP) Tb += vp[9] * f9.d;
+      else yab[s] += vp[9] * f9.d;
+    }
+  }
+  if (clamped) return;
+  // temperature path
+  const double kb = 1.3806504e-16, mh = 1.67e-24;
+  const double gm1 = 1.0 / (5.e0 / 3.e0 - 1);
+  const double lT = log(T);
+  const int bn = chem_bin(c, lT);
+  const double t1 = c.lb + (double)bn * c.dbin;
+  const double t2 = c.lb + (double)(bn + 1) * c.dbin;
+  const double Tdef = (lT - t1) / (t2 - t1);
+  const double Tfactor = (1.0 / T) / (t2 - t1);
+  const double *g1t = c.tab + G_GH2_1 * TLEN, *g2t = c.tab + G_GH2_2 * TLEN;
+  const double gH2_1 = g1t[bn] + Tdef * (g1t[bn + 1] - g1t[bn]);
+  const double gH2_2 = g2t[bn] + Tdef * (g2t[bn + 1] - g2t[bn]);
+  const double sl1 = (g1t[bn + 1] - g1t[bn]) * Tfactor;       // d gamma_H2_1 / dT
+  const double sl2 = (g2t[bn + 1] - g2t[bn]) * Tfactor;
+  const double g1 = 1.0 / (gH2_1 - 1.0), g2 = 1.0 / (gH2_2 - 1.0);
+  const double cs[NSP - 1] = {g1, g2, gm1, gm1, gm1, gm1, gm1, gm1, gm1};
+  const double ws[NSP - 1] = {2.0, 2.0, 1.0079400000000001, 1.0079400000000001,
+                              1.0079400000000001, 4.0026020000000004, 4.0026020000000004,
+                              4.0026020000000004, 0.0};
+  double nsum = 0.0, density = 0.0;
+#pragma unroll
+  for (int s = 0; s < NSP - 1; s++) {
+    nsum += y[s] * cs[s];
+    density += ws[s] * y[s];
+  }
+  const double dm = density * mh;
+  const double dgdT = kb * nsum / dm + T * kb * ((-y[0]) * g1 * g1 * sl1 - y[1] * g2 * g2 * sl2) / dm;
+  const double TbT = Tb / dgdT;
+#pragma unroll
+  for (int s = 0; s < NSP - 1; s++) {
+    const double dgdy = T * kb / mh * (cs[s] * density - nsum * ws[s]) / (density * density);
+    yab[s] -= TbT * dgdy;
+  }
+  yab[NSP - 1] += TbT;                                            // dg/dge = -1
 }
 
 __device__ inline void chem_rhs_physical(const double (&y)[NSP], const double (&k)[NRATE],

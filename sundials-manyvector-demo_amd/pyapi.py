@@ -81,6 +81,7 @@ def lib():
         "mvd_chem_rhs": ([vp, C.c_double, _P, _P], C.c_int),
         "mvd_chem_jac": ([vp, C.c_double, _P, _P], C.c_int),
         "mvd_chem_state": ([vp, C.c_int], _P),
+        "mvd_chem_vjp": ([vp, C.c_double, _P, _P, _P, _P], C.c_int),
         "mvd_dirk_sdirk3": ([C.POINTER(DirkOpts)], C.c_int),
         "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
                                 C.POINTER(DirkStats)], C.c_int),
@@ -219,6 +220,10 @@ class Context:
 
     def chem_jac(self, y, J, t=0.0):
         _check("mvd_chem_jac", lib().mvd_chem_jac(self.h, t, dptr(y), dptr(J)))
+
+    def chem_vjp(self, y, v, ybar, ydot=None, t=0.0):
+        _check("mvd_chem_vjp", lib().mvd_chem_vjp(
+            self.h, t, dptr(y), dptr(v), dptr(ybar), dptr(ydot) if ydot is not None else _P()))
 
     def chem_state(self, which, torch_mod):
         """Copy of a per-cell state array (see mvd_chem_state) as a torch device tensor."""

@@ -129,6 +129,8 @@ def lib():
         L.orc_lu.restype = C.c_int
         L.orc_temperature_exit_check.argtypes = [C.c_void_p, P, P, C.POINTER(C.c_int)]
         L.orc_temperature_exit_check.restype = C.c_long
+        L.orc_chem_vjp.argtypes = [C.c_void_p, P, P, P, C.c_int]
+        L.orc_chem_vjp.restype = C.c_int
         _lib = L
     return _lib
 
@@ -251,6 +253,13 @@ class Chem:
             self.h, _p(y), _p(np.ascontiguousarray(Ts0, dtype=np.float64)),
             steps.ctypes.data_as(C.POINTER(C.c_int)))
         return bad, steps
+
+    def vjp(self, y, v, fixed_T=False):
+        """(d rhs / d y)^T v in the scaled variables (temperature solved and stored)."""
+        yb = np.zeros_like(y)
+        lib().orc_chem_vjp(self.h, _p(y), _p(np.ascontiguousarray(v, dtype=np.float64)),
+                           _p(yb), 1 if fixed_T else 0)
+        return yb
 
     def opcount(self):
         out = (C.c_ulonglong * 12)()
