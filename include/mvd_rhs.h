@@ -64,6 +64,12 @@ int mvd_fe_plain(mvd_context *ctx, double t, const double *const *w, double *con
  * capture); the flag is read by the next mvd_last_state_flags call. */
 int mvd_fe_async(mvd_context *ctx, double t, const double *const *w, double *const *wdot);
 int mvd_last_state_flags(mvd_context *ctx);
+/* Vector-Jacobian product of fe() (no counterpart in the reference, which has no adjoint;
+ * for adjoint/sensitivity callers): wbar = (d wdot / d w)^T v, v and wbar in the layout of
+ * wdot. The derivative of the Lax-Friedrichs speed max() follows its first maximum.
+ * Allocates a face buffer of 6*(5+nchem)*(nx+1)(ny+1)(nz+1) doubles on first use. */
+int mvd_fe_vjp(mvd_context *ctx, double t, const double *const *w, const double *const *v,
+               double *const *wbar);
 
 /* stability (utilities.cpp:483-528; ARKStep stability callback, euler3D_main.cpp:272-275):
  * alpha = max over cells of |mx/rho| + c, dt_stab = cfl * min(dx,dy,dz) / alpha. */

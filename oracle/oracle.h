@@ -26,6 +26,11 @@ int orc_chem_dirk_step(void *chem, const orc_dirk *opt, double h, double *y, int
 int orc_lu(const double *M, double *LU, int *piv, const double *b, double *x);
 /* bitwise check of the early exit of the device temperature solve (orc_chem.cpp) */
 long orc_temperature_exit_check(void *chem, const double *y, const double *Ts0, int *steps);
+/* vector-Jacobian products of fe() and of one face flux (orc_vjp.cpp) */
+int orc_face_flux_vjp(const double *stencil, int dir, int nchem, double gamma,
+                      const double *Fbar, double *sbar);
+int orc_fe_vjp(const orc_grid *g, const double *const *w, const double *const *v,
+               double *const *wbar);
 /* vector-Jacobian product of the chemistry RHS (orc_chem.cpp) */
 int orc_chem_vjp(void *chem, const double *y, const double *v, double *ybar, int fixed_T);
 

@@ -18,6 +18,7 @@ struct mvd_context {
   mvd::ChemState chem;
   int dirk_block = 256;   // batched DIRK launch shape (set by mvd_chem_dirk_tune)
   int dirk_waves = 0;     // resident waves of the persistent grid (0: one per SIMD)
+  double *vjp_buf = nullptr;   // fe() VJP face buffer (allocated on first use)
 };
 
 namespace {
@@ -134,6 +135,7 @@ void mvd_destroy(mvd_context *c) {
   if (c->fe.timer.on)
     for (int i = 0; i < 2 * mvd::PROF_NSLOT; i++) (void)hipEventDestroy(c->fe.timer.ev[i]);
   if (c->host_flag) (void)hipHostFree(c->host_flag);
+  if (c->vjp_buf) (void)hipFree(c->vjp_buf);
   mvd::chem_free(c->chem);
   delete c;
 }
@@ -154,6 +156,23 @@ int mvd_fe_plain(mvd_context *ctx, double, const double *const *w, double *const
 
 int mvd_fe_async(mvd_context *ctx, double, const double *const *w, double *const *wdot) {
   return fe_common(ctx, w, wdot, false, false);
+}
+
+int mvd_fe_vjp(mvd_context *ctx, double, const double *const *w, const double *const *v,
+               double *const *wbar) {
+  if (!ctx) return MVD_BAD_ARGUMENT;
+  int rc = check_ptrs(ctx, w);
+  if (!rc) rc = check_ptrs(ctx, v);
+  if (!rc) rc = check_ptrs(ctx, (const double *const *)wbar);
+  if (rc) return rc;
+  if (!ctx->vjp_buf &&
+      !dev_ok(hipMalloc(&ctx->vjp_buf, sizeof(double) * mvd::fe_vjp_buffer_doubles(ctx->g))))
+    return MVD_DEVICE_ERROR;
+  rc = mvd::fe_vjp_launch(ctx->g, fields_of(ctx, w), fields_of(ctx, v), fields_out(ctx, wbar),
+                          ctx->vjp_buf, ctx->stream);
+  if (rc == MVD_UNSUPPORTED) return rc;
+  if (rc || !dev_ok(hipStreamSynchronize(ctx->stream))) return MVD_DEVICE_ERROR;
+  return MVD_SUCCESS;
 }
 
 int mvd_last_state_flags(mvd_context *ctx) {

@@ -69,4 +69,9 @@ int fslow_prepare_launch(const Grid &g, double *const *w, const double *scale,
 int fslow_finish_launch(const Grid &g, double *const *wdot, hipStream_t st);
 int chem_scale_launch(long n, double *chem, const double *scale, int apply, hipStream_t st);
 
+// fe() vector-Jacobian product (fe_vjp.hip); buf holds fe_vjp_buffer_doubles(g) doubles
+size_t fe_vjp_buffer_doubles(const Grid &g);
+int fe_vjp_launch(const Grid &g, const Fields &w, const Fields &v, const FieldsOut &out,
+                  double *buf, hipStream_t st);
+
 }  // namespace mvd

@@ -69,6 +69,7 @@ def lib():
         "mvd_fe_plain": ([vp, C.c_double, _PP, _PP], C.c_int),
         "mvd_fe_async": ([vp, C.c_double, _PP, _PP], C.c_int),
         "mvd_last_state_flags": ([vp], C.c_int),
+        "mvd_fe_vjp": ([vp, C.c_double, _PP, _PP, _PP], C.c_int),
         "mvd_max_wave_speed": ([vp, _PP, _P], C.c_int),
         "mvd_stability": ([vp, _PP, C.c_double, _P], C.c_int),
         "mvd_fslow": ([vp, C.c_double, _PP, _PP], C.c_int),
@@ -166,6 +167,9 @@ class Context:
 
     def fe_async(self, w, wdot, t=0.0):
         return _check("mvd_fe_async", lib().mvd_fe_async(self.h, t, dptrs(w), dptrs(wdot)))
+
+    def fe_vjp(self, w, v, wbar, t=0.0):
+        return _check("mvd_fe_vjp", lib().mvd_fe_vjp(self.h, t, dptrs(w), dptrs(v), dptrs(wbar)))
 
     def last_state_flags(self):
         return lib().mvd_last_state_flags(self.h)

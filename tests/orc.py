@@ -129,6 +129,8 @@ def lib():
         L.orc_lu.restype = C.c_int
         L.orc_temperature_exit_check.argtypes = [C.c_void_p, P, P, C.POINTER(C.c_int)]
         L.orc_temperature_exit_check.restype = C.c_long
+        L.orc_face_flux_vjp.argtypes = [P, C.c_int, C.c_int, C.c_double, P, P]
+        L.orc_fe_vjp.argtypes = [C.POINTER(Grid), PP, PP, PP]
         L.orc_chem_vjp.argtypes = [C.c_void_p, P, P, P, C.c_int]
         L.orc_chem_vjp.restype = C.c_int
         _lib = L
@@ -193,6 +195,22 @@ def face_flux_opcount(nchem):
     out = (C.c_ulonglong * 5)()
     lib().orc_face_flux_opcount(nchem, out)
     return dict(zip(("add", "mul", "div", "sqrt", "cmp"), list(out)))
+
+
+def fe_vjp(g, w, v):
+    """wbar = (d fe / d w)^T v on the CPU oracle (v in the layout of wdot)."""
+    wbar = new_state(g)
+    lib().orc_fe_vjp(C.byref(g), _pp(w), _pp(v), _pp(wbar))
+    return wbar
+
+
+def face_flux_vjp(stencil, direction, Fbar, gamma=1.4):
+    stencil = np.ascontiguousarray(stencil, dtype=np.float64)
+    nv = stencil.shape[1]
+    out = np.zeros(6 * nv)
+    lib().orc_face_flux_vjp(_p(stencil.reshape(-1).copy()), direction, nv - 5, gamma,
+                            _p(np.ascontiguousarray(Fbar, dtype=np.float64)), _p(out))
+    return out.reshape(6, nv)
 
 
 def load_tables():

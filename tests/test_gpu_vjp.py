@@ -86,3 +86,62 @@ def test_chem_vjp_conservation_cotangents(mvd, torch_gpu):
         got = np.abs(yb.cpu().numpy()).reshape(n, 10).max(axis=1)
         mag = np.abs(v).reshape(n, 10).max(axis=1) / vrm
         assert np.all(got <= 1e-11 * ref * mag + 1e-300)
+
+
+# ---- fe() ----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dims,nchem,bc", [((12, 10, 9), 0, (0,) * 6),
+                                           ((11, 9, 8), 0, (1, 3, 3, 1, 3, 1)),
+                                           ((10, 9, 8), 2, (0, 0, 1, 3, 0, 0)),
+                                           ((9, 8, 10), 10, (3, 1, 0, 0, 1, 3))])
+def test_fe_vjp_matches_oracle(mvd, torch_gpu, dims, nchem, bc):
+    torch = torch_gpu
+    print(f"[fe vjp] dims={dims} nchem={nchem} bc={bc}", flush=True)
+    nx, ny, nz = dims
+    w = problems.smooth_state(nx, ny, nz, nchem, seed=3)
+    rng = np.random.default_rng(4)
+    ncell = nx * ny * nz
+    v = [rng.standard_normal(ncell) for _ in range(5)] + \
+        [rng.standard_normal(max(nchem, 1) * ncell)]
+    g = orc.make_grid(nx, ny, nz, nchem, bc)
+    ref = orc.fe_vjp(g, w, v)
+    ctx = mvd.Context(mvd.make_params(nx, ny, nz, nchem, bc))
+    wd = [torch.from_numpy(a).to("cuda") for a in w]
+    vd = [torch.from_numpy(a).to("cuda") for a in v]
+    if nchem == 0:
+        wd[5] = vd[5] = None
+    wb = [torch.empty(ncell, dtype=torch.float64, device="cuda") for _ in range(5)] + \
+         [torch.empty(nchem * ncell, dtype=torch.float64, device="cuda") if nchem else None]
+    ctx.fe_vjp(wd, vd, wb)
+    for q in range(5):
+        assert normwise_err(wb[q].cpu().numpy(), ref[q]) <= TOL, q
+    if nchem:
+        got = wb[5].cpu().numpy().reshape(ncell, nchem)
+        r = ref[5].reshape(ncell, nchem)
+        for s in range(nchem):
+            assert normwise_err(got[:, s], r[:, s]) <= TOL, ("chem", s)
+
+
+def test_fe_vjp_dot_product_64(mvd, torch_gpu):
+    """<v, (fe(w+hu) - fe(w-hu))/2h> = <wbar, u> with the fast fe() kernels at 64^3."""
+    torch = torch_gpu
+    n = 64
+    print(f"[fe vjp dot] n={n}", flush=True)
+    w = [torch.from_numpy(a).to("cuda") for a in problems.blast3d(n, seed=5)[:5]] + [None]
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    u = [torch.randn(n ** 3, dtype=torch.float64, device="cuda", generator=gen) * a.abs().max()
+         for a in w[:5]] + [None]
+    v = [torch.randn(n ** 3, dtype=torch.float64, device="cuda", generator=gen)
+         for _ in range(5)] + [None]
+    ctx = mvd.Context(mvd.make_params(n, n, n, 0, (0,) * 6))
+    wb = [torch.empty_like(a) for a in w[:5]] + [None]
+    ctx.fe_vjp(w, v, wb)
+    h = 1e-6
+    fp = [torch.empty_like(a) for a in w[:5]] + [None]
+    fm = [torch.empty_like(a) for a in w[:5]] + [None]
+    ctx.fe([a + h * b for a, b in zip(w[:5], u[:5])] + [None], fp)
+    ctx.fe([a - h * b for a, b in zip(w[:5], u[:5])] + [None], fm)
+    lhs = sum(float(torch.dot(vv, (a - b) / (2 * h))) for vv, a, b in zip(v[:5], fp[:5], fm[:5]))
+    rhs = sum(float(torch.dot(a, b)) for a, b in zip(wb[:5], u[:5]))
+    mag = sum(float(torch.dot(a.abs(), b.abs())) for a, b in zip(wb[:5], u[:5]))
+    assert abs(lhs - rhs) <= 1e-6 * mag, (lhs, rhs, mag)
