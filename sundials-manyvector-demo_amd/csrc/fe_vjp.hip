@@ -3,16 +3,19 @@
 // fe(w) = -div F with every face flux a function of its 6-point stencil, so
 //   Fbar(face between cells c-1 | c) = (v_c - v_{c-1}) / delta   (no cell: 0)
 // and each face's stencil adjoint sbar = (dF/ds)^T Fbar goes back to the stencil's source
-// cells (ghost points are +-copies of interior cells). Per direction:
-//   fe_vjp_faces  - one thread per face: recompute the face flux (the reference recipe,
-//                   utilities.cpp:270-479) and sweep it in reverse (WENO5-JS weights,
-//                   Lax-Friedrichs split, characteristic projections, Roe average and
-//                   eigenvectors, point fluxes, wave speeds, EOS); the 6 x NV stencil
-//                   adjoint (ghost signs applied) goes to a face buffer;
-//   fe_vjp_gather - one thread per cell sums the contributions of the six faces whose
-//                   stencils hold it, plus those of ghost points that copy it near a
-//                   physical or periodic boundary, into wbar (first direction stores, the
-//                   others add). No atomics: the result is deterministic.
+// cells (ghost points are +-copies of interior cells). Every face recomputes its flux (the
+// reference recipe, utilities.cpp:270-479) and sweeps it in reverse (WENO5-JS weights,
+// Lax-Friedrichs split, characteristic projections, Roe average and eigenvectors, point
+// fluxes, wave speeds, EOS). No face buffer: one sweep kernel per direction adds the
+// contributions into the cells as it walks along the lines, through a ring of open rows
+// in LDS, and writes each cell once its last face is in:
+//   fe_vjp_sweep_x   - a wave walks whole x-lines 64 faces at a time (lanes along x);
+//   fe_vjp_sweep_yz  - each lane walks its own column along y or z (lanes along x);
+//   fe_vjp_fix_x/_yz - add the contributions that left a line or segment (ghost points,
+//                      rows of neighbouring segments), held in a small side buffer.
+// The x sweep stores wbar, the others add. No atomics: the result is deterministic.
+// Tracers are handled tracer-major inside (cells_to_planes / planes_to_cells around the
+// sweeps); the interface keeps the cell-major layout.
 #include "mvd_device.hpp"
 #include "mvd_internal.hpp"
 #include "../../include/mvd_rhs.h"
@@ -29,17 +32,6 @@ constexpr double W_C16 = 0.1666666666666666666666666666666666666667;
 constexpr double W_C76 = 1.166666666666666666666666666666666666667;
 constexpr double W_C116 = 1.833333333333333333333333333333333333333;
 
-// 1/x from the hardware estimate (relative error below 2^-22) and two Newton steps, to
-// within an ulp or two of the IEEE quotient for the finite, nonzero arguments used here;
-// about a third of the instructions of a correctly rounded division.
-__device__ inline double rcp64(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-}
-
 // WENO5-JS reconstruction from f[0..4] (side 0: the f+ formula over points i-3..i+1,
 // side 1: the mirrored f- formula over points i-2..i+2) and its gradient g = dr/df.
 __device__ inline double weno5_grad(const double (&f)[5], int side, double (&g)[5]) {
@@ -51,7 +43,7 @@ __device__ inline double weno5_grad(const double (&f)[5], int side, double (&g)[
   const double b3 = W_BC * A3 * A3 + 0.25 * B3 * B3;
   const double d1 = side == 0 ? 0.3 : 0.1, d3 = side == 0 ? 0.1 : 0.3;
   const double e1 = W_EPS + b1, e2 = W_EPS + b2, e3 = W_EPS + b3;
-  const double ie1 = rcp64(e1), ie2 = rcp64(e2), ie3 = rcp64(e3);
+  const double ie1 = fast_rcp(e1), ie2 = fast_rcp(e2), ie3 = fast_rcp(e3);
   const double w1 = d1 * ie1 * ie1, w2 = 0.6 * ie2 * ie2, w3 = d3 * ie3 * ie3;
   double q1, q2, q3;
   double dq1[5] = {0, 0, 0, 0, 0}, dq2[5] = {0, 0, 0, 0, 0}, dq3[5] = {0, 0, 0, 0, 0};
@@ -70,7 +62,7 @@ __device__ inline double weno5_grad(const double (&f)[5], int side, double (&g)[
     dq2[1] = W_C13; dq2[2] = W_C56; dq2[3] = -W_C16;
     dq3[0] = -W_C16; dq3[1] = W_C56; dq3[2] = W_C13;
   }
-  const double iS = rcp64(w1 + w2 + w3);
+  const double iS = fast_rcp(w1 + w2 + w3);
   const double r = (q1 * w1 + q2 * w2 + q3 * w3) * iS;
   // dw_i/df = -2 w_i / e_i * db_i/df
   const double k1 = -2.0 * w1 * ie1 * (q1 - r) * iS;
@@ -105,7 +97,7 @@ __device__ inline void face_prim(const double (&s)[6][5], double gamma, FacePrim
   P.lstar = -1;
 #pragma unroll
   for (int l = 0; l < 6; l++) {
-    P.ir[l] = rcp64(s[l][0]);
+    P.ir[l] = fast_rcp(s[l][0]);
     P.p[l] = gm * (s[l][4] - (s[l][1] * s[l][1] + s[l][2] * s[l][2] + s[l][3] * s[l][3]) * 0.5 * P.ir[l]);
     P.ul[l] = s[l][1] * P.ir[l];
     const double a = fabs(P.ul[l]) + sun_sqrt(gamma * P.p[l] * P.ir[l]);
@@ -148,15 +140,15 @@ __device__ inline void fluid_vjp(const double (&s)[6][5], double gamma, const Fa
   const double gm = gamma - 1.0;
   const double alpha = P.alpha;
   const double sL = sun_sqrt(s[2][0]), sR = sun_sqrt(s[3][0]);
-  const double isL = rcp64(sL), isR = rcp64(sR);
-  const double sbar = 0.5 * (sL + sR), isbar = rcp64(sbar);
+  const double isL = fast_rcp(sL), isR = fast_rcp(sR);
+  const double sbar = 0.5 * (sL + sR), isbar = fast_rcp(sbar);
   const double u = 0.5 * (s[2][1] * isL + s[3][1] * isR) * isbar;
   const double v = 0.5 * (s[2][2] * isL + s[3][2] * isR) * isbar;
   const double w = 0.5 * (s[2][3] * isL + s[3][3] * isR) * isbar;
   const double H = 0.5 * ((P.p[2] + s[2][4]) * isL + (P.p[3] + s[3][4]) * isR) * isbar;
   const double qsq = u * u + v * v + w * w;
   const double cs = gm * (H - 0.5 * qsq);
-  const double ci = rcp64(cs);
+  const double ci = fast_rcp(cs);
   const double LV[5][5] = {
       {0.5 * ci * (u + 0.5 * gm * qsq), -0.5 * ci * (gm * u + 1.0), -0.5 * v * gm * ci,
        -0.5 * w * gm * ci, 0.5 * gm * ci},
@@ -300,7 +292,7 @@ __device__ inline void fluid_vjp(const double (&s)[6][5], double gamma, const Fa
     const double ulbS = ulS < 0.0 ? -alphab : alphab;
     double pbS = 0.0, rbS = 0.0;
     if (X > 0.0) {
-      const double Xb = 0.5 * alphab * rcp64(sun_sqrt(X));
+      const double Xb = 0.5 * alphab * fast_rcp(sun_sqrt(X));
       pbS = Xb * gamma * irS;
       rbS = -Xb * X * irS;
     }
@@ -341,25 +333,27 @@ __device__ inline long face_index(const Grid &g, int d, long i, long j, long k) 
                 : i + (long)g.nx * (j + (long)g.ny * k);
 }
 
-__device__ inline double field(const Fields &f, int nc, int q, long cell) {
-  return q < 5 ? f.w[q][cell] : f.chem[(long)nc * cell + (q - 5)];
-}
 
 }  // namespace
 
-template <int NC>
-__global__ void __launch_bounds__(256)
-fe_vjp_faces(Grid g, Fields w, Fields vc, int d, double *buf, long nf) {
-  constexpr int NV = 5 + NC;
-  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= nf) return;
-  const long ex = g.nx + (d == 0), ey = g.ny + (d == 1);
-  const int i = (int)(tid % ex), j = (int)((tid / ex) % ey), k = (int)(tid / (ex * ey));
+// One face's stencil adjoint, handed to a sink with the ghost rule's signs applied, one
+// component at a time: the tracers first (the fluid part needs the velocity and
+// wave-speed adjoints they produce), then the fluid in global component order. For each
+// component the sink's pre(q) is called before the adjoint is computed (to start the
+// loads its put needs) and put(q, x[6], pre) after. The next tracer's stencil is loaded
+// while the current one is swept. Face (i, j, k) along d lies between cells c-1 and c.
+template <int NC, class Sink>
+__device__ inline void face_vjp(const Grid &g, const Fields &w, const Fields &vc, int d, int i,
+                                int j, int k, Sink &sink) {
   const int n = extent(g, d);
   const int c0 = d == 0 ? i : d == 1 ? j : k;
   const double idelta = 1.0 / (d == 0 ? g.dx : d == 1 ? g.dy : g.dz);
-  const long cr = c0 < n ? cell_idx(g, i, j, k) : -1;
-  const long cl = c0 > 0 ? cell_idx(g, i - (d == 0), j - (d == 1), k - (d == 2)) : -1;
+  // cells either side of the face; a missing one reads the other and counts as zero (the
+  // loads stay unconditional, which keeps the compiler's waits on them precise)
+  const bool hr = c0 < n, hl = c0 > 0;
+  const long cr = cell_idx(g, i - (d == 0 && !hr), j - (d == 1 && !hr), k - (d == 2 && !hr));
+  const long cl = hl ? cell_idx(g, i - (d == 0), j - (d == 1), k - (d == 2)) : cr;
+  const double mr = hr ? 1.0 : 0.0, ml = hl ? 1.0 : 0.0;
   long cell[6];
   double sa[6], sn[6];
   double s[6][5];
@@ -373,107 +367,411 @@ fe_vjp_faces(Grid g, Fields w, Fields vc, int d, double *buf, long nf) {
     sn[l] = m.sgn_norm;
 #pragma unroll
     for (int a = 0; a < 5; a++)
-      s[l][a] = field(w, NC, gcomp(d, a), cell[l]) * (a == 1 ? sn[l] : sa[l]);
+      s[l][a] = w.w[gcomp(d, a)][cell[l]] * (a == 1 ? sn[l] : sa[l]);
   }
   FacePrim P;
   face_prim(s, g.gamma, P);
   double ulb[6] = {0, 0, 0, 0, 0, 0};
   double alphab = 0.0;
-  // tracers, one at a time
+  if (NC > 0) {
+    // software pipeline: tracer c+1's stencil and sink loads are issued before tracer c
+    // is swept (the empty asm statements keep the compiler from sinking them)
+    double tn[6], rn, ln;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    auto load_tracer = [&](int c) {
+      const long pl = (long)c * ncell;
+#pragma unroll
+      for (int l = 0; l < 6; l++) tn[l] = w.chem[pl + cell[l]];
+      rn = vc.chem[pl + cr];
+      ln = vc.chem[pl + cl];
+    };
+    auto pre_next = sink.pre(5);
+    load_tracer(0);
 #pragma unroll 1
-  for (int c = 0; c < NC; c++) {
-    double t[6], tb[6];
+    for (int c = 0; c < NC; c++) {
+      double t[6], tb[6];
 #pragma unroll
-    for (int l = 0; l < 6; l++) t[l] = w.chem[(long)NC * cell[l] + c] * sa[l];
-    const double r = cr >= 0 ? vc.chem[(long)NC * cr + c] : 0.0;
-    const double lv = cl >= 0 ? vc.chem[(long)NC * cl + c] : 0.0;
-    tracer_vjp(t, (r - lv) * idelta, P, tb, ulb, alphab);
+      for (int l = 0; l < 6; l++) t[l] = tn[l] * sa[l];
+      const double Fbc = (mr * rn - ml * ln) * idelta;
+      const auto pre = pre_next;
+      if (c + 1 < NC) {
+        pre_next = sink.pre(5 + c + 1);
+        load_tracer(c + 1);
+      }
+      __asm__ volatile("" ::: "memory");
+      tracer_vjp(t, Fbc, P, tb, ulb, alphab);
 #pragma unroll
-    for (int l = 0; l < 6; l++) buf[(long)(l * NV + 5 + c) * nf + tid] = tb[l] * sa[l];
+      for (int l = 0; l < 6; l++) tb[l] *= sa[l];
+      sink.put(5 + c, tb, pre);
+    }
   }
   double Fb[5];
 #pragma unroll
   for (int a = 0; a < 5; a++) {
     const int q = gcomp(d, a);
-    const double r = cr >= 0 ? vc.w[q][cr] : 0.0;
-    const double lv = cl >= 0 ? vc.w[q][cl] : 0.0;
-    Fb[a] = (r - lv) * idelta;
+    Fb[a] = (mr * vc.w[q][cr] - ml * vc.w[q][cl]) * idelta;
   }
   double sb[6][5];
   fluid_vjp(s, g.gamma, P, Fb, ulb, alphab, sb);
 #pragma unroll
-  for (int l = 0; l < 6; l++)
+  for (int a = 0; a < 5; a++) {
+    const int q = gcomp(d, a);
+    const auto pre = sink.pre(q);
+    double x[6];
 #pragma unroll
-    for (int a = 0; a < 5; a++)
-      buf[(long)(l * NV + a) * nf + tid] = sb[l][a] * (a == 1 ? sn[l] : sa[l]);
+    for (int l = 0; l < 6; l++) x[l] = sb[l][a] * (a == 1 ? sn[l] : sa[l]);
+    sink.put(q, x, pre);
+  }
 }
 
-template <int NC>
-__global__ void __launch_bounds__(256)
-fe_vjp_gather(Grid g, int d, const double *buf, long nf, FieldsOut out, int accumulate) {
-  constexpr int NV = 5 + NC;
-  const long ncell = (long)g.nx * g.ny * g.nz;
-  const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= ncell) return;
-  const int i = (int)(cell % g.nx), j = (int)((cell / g.nx) % g.ny),
-            k = (int)(cell / ((long)g.nx * g.ny));
-  const int n = extent(g, d);
-  const int c = d == 0 ? i : d == 1 ? j : k;
-  auto fidx = [&](int f) {
-    return face_index(g, d, d == 0 ? f : i, d == 1 ? f : j, d == 2 ? f : k);
-  };
-  double acc[NV];
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline void out_put(const FieldsOut &o, long ncell, int q, long cell, double v,
+                               bool add) {
+  double *p = q < 5 ? &o.w[q][cell] : &o.chem[(long)(q - 5) * ncell + cell];
+  *p = add ? *p + v : v;
+}
+
+// Ghost-point contributions of an x-line: the (face, point) pairs whose extended index
+// lies outside the line, 6 on each end. Index of pair (f, l), or -1 for an interior point.
+__device__ inline int xghost_pair(int n, int f, int l) {
+  const int e = f - 3 + l;
+  if (e < 0) return f == 0 ? l : f == 1 ? 3 + l : 5;
+  if (e >= n) {
+    const int b = n - f;
+    return b == 0 ? 3 + l : b == 1 ? 5 + l : 11;   // l in 3..5 / 4..5 / 5
+  }
+  return -1;
+}
+
+// ------------------------------------------------------------------------------------
+// x sweep: one wave walks a run of whole x-lines 64 faces at a time (faces consecutive in
+// memory, lines back to back); the contributions go to a 128-slot ring indexed by the
+// flattened position line * (nx+1) + e of their target. Lanes share slots, so the six
+// stencil points are added in six ordered passes. A slot is final once the faces up to
+// three beyond it are in; it is then written out and cleared. Ghost points go to the side
+// buffer per (line, pair) and are added by fe_vjp_fix_x.
+// ------------------------------------------------------------------------------------
+// The six points go in as six no-return LDS adds, one point per instruction: within one
+// instruction the lanes hit distinct slots, and a wave's LDS operations complete in
+// order, so the sum order (point 0 first) is fixed.
+template <int NV>
+struct XSink {
+  double *ring;            // [NV][128]
+  double *side;            // [(pair * NV + q) * nlines + line]
+  long q;                  // flattened face
+  int f, n, line, nlines;
+  __device__ int pre(int) const { return 0; }
+  __device__ void put(int comp, const double (&x)[6], int) {
 #pragma unroll
-  for (int a = 0; a < NV; a++) acc[a] = 0.0;
-  // faces f = c+3-l whose point l is this cell
-#pragma unroll
-  for (int l = 0; l < 6; l++) {
-    const int f = c + 3 - l;
-    if (f >= 0 && f <= n) {
-      const long fi = fidx(f);
-#pragma unroll
-      for (int a = 0; a < NV; a++) acc[a] += buf[(long)(l * NV + a) * nf + fi];
+    for (int l = 0; l < 6; l++) {
+      const int pr = xghost_pair(n, f, l);
+      if (pr < 0) atomicAdd(&ring[comp * 128 + (int)((q - 3 + l) & 127)], x[l]);
+      else side[((long)pr * NV + comp) * nlines + line] = x[l];
+      wave_sync_lds();
     }
   }
-  // ghost points that copy this cell (faces 0..2 and n-2..n)
-  if (c < 3 || c >= n - 3) {
-    for (int f = 0; f <= n; f++) {
-      if (f > 2 && f < n - 2) { f = n - 3; continue; }
-      const long fi = fidx(f);
-      for (int l = 0; l < 6; l++) {
-        const int e = f - 3 + l;
-        if (e >= 0 && e < n) continue;
-        if (ghost_map(g, d, e).src != c) continue;
+};
+
+template <int NC>
+__global__ void __launch_bounds__(64)
+fe_vjp_sweep_x(Grid g, Fields w, Fields vc, FieldsOut out, double *side, int lps, int add) {
+  constexpr int NV = 5 + NC;
+  __shared__ double ring[NV * 128];
+  const int lane = threadIdx.x;
+  const int nx = g.nx, nxp = g.nx + 1;
+  const int nlines = g.ny * g.nz;
+  const int L0 = blockIdx.x * lps;
+  const int L1 = min(L0 + lps, nlines);
+  if (L0 >= nlines) return;
+  const long Q0 = (long)L0 * nxp, Q1 = (long)L1 * nxp;
+  const long ncell = (long)nx * nlines;
+  for (int s = lane; s < NV * 128; s += 64) ring[s] = 0.0;
+  wave_sync_lds();
+  long done = Q0;          // first slot not yet written out
+  for (long B = Q0; B < Q1; B += 64) {
+    const long q = B + lane;
+    if (q < Q1) {
+      const int line = (int)(q / nxp), f = (int)(q - (long)line * nxp);
+      XSink<NV> sink{ring, side, q, f, nx, line, nlines};
+      face_vjp<NC>(g, w, vc, 0, f, line % g.ny, line / g.ny, sink);
+    }
+    wave_sync_lds();
+    const long hi = (B + 64 >= Q1) ? Q1 - 1 : B + 60;
+    for (long P = done + lane; P <= hi; P += 64) {
+      const int line = (int)(P / nxp), e = (int)(P - (long)line * nxp);
+      const int slot = (int)(P & 127);
+      if (e < nx) {
+        const long cell = e + (long)nx * line;
+#pragma unroll 1
+        for (int qq = 0; qq < NV; qq++) out_put(out, ncell, qq, cell, ring[qq * 128 + slot], add);
+      }
+#pragma unroll 1
+      for (int qq = 0; qq < NV; qq++) ring[qq * 128 + slot] = 0.0;
+    }
+    done = hi + 1;
+    wave_sync_lds();
+  }
+}
+
+// Ghost contributions of each x-line into their source cells (one thread per line and
+// component, pairs in a fixed order).
+template <int NC>
+__global__ void __launch_bounds__(256)
+fe_vjp_fix_x(Grid g, const double *side, FieldsOut out) {
+  constexpr int NV = 5 + NC;
+  const int nlines = g.ny * g.nz;
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long)nlines * NV) return;
+  const int line = (int)(tid % nlines), qq = (int)(tid / nlines);
+  const int n = g.nx;
+  for (int f = 0; f <= n; f++) {
+    if (f > 2 && f < n - 2) { f = n - 3; continue; }
+#pragma unroll 1
+    for (int l = 0; l < 6; l++) {
+      const int pr = xghost_pair(n, f, l);
+      if (pr < 0) continue;
+      const long cell = ghost_map(g, 0, f - 3 + l).src + (long)n * line;
+      out_put(out, (long)n * nlines, qq, cell, side[((long)pr * NV + qq) * nlines + line], true);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// y and z sweeps: each lane walks its own column (x = 64 * block + lane, fixed transverse
+// index t) over a segment [c0, c1) of faces along D; rows belong to that lane alone, so
+// no lane waits on another. The five rows f-2..f+2 still open live in an LDS ring
+// ([5][NV][64] per wave); row f-3 is complete with face f's first point and is written out
+// at once, its slot taking row f+2. Rows outside [c0, c1) (ghost rows and the three rows
+// either side that belong to neighbouring segments) go to the side buffer
+// [((seg * 6 + k) * NV + q) * ncol + col] and are added by fe_vjp_fix_yz.
+// ------------------------------------------------------------------------------------
+template <int NV>
+struct ColSink {
+  double *ring;            // this wave's [5][NV][64]
+  double *side;
+  FieldsOut out;
+  int lane, slot0;         // slot0: ring slot of row f-3 (slot of row f-3+l is (slot0+l)%5)
+  int e0, c0, c1, seg;
+  long ncell;
+  long col, ncol;
+  long cell0;              // cell of row f-3 when it lies in [c0, c1)
+  long cbase;              // cell of the column's row 0
+  bool add;
+  __device__ int slot(int l) const { const int s = slot0 + l; return s >= 5 ? s - 5 : s; }
+  __device__ bool owned() const { return e0 >= c0 && e0 < c1; }
+  __device__ double *dst(int comp) const {
+    return comp < 5 ? &out.w[comp][cell0] : &out.chem[(long)(comp - 5) * ncell + cell0];
+  }
+  struct Pre { double r0, w0; };
+  // row f-3 so far, and the value already in wbar where this sweep adds to it (read
+  // unconditionally, from the column's first row when f-3 is not an owned row, and only
+  // looked at in put, so that the loads stay in flight meanwhile)
+  __device__ Pre pre(int comp) const {
+    const long c = add && owned() ? cell0 : cbase;
+    const double w0 = comp < 5 ? out.w[comp][c] : out.chem[(long)(comp - 5) * ncell + c];
+    return Pre{ring[((long)slot0 * NV + comp) * 64 + lane], w0};
+  }
+  __device__ void put(int comp, const double (&x)[6], const Pre &p) {
+    const double v = p.r0 + x[0];         // row f-3 is now complete
+    if (owned()) *dst(comp) = add ? p.w0 + v : v;
+    else {
+      const int k = e0 - (c0 - 3);        // e0 < c0 here
+      side[(((long)seg * 6 + k) * NV + comp) * ncol + col] = v;
+    }
+    ring[((long)slot0 * NV + comp) * 64 + lane] = x[5];   // the slot now holds row f+2
 #pragma unroll
-        for (int a = 0; a < NV; a++) acc[a] += buf[(long)(l * NV + a) * nf + fi];
+    for (int l = 1; l < 5; l++) atomicAdd(&ring[((long)slot(l) * NV + comp) * 64 + lane], x[l]);
+  }
+};
+
+template <int D, int NC>
+__global__ void __launch_bounds__(64)
+fe_vjp_sweep_yz(Grid g, Fields w, Fields vc, FieldsOut out, double *side, int nseg, int add) {
+  constexpr int NV = 5 + NC;
+  __shared__ double ring[5 * NV * 64];
+  const int lane = threadIdx.x;
+  const int nxb = (g.nx + 63) / 64;
+  const int nt = D == 1 ? g.nz : g.ny;
+  const int n = D == 1 ? g.ny : g.nz;
+  const int xb = blockIdx.x % nxb;
+  const int rest = blockIdx.x / nxb;
+  const int t = rest % nt, seg = rest / nt;
+  const int i = xb * 64 + lane;
+  if (i >= g.nx) return;
+  const int c0 = (int)((long)seg * n / nseg), c1 = (int)((long)(seg + 1) * n / nseg);
+  const int flast = seg == nseg - 1 ? n : c1 - 1;
+  const long ncol = (long)g.nx * nt, col = i + (long)g.nx * t;
+  for (int s = 0; s < 5 * NV; s++) ring[s * 64 + lane] = 0.0;
+  const long rstride = D == 1 ? g.nx : (long)g.nx * g.ny;
+  const long cbase = D == 1 ? i + (long)g.nx * g.ny * t : i + (long)g.nx * t;
+  const long ncell = (long)g.nx * g.ny * g.nz;
+  ColSink<NV> sink{ring, side, out, lane, 0, 0, c0, c1, seg, ncell, col, ncol, 0, cbase,
+                   add != 0};
+  for (int f = c0; f <= flast; f++) {
+    sink.slot0 = (f - c0) % 5;
+    sink.e0 = f - 3;
+    sink.cell0 = cbase + (long)(f - 3) * rstride;
+    if (D == 1) face_vjp<NC>(g, w, vc, 1, i, f, t, sink);
+    else face_vjp<NC>(g, w, vc, 2, i, t, f, sink);
+  }
+  // rows flast-2 .. flast+2 are complete
+  for (int r = 0; r < 5; r++) {
+    const int e = flast - 2 + r;
+    const int sl = (flast + 1 - c0 + r) % 5;
+    for (int qq = 0; qq < NV; qq++) {
+      const double v = ring[((long)sl * NV + qq) * 64 + lane];
+      if (e >= c0 && e < c1) out_put(out, ncell, qq, cbase + (long)e * rstride, v, add != 0);
+      else {
+        const int k = e < c0 ? e - (c0 - 3) : 3 + (e - c1);
+        side[(((long)seg * 6 + k) * NV + qq) * ncol + col] = v;
       }
     }
   }
-#pragma unroll
-  for (int a = 0; a < NV; a++) {
-    const int q = gcomp(d, a);
-    double *dst = q < 5 ? &out.w[q][cell] : &out.chem[(long)NC * cell + (q - 5)];
-    *dst = accumulate ? *dst + acc[a] : acc[a];
+  if (flast + 2 < c1 + 2)      // an inner segment's last side row (c1 + 2) gets nothing
+    for (int qq = 0; qq < NV; qq++) side[(((long)seg * 6 + 5) * NV + qq) * ncol + col] = 0.0;
+}
+
+// Side rows of every segment of a column into their cells (one thread per column and
+// component, segments and rows in a fixed order): rows of neighbouring segments directly,
+// ghost rows through the ghost rule's source (their signs were applied at the faces).
+template <int D, int NC>
+__global__ void __launch_bounds__(256)
+fe_vjp_fix_yz(Grid g, const double *side, FieldsOut out, int nseg) {
+  constexpr int NV = 5 + NC;
+  const int nt = D == 1 ? g.nz : g.ny;
+  const int n = D == 1 ? g.ny : g.nz;
+  const long ncol = (long)g.nx * nt;
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= ncol * NV) return;
+  const long col = tid % ncol;
+  const int qq = (int)(tid / ncol);
+  const int i = (int)(col % g.nx), t = (int)(col / g.nx);
+  const long rstride = D == 1 ? g.nx : (long)g.nx * g.ny;
+  const long cbase = D == 1 ? i + (long)g.nx * g.ny * t : i + (long)g.nx * t;
+  for (int seg = 0; seg < nseg; seg++) {
+    const int c0 = (int)((long)seg * n / nseg), c1 = (int)((long)(seg + 1) * n / nseg);
+#pragma unroll 1
+    for (int k = 0; k < 6; k++) {
+      const int e = k < 3 ? c0 - 3 + k : c1 + (k - 3);
+      const int src = ghost_map(g, D, e).src;
+      const long cell = cbase + (long)src * rstride;
+      out_put(out, (long)g.nx * g.ny * g.nz, qq, cell,
+              side[(((long)seg * 6 + k) * NV + qq) * ncol + col], true);
+    }
   }
 }
 
+// Tracer layout: the C ABI holds tracers cell-major ([cell][nchem]); the sweeps read
+// and write them tracer-major ([nchem][cell]) so that every tracer access of a wave is one
+// contiguous 512-byte run. 256 cells per block through an LDS tile, both sides coalesced.
+template <int NC>
+__global__ void __launch_bounds__(256) cells_to_planes(const double *a, double *p, long ncell) {
+  __shared__ double t[(NC > 0 ? NC : 1) * 257];
+  const long base = (long)blockIdx.x * 256;
+  const long lim = (ncell - base) * NC;
+  for (int k = threadIdx.x; k < NC * 256; k += 256)
+    if (k < lim) t[(k % NC) * 257 + k / NC] = a[base * NC + k];
+  __syncthreads();
+  const long cell = base + threadIdx.x;
+  if (cell < ncell)
+#pragma unroll
+    for (int c = 0; c < NC; c++) p[(long)c * ncell + cell] = t[c * 257 + threadIdx.x];
+}
+
+template <int NC>
+__global__ void __launch_bounds__(256) planes_to_cells(const double *p, double *a, long ncell) {
+  __shared__ double t[(NC > 0 ? NC : 1) * 257];
+  const long base = (long)blockIdx.x * 256;
+  const long cell = base + threadIdx.x;
+  if (cell < ncell)
+#pragma unroll
+    for (int c = 0; c < NC; c++) t[c * 257 + threadIdx.x] = p[(long)c * ncell + cell];
+  __syncthreads();
+  const long lim = (ncell - base) * NC;
+  for (int k = threadIdx.x; k < NC * 256; k += 256)
+    if (k < lim) a[base * NC + k] = t[(k % NC) * 257 + k / NC];
+}
+
+// Launch shape: one wave per SIMD over the device (4 per CU) where the grid allows.
+struct VjpPlan {
+  int lps;          // x-lines per wave
+  int nseg[3];      // segments per y / z column (index 1, 2)
+  size_t side;      // side-buffer doubles (the largest direction's)
+};
+
+static VjpPlan vjp_plan(const Grid &g) {
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const long target = 4L * (ncu > 0 ? ncu : 256);
+  const long nv = 5 + g.nchem;
+  VjpPlan p{};
+  const long nlines = (long)g.ny * g.nz;
+  p.lps = (int)((nlines + target - 1) / target);
+  size_t side = (size_t)nlines * 12 * nv;
+  for (int d = 1; d < 3; d++) {
+    const long n = d == 1 ? g.ny : g.nz, nt = d == 1 ? g.nz : g.ny;
+    const long waves = (long)((g.nx + 63) / 64) * nt;
+    long ns = (target + waves - 1) / waves;
+    const long maxseg = n >= 16 ? n / 8 : 1;
+    if (ns > maxseg) ns = maxseg;
+    if (ns < 1) ns = 1;
+    p.nseg[d] = (int)ns;
+    const size_t sd = (size_t)g.nx * nt * ns * 6 * nv;
+    if (sd > side) side = sd;
+  }
+  p.side = side;
+  return p;
+}
+
+// Scratch: the side buffer, then (with tracers) tracer planes of w, v and wbar.
 template <int NC>
 static int fe_vjp_run(const Grid &g, const Fields &w, const Fields &v, const FieldsOut &out,
                       double *buf, hipStream_t st) {
-  const long ncell = (long)g.nx * g.ny * g.nz;
-  for (int d = 0; d < 3; d++) {
-    const long nf = (long)(g.nx + (d == 0)) * (g.ny + (d == 1)) * (g.nz + (d == 2));
-    hipLaunchKernelGGL(fe_vjp_faces<NC>, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st,
-                       g, w, v, d, buf, nf);
-    hipLaunchKernelGGL(fe_vjp_gather<NC>, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
-                       st, g, d, (const double *)buf, nf, out, d > 0 ? 1 : 0);
+  const VjpPlan p = vjp_plan(g);
+  const long nlines = (long)g.ny * g.nz;
+  const long ncell = (long)g.nx * nlines;
+  const int nxb = (g.nx + 63) / 64;
+  constexpr long NV = 5 + NC;
+  double *side = buf;
+  Fields wp = w, vp = v;
+  FieldsOut op = out;
+  const unsigned tb = (unsigned)((ncell + 255) / 256);
+  if constexpr (NC > 0) {
+    double *pw = buf + p.side, *pv = pw + NC * ncell, *po = pv + NC * ncell;
+    hipLaunchKernelGGL(cells_to_planes<NC>, dim3(tb), dim3(256), 0, st, w.chem, pw, ncell);
+    hipLaunchKernelGGL(cells_to_planes<NC>, dim3(tb), dim3(256), 0, st, v.chem, pv, ncell);
+    wp.chem = pw;
+    vp.chem = pv;
+    op.chem = po;
   }
+  hipLaunchKernelGGL(fe_vjp_sweep_x<NC>, dim3((unsigned)((nlines + p.lps - 1) / p.lps)),
+                     dim3(64), 0, st, g, wp, vp, op, side, p.lps, 0);
+  hipLaunchKernelGGL(fe_vjp_fix_x<NC>, dim3((unsigned)((nlines * NV + 255) / 256)), dim3(256), 0,
+                     st, g, (const double *)side, op);
+  hipLaunchKernelGGL((fe_vjp_sweep_yz<1, NC>), dim3((unsigned)(nxb * g.nz * p.nseg[1])), dim3(64),
+                     0, st, g, wp, vp, op, side, p.nseg[1], 1);
+  hipLaunchKernelGGL((fe_vjp_fix_yz<1, NC>),
+                     dim3((unsigned)(((long)g.nx * g.nz * NV + 255) / 256)), dim3(256), 0, st, g,
+                     (const double *)side, op, p.nseg[1]);
+  hipLaunchKernelGGL((fe_vjp_sweep_yz<2, NC>), dim3((unsigned)(nxb * g.ny * p.nseg[2])), dim3(64),
+                     0, st, g, wp, vp, op, side, p.nseg[2], 1);
+  hipLaunchKernelGGL((fe_vjp_fix_yz<2, NC>),
+                     dim3((unsigned)(((long)g.nx * g.ny * NV + 255) / 256)), dim3(256), 0, st, g,
+                     (const double *)side, op, p.nseg[2]);
+  if constexpr (NC > 0)
+    hipLaunchKernelGGL(planes_to_cells<NC>, dim3(tb), dim3(256), 0, st, (const double *)op.chem,
+                       out.chem, ncell);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 size_t fe_vjp_buffer_doubles(const Grid &g) {
-  const long nf = (long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-  return (size_t)6 * (5 + g.nchem) * (size_t)nf;
+  return vjp_plan(g).side + (size_t)3 * g.nchem * g.nx * g.ny * g.nz;
 }
 
 int fe_vjp_launch(const Grid &g, const Fields &w, const Fields &v, const FieldsOut &out,

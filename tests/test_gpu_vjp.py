@@ -93,7 +93,10 @@ def test_chem_vjp_conservation_cotangents(mvd, torch_gpu):
 @pytest.mark.parametrize("dims,nchem,bc", [((12, 10, 9), 0, (0,) * 6),
                                            ((11, 9, 8), 0, (1, 3, 3, 1, 3, 1)),
                                            ((10, 9, 8), 2, (0, 0, 1, 3, 0, 0)),
-                                           ((9, 8, 10), 10, (3, 1, 0, 0, 1, 3))])
+                                           ((9, 8, 10), 10, (3, 1, 0, 0, 1, 3)),
+                                           # several segments per y and z column and
+                                           # several x-lines per wave
+                                           ((18, 24, 16), 1, (1, 3, 0, 0, 3, 1))])
 def test_fe_vjp_matches_oracle(mvd, torch_gpu, dims, nchem, bc):
     torch = torch_gpu
     print(f"[fe vjp] dims={dims} nchem={nchem} bc={bc}", flush=True)
