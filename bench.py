@@ -165,6 +165,63 @@ def bench_chem(api, problems, torch, n, seed, steps, h=1e3):
     return out
 
 
+def bench_vjp(api, problems, torch, n, seed, reps):
+    """Config-4 cost ratio t(VJP fe + VJP chem) / t(fe + chem RHS) at n^3 with 10 species:
+    the blast fluid state (seed), the config-3 cell generator for the species (prepared,
+    i.e. Dengo-scaled, so they are both the advected tracers of fe() and the chemistry
+    state), cotangents ~ N(0,1). Each figure is the mean over `reps` synchronous calls,
+    timed with HIP events on the null stream the library uses."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    ncell = n ** 3
+    fluid = problems.blast3d(n, seed=seed)[:5]
+    y, _ = problems.chem_cells(ncell, seed=seed + 1)
+    ctx = api.Context(api.make_params(n, n, n, 10, (0,) * 6))
+    ctx.chem_setup(orc.load_tables(), ncell)
+    chem = torch.from_numpy(y).to("cuda")
+    ctx.chem_prepare(chem)
+    w = [torch.from_numpy(a.reshape(-1)).to("cuda") for a in fluid] + [chem]
+    gen = torch.Generator(device="cuda").manual_seed(seed + 2)
+    v = [torch.randn(a.numel(), dtype=torch.float64, device="cuda", generator=gen) for a in w]
+    wd = [torch.empty_like(a) for a in w]
+    wb = [torch.empty_like(a) for a in w]
+    f = torch.empty_like(chem)
+    yb = torch.empty_like(chem)
+    for _ in range(3):                      # temperature warm start, allocations
+        ctx.chem_rhs(chem, f)
+
+    def tm(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def chem_rhs():
+        ctx.chem_rhs(chem, f)
+
+    def chem_vjp():
+        ctx.chem_vjp(chem, v[5], yb)
+
+    t_fe = tm(lambda: ctx.fe(w, wd))
+    t_rhs = tm(chem_rhs)
+    t_fevjp = tm(lambda: ctx.fe_vjp(w, v, wb))
+    t_chvjp = tm(chem_vjp)
+    ctx.close()
+    return {
+        "workload": f"{n}^3 blast fluid (seed {seed}) with 10 prepared primordial species "
+                    f"(seed {seed + 1}), periodic, cotangents N(0,1)",
+        "fe_ms": t_fe, "chem_rhs_ms": t_rhs, "fe_vjp_ms": t_fevjp, "chem_vjp_ms": t_chvjp,
+        "fe_vjp_over_fe": t_fevjp / t_fe, "chem_vjp_over_rhs": t_chvjp / t_rhs,
+        "cost_ratio": (t_fevjp + t_chvjp) / (t_fe + t_rhs),
+        "target": "<= 3",
+    }
+
+
 def cpu_chem_baseline(n_sample, seed, threads, reps, h=1e3):
     """The CPU oracle's per-cell SDIRK3 step on config-3 cells, `reps` times from the same
     initial state; returns cells/s and the total seconds."""
@@ -202,6 +259,9 @@ def main():
     ap.add_argument("--chem-cpu-n", type=int, default=64 ** 3)
     ap.add_argument("--chem-cpu-reps", type=int, default=8)
     ap.add_argument("--no-chem", action="store_true")
+    ap.add_argument("--no-vjp", action="store_true")
+    ap.add_argument("--vjp-n", type=int, default=128)
+    ap.add_argument("--vjp-reps", type=int, default=10)
     ap.add_argument("--profile-fe", action="store_true",
                     help="for hardware-counter runs: --steps fe() evaluations, then two "
                          "copies of a 1 GiB device buffer (the known-size calibration "
@@ -297,6 +357,8 @@ def main():
 
     if not args.no_chem:
         extra["chem"] = bench_chem(api, problems, torch, args.chem_n, 4321, args.chem_steps)
+    if not args.no_vjp:
+        extra["vjp"] = bench_vjp(api, problems, torch, args.vjp_n, 777, args.vjp_reps)
 
     cpu = None
     if not args.no_cpu:

@@ -109,6 +109,15 @@ int mvd_chem_rhs(mvd_context *ctx, double t, const double *y, double *ydot);
  * 10x10 blocks J[100*cell + 10*col + row] (DENSEIDX, line 40), evaluated from the
  * temperature of the last mvd_chem_rhs call. */
 int mvd_chem_jac(mvd_context *ctx, double t, const double *y, double *J);
+/* Block linear solver of the chemistry subvector: Setup and Solve of SUNLinSol_RankLocalLS
+ * (multirate_chem_hydro_main.cpp:1348-1411; the BDMPIMV twin, imex_chem_hydro_main.cpp:
+ * 1075-1194) over MAGMA's batched getrf/getrs. lsetup factors A = I - gamma*J per cell,
+ * J in the mvd_chem_jac layout, with partial pivoting; it returns 0, or the number (> 0,
+ * recoverable, as SUNLS_LUFACT_FAIL) of blocks with an exact zero pivot, after which
+ * lsolve refuses (MVD_NOT_READY) until a successful lsetup. lsolve solves A x = b for the
+ * chemistry subvector ([cell][10]; x may equal b). */
+int mvd_chem_lsetup(mvd_context *ctx, double gamma, const double *J);
+int mvd_chem_lsolve(mvd_context *ctx, const double *b, double *x);
 /* Vector-Jacobian product of calculate_rhs_cvklu (no counterpart in the reference, which
  * has only the forward Jacobian callback; the adjoint callers are sensitivity and
  * checkpointed-adjoint drivers): ybar = (d ydot / d y)^T v in the scaled variables, for the

@@ -18,7 +18,12 @@ struct mvd_context {
   mvd::ChemState chem;
   int dirk_block = 256;   // batched DIRK launch shape (set by mvd_chem_dirk_tune)
   int dirk_waves = 0;     // resident waves of the persistent grid (0: one per SIMD)
-  double *vjp_buf = nullptr;   // fe() VJP face buffer (allocated on first use)
+  double *vjp_buf = nullptr;   // fe() VJP scratch: side buffer, tracer planes (first use)
+  // block linear solver of the chemistry subvector (allocated by the first lsetup)
+  double *ls_lu = nullptr;
+  unsigned long long *ls_piv = nullptr;
+  int *ls_nsing = nullptr;
+  bool ls_ready = false;
 };
 
 namespace {
@@ -136,6 +141,9 @@ void mvd_destroy(mvd_context *c) {
     for (int i = 0; i < 2 * mvd::PROF_NSLOT; i++) (void)hipEventDestroy(c->fe.timer.ev[i]);
   if (c->host_flag) (void)hipHostFree(c->host_flag);
   if (c->vjp_buf) (void)hipFree(c->vjp_buf);
+  if (c->ls_lu) (void)hipFree(c->ls_lu);
+  if (c->ls_piv) (void)hipFree(c->ls_piv);
+  if (c->ls_nsing) (void)hipFree(c->ls_nsing);
   mvd::chem_free(c->chem);
   delete c;
 }
@@ -301,6 +309,38 @@ int mvd_chem_jac(mvd_context *ctx, double, const double *y, double *J) {
   if (!ctx || !y || !J) return MVD_BAD_ARGUMENT;
   if (!ctx->chem.ready) return MVD_NOT_READY;
   return mvd::chem_jac(ctx->chem, y, J, ctx->stream, true);
+}
+
+int mvd_chem_lsetup(mvd_context *ctx, double gamma, const double *J) {
+  if (!ctx || !J) return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  if (!std::isfinite(gamma)) return MVD_BAD_ARGUMENT;
+  const long n = ctx->chem.n;
+  if (!ctx->ls_lu) {
+    if (!dev_ok(hipMalloc(&ctx->ls_lu, sizeof(double) * 100 * n)) ||
+        !dev_ok(hipMalloc(&ctx->ls_piv, sizeof(unsigned long long) * n)) ||
+        !dev_ok(hipMalloc(&ctx->ls_nsing, sizeof(int))))
+      return MVD_DEVICE_ERROR;
+  }
+  ctx->ls_ready = false;
+  if (mvd::chem_ls_setup(n, gamma, J, ctx->ls_lu, ctx->ls_piv, ctx->ls_nsing, ctx->stream))
+    return MVD_DEVICE_ERROR;
+  int nsing = 0;
+  if (!dev_ok(hipMemcpyAsync(&nsing, ctx->ls_nsing, sizeof(int), hipMemcpyDeviceToHost,
+                             ctx->stream)) ||
+      !dev_ok(hipStreamSynchronize(ctx->stream)))
+    return MVD_DEVICE_ERROR;
+  ctx->ls_ready = nsing == 0;
+  return nsing;
+}
+
+int mvd_chem_lsolve(mvd_context *ctx, const double *b, double *x) {
+  if (!ctx || !b || !x) return MVD_BAD_ARGUMENT;
+  if (!ctx->ls_ready) return MVD_NOT_READY;
+  if (mvd::chem_ls_solve(ctx->chem.n, ctx->ls_lu, ctx->ls_piv, b, x, ctx->stream) ||
+      !dev_ok(hipStreamSynchronize(ctx->stream)))
+    return MVD_DEVICE_ERROR;
+  return MVD_SUCCESS;
 }
 
 double *mvd_chem_state(mvd_context *ctx, int which) {

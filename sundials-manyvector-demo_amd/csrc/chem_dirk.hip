@@ -686,4 +686,65 @@ int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStrea
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ---------------------------------------------------------------------------------
+// Block linear solver of the chemistry subvector: the Setup/Solve of
+// SUNLinSol_RankLocalLS (multirate_chem_hydro_main.cpp:1348-1411) over MAGMA's batched
+// getrf/getrs. Setup forms A = I - gamma J per cell from the dense column-major blocks
+// (DENSEIDX(b,r,c) = 100b + 10c + r) and factors it with partial pivoting (first maximum,
+// LAPACK row interchanges); Solve applies the factors to one right-hand side per cell
+// ([cell][10], x may alias b). A block whose U has an exact zero on the diagonal counts as
+// singular, as getrf's info > 0.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+ls_setup_kernel(long n, double gamma, const double *J, double *LU, unsigned long long *piv,
+                int *nsing) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *jb = J + (long)NSP * NSP * i;
+  double a[NSP][NSP];
+#pragma unroll
+  for (int q = 0; q < NSP; q++)
+#pragma unroll
+    for (int r = 0; r < NSP; r++) a[r][q] = (r == q ? 1.0 : 0.0) - gamma * jb[NSP * q + r];
+  piv[i] = lu_factor_regs(a);
+  bool sing = false;
+  double *blk = LU + (long)NSP * NSP * i;
+#pragma unroll
+  for (int q = 0; q < NSP; q++) {
+    sing |= a[q][q] == 0.0;
+#pragma unroll
+    for (int r = 0; r < NSP; r++) blk[NSP * q + r] = a[r][q];
+  }
+  if (sing) atomicAdd(nsing, 1);
+}
+
+__global__ void __launch_bounds__(256)
+ls_solve_kernel(long n, const double *LU, const unsigned long long *piv, const double *b,
+                double *x) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) v[s] = b[NSP * i + s];
+  lu_solve(Mat{const_cast<double *>(LU) + (long)NSP * NSP * i, 1, NSP}, piv[i], v);
+#pragma unroll
+  for (int s = 0; s < NSP; s++) x[NSP * i + s] = v[s];
+}
+
+int chem_ls_setup(long n, double gamma, const double *J, double *LU, unsigned long long *piv,
+                  int *nsing, hipStream_t st) {
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, 1L, nsing, 0);
+  hipLaunchKernelGGL(ls_setup_kernel, dim3(nb), dim3(256), 0, st, n, gamma, J, LU, piv, nsing);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int chem_ls_solve(long n, const double *LU, const unsigned long long *piv, const double *b,
+                  double *x, hipStream_t st) {
+  hipLaunchKernelGGL(ls_solve_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, LU,
+                     piv, b, x);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace mvd

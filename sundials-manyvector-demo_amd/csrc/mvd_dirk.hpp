@@ -28,5 +28,10 @@ struct DirkDev {
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
                         int grid_waves, int block);
 int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st);
+// block linear solver of the chemistry subvector (A = I - gamma J, batched LU)
+int chem_ls_setup(long n, double gamma, const double *J, double *LU, unsigned long long *piv,
+                  int *nsing, hipStream_t st);
+int chem_ls_solve(long n, const double *LU, const unsigned long long *piv, const double *b,
+                  double *x, hipStream_t st);
 
 }  // namespace mvd

@@ -82,6 +82,8 @@ def lib():
         "mvd_chem_rhs": ([vp, C.c_double, _P, _P], C.c_int),
         "mvd_chem_jac": ([vp, C.c_double, _P, _P], C.c_int),
         "mvd_chem_state": ([vp, C.c_int], _P),
+        "mvd_chem_lsetup": ([vp, C.c_double, _P], C.c_int),
+        "mvd_chem_lsolve": ([vp, _P, _P], C.c_int),
         "mvd_chem_vjp": ([vp, C.c_double, _P, _P, _P, _P], C.c_int),
         "mvd_dirk_sdirk3": ([C.POINTER(DirkOpts)], C.c_int),
         "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
@@ -224,6 +226,16 @@ class Context:
 
     def chem_jac(self, y, J, t=0.0):
         _check("mvd_chem_jac", lib().mvd_chem_jac(self.h, t, dptr(y), dptr(J)))
+
+    def chem_lsetup(self, gamma, J):
+        """Factor A = I - gamma*J per cell; returns the number of singular blocks (0: ok)."""
+        rc = lib().mvd_chem_lsetup(self.h, gamma, dptr(J))
+        if rc < 0:
+            _check("mvd_chem_lsetup", rc)
+        return rc
+
+    def chem_lsolve(self, b, x):
+        _check("mvd_chem_lsolve", lib().mvd_chem_lsolve(self.h, dptr(b), dptr(x)))
 
     def chem_vjp(self, y, v, ybar, ydot=None, t=0.0):
         _check("mvd_chem_vjp", lib().mvd_chem_vjp(

@@ -125,3 +125,48 @@ def test_fslow_matches_oracle(mvd, torch_gpu):
     assert normwise_err(wd[4].cpu().numpy(), wo[4]) <= 1e-15
     assert np.array_equal(wd[5].cpu().numpy(), wo[5])
     ctx.close()
+
+
+def test_chem_block_ls_matches_scipy(mvd, torch_gpu):
+    """mvd_chem_lsetup / mvd_chem_lsolve (A = I - gamma J per cell, partial pivoting) against
+    scipy's getrf/getrs on the same blocks, on the Jacobian of config-3 cells at the
+    SDIRK3 diagonal of a 1e3 s step; then a block made exactly singular is reported."""
+    import scipy.linalg as sl
+    torch = torch_gpu
+    n = 2048
+    print(f"[chem ls] n={n}", flush=True)
+    y, _ = problems.chem_cells(n, seed=21)
+    ctx, yd = gpu_chem(mvd, torch, n, y)
+    rd = torch.empty_like(yd)
+    ctx.chem_rhs(yd, rd)
+    Jd = torch.empty(100 * n, dtype=torch.float64, device="cuda")
+    ctx.chem_jac(yd, Jd)
+    gamma = 1e3 * 0.435866521508459
+    assert ctx.chem_lsetup(gamma, Jd) == 0
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(10 * n)
+    bd = torch.from_numpy(b).to("cuda")
+    xd = torch.empty_like(bd)
+    ctx.chem_lsolve(bd, xd)
+    x = xd.cpu().numpy().reshape(n, 10)
+    Jc = Jd.cpu().numpy().reshape(n, 10, 10).transpose(0, 2, 1)      # [cell][row][col]
+    A = np.eye(10)[None] - gamma * Jc
+    bb = b.reshape(n, 10)
+    xr = np.stack([sl.lu_solve(sl.lu_factor(A[i]), bb[i]) for i in range(n)])
+    res = np.abs(np.einsum("nij,nj->ni", A, x) - bb).max(axis=1)
+    scale = (np.abs(A) * np.abs(x)[:, None, :]).sum(axis=2).max(axis=1)
+    assert np.all(res <= 1e-13 * scale)
+    rel = np.abs(x - xr).max(axis=1) / np.abs(xr).max(axis=1)
+    cond = np.linalg.cond(A)
+    assert np.all(rel <= 1e-15 * cond + 1e-13)
+    # in place (x aliases b)
+    ctx.chem_lsolve(bd, bd)
+    assert np.array_equal(bd.cpu().numpy().reshape(n, 10), x)
+    # two singular blocks: J = I / gamma makes A = 0
+    Jm = Jd.clone().reshape(n, 10, 10)
+    Jm[3] = torch.eye(10, dtype=torch.float64, device="cuda") / gamma
+    Jm[17] = torch.eye(10, dtype=torch.float64, device="cuda") / gamma
+    assert ctx.chem_lsetup(gamma, Jm.reshape(-1)) == 2
+    with pytest.raises(mvd.MvdError):
+        ctx.chem_lsolve(bd, xd)
+    ctx.close()
