@@ -109,6 +109,38 @@ int mvd_chem_rhs(mvd_context *ctx, double t, const double *y, double *ydot);
  * 10x10 blocks J[100*cell + 10*col + row] (DENSEIDX, line 40), evaluated from the
  * temperature of the last mvd_chem_rhs call. */
 int mvd_chem_jac(mvd_context *ctx, double t, const double *y, double *J);
+/* N_Vector operations on device arrays, as the integrators' stage algebra uses them (the
+ * ManyVector ops of SUNDIALS 6.2.0 that ARKStep/MRIStep call; no counterpart in the
+ * reference tree, which links them from the library).
+ * N_VLinearCombination: z = sum_k c[k] X[k] over len doubles, summed in k order; z may be
+ * X[0] (no other X[k]). c is a host array of nv coefficients, X a host array of device
+ * pointers. */
+int mvd_nvec_linear_combination(mvd_context *ctx, int nv, const double *c,
+                                const double *const *X, double *z, long len);
+/* N_VWrmsNorm of a ManyVector of nsub (<= 8) subvectors: sqrt(sum_q sum_i (x_q[i] w_q[i])^2 /
+ * sum_q len[q]), deterministic; x, w host arrays of device pointers, len host array. */
+int mvd_nvec_wrms_norm(mvd_context *ctx, int nsub, const double *const *x,
+                       const double *const *w, const long *len, double *out);
+
+/* Explicit Runge-Kutta table (up to 6 stages; A row-major 6x6, strictly lower triangular). */
+typedef struct {
+  int stages;
+  double A[36];
+  double b[6];
+  double c[6];
+} mvd_erk_table;
+/* ARKStep's default third-order explicit table, Bogacki-Shampine 3(2) (4 stages), the one the
+ * fluid driver runs with --order=3 (euler3D_main.cpp:209). */
+int mvd_erk_bs3(mvd_erk_table *tab);
+/* Fixed-step explicit evolution of fEuler (euler3D_main.cpp:194 ARKStepCreate with fEuler
+ * explicit, :280 ARKStepSetFixedStep, :357 ARKStepEvolve): nsteps steps of size h, in place
+ * on w (device arrays of the ManyVector). Stages z_i = y + sum_{j<i} h a_ij F_j, F_i = fe(z_i),
+ * y <- y + sum_i h b_i F_i; one step is captured as a device graph and replayed. Returns 0,
+ * or MVD_ILLEGAL_STATE when any stage met an illegal state (w then holds the state after
+ * nsteps steps, as an unchecked fixed-step run would), or an error code. */
+int mvd_erk_evolve(mvd_context *ctx, const mvd_erk_table *tab, double t0, double h, int nsteps,
+                   double *const *w);
+
 /* Block linear solver of the chemistry subvector: Setup and Solve of SUNLinSol_RankLocalLS
  * (multirate_chem_hydro_main.cpp:1348-1411; the BDMPIMV twin, imex_chem_hydro_main.cpp:
  * 1075-1194) over MAGMA's batched getrf/getrs. lsetup factors A = I - gamma*J per cell,

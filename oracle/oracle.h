@@ -23,6 +23,14 @@ typedef struct {
   double rhs_scale;      /* 1 (frhs) or TimeUnits (ffast / fimpl) */
 } orc_dirk;
 int orc_chem_dirk_step(void *chem, const orc_dirk *opt, double h, double *y, int *iters);
+/* fixed-step explicit Runge-Kutta evolution of fe() (orc_erk.cpp) */
+typedef struct {
+  int stages;
+  double A[36];          /* 6x6 row-major, strictly lower triangular */
+  double b[6];
+  double c[6];
+} orc_erk;
+int orc_erk_evolve(const orc_grid *g, const orc_erk *tab, double h, int nsteps, double *const *w);
 int orc_lu(const double *M, double *LU, int *piv, const double *b, double *x);
 /* bitwise check of the early exit of the device temperature solve (orc_chem.cpp) */
 long orc_temperature_exit_check(void *chem, const double *y, const double *Ts0, int *steps);

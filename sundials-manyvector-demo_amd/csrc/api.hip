@@ -24,6 +24,7 @@ struct mvd_context {
   unsigned long long *ls_piv = nullptr;
   int *ls_nsing = nullptr;
   bool ls_ready = false;
+  mvd::ErkState erk;                  // fixed-step ERK evolution (first use)
 };
 
 namespace {
@@ -144,6 +145,7 @@ void mvd_destroy(mvd_context *c) {
   if (c->ls_lu) (void)hipFree(c->ls_lu);
   if (c->ls_piv) (void)hipFree(c->ls_piv);
   if (c->ls_nsing) (void)hipFree(c->ls_nsing);
+  mvd::erk_free(c->erk);
   mvd::chem_free(c->chem);
   delete c;
 }
@@ -341,6 +343,71 @@ int mvd_chem_lsolve(mvd_context *ctx, const double *b, double *x) {
       !dev_ok(hipStreamSynchronize(ctx->stream)))
     return MVD_DEVICE_ERROR;
   return MVD_SUCCESS;
+}
+
+// ---- N_Vector operations and explicit evolution of fe() ----
+
+int mvd_nvec_linear_combination(mvd_context *ctx, int nv, const double *c,
+                                const double *const *X, double *z, long len) {
+  if (!ctx || nv < 1 || !c || !X || !z || len < 0) return MVD_BAD_ARGUMENT;
+  for (int k = 0; k < nv; k++)
+    if (!X[k]) return MVD_BAD_ARGUMENT;
+  for (int k = 1; k < nv; k++)
+    if (X[k] == z) return MVD_BAD_ARGUMENT;     // only X[0] may be the output
+  if (mvd::linear_combination_launch(nv, c, X, z, len, ctx->stream) ||
+      !dev_ok(hipStreamSynchronize(ctx->stream)))
+    return MVD_DEVICE_ERROR;
+  return MVD_SUCCESS;
+}
+
+int mvd_nvec_wrms_norm(mvd_context *ctx, int nsub, const double *const *x,
+                       const double *const *w, const long *len, double *out) {
+  if (!ctx || nsub < 1 || nsub > mvd::WRMS_MAXSUB || !x || !w || !len || !out)
+    return MVD_BAD_ARGUMENT;
+  for (int q = 0; q < nsub; q++)
+    if (len[q] < 0 || (len[q] > 0 && (!x[q] || !w[q]))) return MVD_BAD_ARGUMENT;
+  return mvd::wrms_norm_launch(nsub, x, w, len, ctx->fe, ctx->stream, out) ? MVD_DEVICE_ERROR
+                                                                           : MVD_SUCCESS;
+}
+
+int mvd_erk_bs3(mvd_erk_table *t) {
+  if (!t) return MVD_BAD_ARGUMENT;
+  std::memset(t, 0, sizeof(*t));
+  // Bogacki & Shampine (1989): c = (0, 1/2, 3/4, 1); b = a_4j = (2/9, 1/3, 4/9, 0)
+  t->stages = 4;
+  t->A[6 * 1 + 0] = 1.0 / 2.0;
+  t->A[6 * 2 + 1] = 3.0 / 4.0;
+  t->A[6 * 3 + 0] = 2.0 / 9.0;
+  t->A[6 * 3 + 1] = 1.0 / 3.0;
+  t->A[6 * 3 + 2] = 4.0 / 9.0;
+  t->b[0] = 2.0 / 9.0;
+  t->b[1] = 1.0 / 3.0;
+  t->b[2] = 4.0 / 9.0;
+  t->c[1] = 0.5;
+  t->c[2] = 0.75;
+  t->c[3] = 1.0;
+  return MVD_SUCCESS;
+}
+
+int mvd_erk_evolve(mvd_context *ctx, const mvd_erk_table *tab, double, double h, int nsteps,
+                   double *const *w) {
+  if (!ctx || !tab || nsteps < 0 || !std::isfinite(h)) return MVD_BAD_ARGUMENT;
+  if (tab->stages < 1 || tab->stages > mvd::ERK_MAXSTAGE) return MVD_BAD_ARGUMENT;
+  for (int i = 0; i < tab->stages; i++)
+    for (int j = i; j < mvd::ERK_MAXSTAGE; j++)
+      if (tab->A[6 * i + j] != 0.0) return MVD_BAD_ARGUMENT;   // explicit tables only
+  int rc = check_ptrs(ctx, (const double *const *)w);
+  if (rc) return rc;
+  if (nsteps == 0) return MVD_SUCCESS;
+  mvd::ErkTable t;
+  t.stages = tab->stages;
+  for (int k = 0; k < 36; k++) t.A[k] = tab->A[k];
+  for (int k = 0; k < 6; k++) { t.b[k] = tab->b[k]; t.c[k] = tab->c[k]; }
+  rc = mvd::erk_evolve(ctx->g, ctx->fe, ctx->erk, t, h, nsteps, fields_of(ctx, w),
+                       fields_out(ctx, w), ctx->stream, ctx->host_flag);
+  if (rc == -2) return MVD_UNSUPPORTED;
+  if (rc) return MVD_DEVICE_ERROR;
+  return *ctx->host_flag ? MVD_ILLEGAL_STATE : MVD_SUCCESS;
 }
 
 double *mvd_chem_state(mvd_context *ctx, int which) {

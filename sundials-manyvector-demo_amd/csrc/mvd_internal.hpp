@@ -74,4 +74,32 @@ size_t fe_vjp_buffer_doubles(const Grid &g);
 int fe_vjp_launch(const Grid &g, const Fields &w, const Fields &v, const FieldsOut &out,
                   double *buf, hipStream_t st);
 
+// N_Vector operations (nvec.hip)
+constexpr int LC_MAX = 16;        // vectors per linear-combination launch
+constexpr int WRMS_MAXSUB = 8;    // subvectors per WRMS norm
+int linear_combination_launch(int nv, const double *c, const double *const *X, double *z,
+                              long len, hipStream_t st);
+int wrms_norm_launch(int nsub, const double *const *x, const double *const *w, const long *len,
+                     FeScratch &s, hipStream_t st, double *host_out);
+
+// fixed-step explicit Runge-Kutta evolution of fe() (erk.hip)
+constexpr int ERK_MAXSTAGE = 6;
+struct ErkTable {
+  int stages = 0;
+  double A[ERK_MAXSTAGE * ERK_MAXSTAGE] = {};
+  double b[ERK_MAXSTAGE] = {}, c[ERK_MAXSTAGE] = {};
+};
+struct ErkState {
+  double *Y = nullptr, *Z = nullptr, *K = nullptr;   // one allocation: Y, Z, K_0..K_5
+  unsigned *flag = nullptr;
+  hipStream_t cap = nullptr;                         // capture stream
+  hipGraphExec_t exec = nullptr;                     // one captured step
+  double h = 0.0;
+  ErkTable tab;
+};
+int erk_evolve(const Grid &g, FeScratch &fe, ErkState &e, const ErkTable &tab, double h,
+               int nsteps, const Fields &w_in, const FieldsOut &w_out, hipStream_t st,
+               unsigned *host_flag);
+void erk_free(ErkState &e);
+
 }  // namespace mvd

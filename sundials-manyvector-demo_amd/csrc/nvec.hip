@@ -1,7 +1,8 @@
 // Reductions and small elementwise kernels around fe(): the CFL wave-speed reduction
 // (stability, utilities.cpp:483-528), the fslow energy/scaling glue
 // (multirate_chem_hydro_main.cpp:1024-1077) and the Dengo scaling helpers
-// (primordial_blast.cpp:404-458).
+// (primordial_blast.cpp:404-458), and the N_Vector operations of the integrators' stage
+// algebra (linear combination, WRMS norm).
 //
 // Reductions are two-pass and deterministic: each block reduces its grid-stride share
 // with 64-lane wavefront shuffles and one LDS exchange, writes one partial; a single
@@ -122,6 +123,96 @@ int chem_scale_launch(long n, double *chem, const double *scale, int apply, hipS
   hipLaunchKernelGGL(chem_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, chem,
                      scale, apply);
   return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// ---------------------------------------------------------------------------------
+// N_Vector operations used by the integrators' stage algebra on the ManyVector's device
+// arrays: N_VLinearCombination (z = sum_i c_i X_i, summed in index order; z may be X_0) and
+// N_VWrmsNorm of a ManyVector (sqrt(sum over every subvector of (x w)^2 / total length),
+// two-pass and deterministic).
+// ---------------------------------------------------------------------------------
+struct LcArgs {
+  const double *x[LC_MAX];
+  double c[LC_MAX];
+};
+
+__global__ void __launch_bounds__(256)
+lin_comb(long len, int nv, LcArgs a, double *z, int add) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double acc = add ? z[i] : a.c[0] * a.x[0][i];
+  for (int k = add ? 0 : 1; k < nv; k++) acc += a.c[k] * a.x[k][i];
+  z[i] = acc;
+}
+
+int linear_combination_launch(int nv, const double *c, const double *const *X, double *z,
+                              long len, hipStream_t st) {
+  if (len <= 0) return 0;
+  const unsigned nb = (unsigned)((len + 255) / 256);
+  for (int k0 = 0; k0 < nv; k0 += LC_MAX) {
+    LcArgs a{};
+    const int m = nv - k0 < LC_MAX ? nv - k0 : LC_MAX;
+    for (int k = 0; k < m; k++) {
+      a.x[k] = X[k0 + k];
+      a.c[k] = c[k0 + k];
+    }
+    hipLaunchKernelGGL(lin_comb, dim3(nb), dim3(256), 0, st, len, m, a, z, k0 > 0 ? 1 : 0);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+struct WrmsArgs {
+  const double *x[WRMS_MAXSUB];
+  const double *w[WRMS_MAXSUB];
+  long len[WRMS_MAXSUB];
+};
+
+__global__ void __launch_bounds__(256)
+wrms_partial(int nsub, WrmsArgs a, double *partial) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  for (int q = 0; q < nsub; q++)
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.len[q];
+         i += (long)gridDim.x * blockDim.x) {
+      const double p = a.x[q][i] * a.w[q][i];
+      s += p * p;
+    }
+  s = block_reduce<false>(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256)
+wrms_final(const double *partial, int n, double inv_len, double *out) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+  s = block_reduce<false>(s);
+  if (threadIdx.x == 0) out[0] = sqrt(s * inv_len);
+}
+
+int wrms_norm_launch(int nsub, const double *const *x, const double *const *w, const long *len,
+                     FeScratch &s, hipStream_t st, double *host_out) {
+  WrmsArgs a{};
+  long total = 0, maxlen = 0;
+  for (int q = 0; q < nsub; q++) {
+    a.x[q] = x[q];
+    a.w[q] = w[q];
+    a.len[q] = len[q];
+    total += len[q];
+    if (len[q] > maxlen) maxlen = len[q];
+  }
+  if (total <= 0) { *host_out = 0.0; return 0; }
+  int nb = (int)std::min<long>((maxlen + 255) / 256, 2048);
+  if (nb < 1) nb = 1;
+  if (s.red_len < nb + 1) return -3;
+  hipLaunchKernelGGL(wrms_partial, dim3(nb), dim3(256), 0, st, nsub, a, s.red);
+  hipLaunchKernelGGL(wrms_final, dim3(1), dim3(256), 0, st, (const double *)s.red, nb,
+                     1.0 / (double)total, s.red + nb);
+  if (hipMemcpyAsync(host_out, s.red + nb, sizeof(double), hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return -3;
+  return 0;
 }
 
 }  // namespace mvd

@@ -36,6 +36,12 @@ class DirkOpts(C.Structure):
                 ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double)]
 
 
+class ErkTable(C.Structure):
+    """mvd_erk_table: explicit Butcher table, up to 6 stages (row-major 6x6 A)."""
+    _fields_ = [("stages", C.c_int), ("A", C.c_double * 36), ("b", C.c_double * 6),
+                ("c", C.c_double * 6)]
+
+
 class DirkStats(C.Structure):
     _fields_ = [("newton_iters", C.c_long), ("max_cell_iters", C.c_int),
                 ("failed_cells", C.c_long), ("setups", C.c_long), ("hist", C.c_long * 16),
@@ -83,6 +89,13 @@ def lib():
         "mvd_chem_jac": ([vp, C.c_double, _P, _P], C.c_int),
         "mvd_chem_state": ([vp, C.c_int], _P),
         "mvd_chem_lsetup": ([vp, C.c_double, _P], C.c_int),
+        "mvd_nvec_linear_combination": ([vp, C.c_int, _P, C.POINTER(_P), _P, C.c_long],
+                                        C.c_int),
+        "mvd_nvec_wrms_norm": ([vp, C.c_int, C.POINTER(_P), C.POINTER(_P),
+                                C.POINTER(C.c_long), C.POINTER(C.c_double)], C.c_int),
+        "mvd_erk_bs3": ([C.POINTER(ErkTable)], C.c_int),
+        "mvd_erk_evolve": ([vp, C.POINTER(ErkTable), C.c_double, C.c_double, C.c_int,
+                           C.POINTER(_P)], C.c_int),
         "mvd_chem_lsolve": ([vp, _P, _P], C.c_int),
         "mvd_chem_vjp": ([vp, C.c_double, _P, _P, _P, _P], C.c_int),
         "mvd_dirk_sdirk3": ([C.POINTER(DirkOpts)], C.c_int),
@@ -128,6 +141,12 @@ def _check(fn, rc, allow=(MVD_SUCCESS,)):
     return rc
 
 
+def bs3():
+    t = ErkTable()
+    _check("mvd_erk_bs3", lib().mvd_erk_bs3(C.byref(t)))
+    return t
+
+
 def make_params(nx, ny, nz, nchem=0, bc=(0,) * 6, gamma=1.4, extent=((0, 1), (0, 1), (0, 1)),
                 cfl=0.0, units=(1.0, 1.0, 1.0)):
     p = Params()
@@ -166,6 +185,29 @@ class Context:
         fn = lib().mvd_fe_plain if plain else lib().mvd_fe
         allow = (MVD_SUCCESS, MVD_ILLEGAL_STATE) if allow_illegal else (MVD_SUCCESS,)
         return _check("mvd_fe", fn(self.h, t, dptrs(w), dptrs(wdot)), allow)
+
+    def erk_evolve(self, w, h, nsteps, table=None, t0=0.0, allow_illegal=False):
+        """Fixed-step explicit RK of fe() in place on w (default table: Bogacki-Shampine 3)."""
+        tab = table if table is not None else bs3()
+        allow = (MVD_SUCCESS, MVD_ILLEGAL_STATE) if allow_illegal else (MVD_SUCCESS,)
+        return _check("mvd_erk_evolve", lib().mvd_erk_evolve(
+            self.h, C.byref(tab), t0, h, int(nsteps), dptrs(w)), allow)
+
+    def linear_combination(self, c, X, z):
+        import numpy as np
+        cc = np.ascontiguousarray(c, dtype=np.float64)
+        xs = (_P * len(X))(*[dptr(x) for x in X])
+        _check("mvd_nvec_linear_combination", lib().mvd_nvec_linear_combination(
+            self.h, len(X), cc.ctypes.data_as(_P), xs, dptr(z), z.numel()))
+
+    def wrms_norm(self, xs, ws):
+        n = len(xs)
+        xp = (_P * n)(*[dptr(x) for x in xs])
+        wp = (_P * n)(*[dptr(x) for x in ws])
+        ln = (C.c_long * n)(*[x.numel() for x in xs])
+        out = C.c_double(0.0)
+        _check("mvd_nvec_wrms_norm", lib().mvd_nvec_wrms_norm(self.h, n, xp, wp, ln, C.byref(out)))
+        return out.value
 
     def fe_async(self, w, wdot, t=0.0):
         return _check("mvd_fe_async", lib().mvd_fe_async(self.h, t, dptrs(w), dptrs(wdot)))

@@ -54,6 +54,39 @@ def sdirk3_table():
     return A, np.array([g, c2, 1.0, 0.0])
 
 
+class Erk(C.Structure):
+    """orc_erk (same layout as the library's mvd_erk_table)."""
+    _fields_ = [("stages", C.c_int), ("A", C.c_double * 36), ("b", C.c_double * 6),
+                ("c", C.c_double * 6)]
+
+
+def bs3_table():
+    """Bogacki-Shampine 3(2) (Bogacki & Shampine, Appl. Math. Lett. 2 (1989) 321-325),
+    from its published rationals: c = (0, 1/2, 3/4, 1), a21 = 1/2, a32 = 3/4,
+    a4j = b = (2/9, 1/3, 4/9, 0) (first same as last)."""
+    A = np.zeros((4, 4))
+    A[1, 0] = 1 / 2
+    A[2, 1] = 3 / 4
+    A[3, :3] = [2 / 9, 1 / 3, 4 / 9]
+    return A, np.array([2 / 9, 1 / 3, 4 / 9, 0.0]), np.array([0.0, 0.5, 0.75, 1.0])
+
+
+def erk(A, b, c):
+    s = len(b)
+    t = Erk()
+    t.stages = s
+    for i in range(s):
+        t.b[i], t.c[i] = b[i], c[i]
+        for j in range(s):
+            t.A[6 * i + j] = A[i, j]
+    return t
+
+
+def erk_evolve(g, tab, h, nsteps, w):
+    """Fixed-step explicit RK of fe() on the oracle, in place on w; returns the flag."""
+    return lib().orc_erk_evolve(C.byref(g), C.byref(tab), C.c_double(h), C.c_int(nsteps), _pp(w))
+
+
 def dirk_opts(A=None, c=None, nstage=3, rtol=1e-5, atol=1e-9, newton_tol=0.1, max_iters=5,
               max_setups=2, rhs_scale=1.0):
     if A is None:
