@@ -194,6 +194,20 @@ int mvd_dirk_sdirk3(mvd_dirk_opts *o);
  * multi-kernel path. Returns 0, or the number of failed cells (> 0, recoverable). */
 int mvd_chem_dirk_step(mvd_context *ctx, double t, double h, const mvd_dirk_opts *o, double *y,
                        int plain, mvd_dirk_stats *stats);
+/* Checkpointed discrete adjoint of nsteps chemistry DIRK steps (no counterpart in the
+ * reference, which has no adjoint; for sensitivity drivers, e.g. of the domain-integrated
+ * energy to the initial species). y (scaled, [cell][10]) is advanced nsteps steps of size h
+ * by a thread-per-cell run of the plain solver's per-cell algorithm, keeping the state every
+ * ckpt_every steps; then lam0 = (d y_N / d y_0)^T lamT (both [cell][10]) for the stage
+ * equations solved exactly along that trajectory, with the exact stage Jacobians; each
+ * segment is recomputed from its checkpoint. o must be stiffly accurate (the last row of A
+ * sums to 1, as mvd_dirk_sdirk3). Ts/dTs end at the forward run's final state. Returns 0,
+ * the number (> 0) of cells whose forward Newton budget ran out (lam0 is still computed
+ * along the computed trajectory), or an error code. Device scratch held by the context:
+ * (ceil(nsteps/ckpt_every) + ckpt_every) * 12 + 40 doubles per cell. */
+int mvd_chem_dirk_adjoint(mvd_context *ctx, double t, double h, int nsteps, int ckpt_every,
+                          const mvd_dirk_opts *o, double *y, const double *lamT, double *lam0);
+
 /* Autotune the batched kernel's launch shape (threads per block, resident waves per CU) by
  * timing one step on a scratch copy of y; the per-cell temperatures are restored. The
  * choice is kept in the context; returns the best time in ms through best_ms. */

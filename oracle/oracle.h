@@ -23,6 +23,11 @@ typedef struct {
   double rhs_scale;      /* 1 (frhs) or TimeUnits (ffast / fimpl) */
 } orc_dirk;
 int orc_chem_dirk_step(void *chem, const orc_dirk *opt, double h, double *y, int *iters);
+/* discrete adjoint of nsteps DIRK steps (orc_dirk.cpp); exact scaled Jacobian of one cell
+ * (orc_chem.cpp) */
+int orc_chem_dirk_adjoint(void *chem, const orc_dirk *opt, double h, int nsteps, double *y,
+                          const double *lamT, double *lam0);
+int orc_chem_exact_jac_cell(void *chem, long i, const double *y, double *T, double *J);
 /* fixed-step explicit Runge-Kutta evolution of fe() (orc_erk.cpp) */
 typedef struct {
   int stages;

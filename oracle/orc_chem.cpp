@@ -307,58 +307,86 @@ inline DualN log(const DualN &a) {
 // likewise (its T-slope from the interpolated gamma tables), dT/dy = -(dg/dy)/(dg/dT).
 // fixed_T != 0 drops the temperature path (the derivative at fixed rates). Temperatures are
 // solved from the warm start and stored, as orc_chem_rhs does.
+// Exact physical Jacobian J[s][q] = d f_s / d y_q of one cell at the physical state ya,
+// with T solved from the warm start T (updated, with dT): the RHS in forward mode with
+// tangents along the ten inputs and T, and dT/dy from the temperature equation unless
+// fixed_T or the temperature is clamped.
+static void exact_jac_cell(const Chem *c, long i, const double (&ya)[NSP], double &T, double &dT,
+                           int fixed_T, double (&J)[NSP][NSP]) {
+  const TableGrid &g = c->g;
+  temperature(g, ya, T, dT);
+  double rs[NRATE], drs[NRATE];
+  interp_rates(g, T, rs, drs);
+  DualN yd[NSP], kd[NRATE], f[NSP];
+  for (int s = 0; s < NSP; s++) { yd[s] = DualN(ya[s]); yd[s].d[s] = 1.0; }
+  for (int r = 0; r < NRATE; r++) { kd[r] = DualN(rs[r]); kd[r].d[NSP] = drs[r]; }
+  DualN Td(T);
+  Td.d[NSP] = 1.0;
+  rhs_physical(yd, kd, Td, DualN(c->z), aux_of(c, i), f);
+  double dTdy[NSP] = {};
+  const bool clamped = !(T > g.tmin && T < g.tmax);
+  if (!fixed_T && !clamped) {
+    // g(T, y) with the interpolated gamma_H2 tables
+    const double kb = 1.3806504e-16, mh = 1.67e-24, gm1 = 1.0 / (5.0 / 3.0 - 1.0);
+    const int b = table_bin(g, std::log(T));
+    const double t1 = g.lb + double(b) * g.dbin, t2 = g.lb + double(b + 1) * g.dbin;
+    const DualN Tdef = (log(Td) - DualN(t1)) / DualN(t2 - t1);
+    auto lin = [&](int tab) {
+      const double *a = g.tab + tab * TLEN;
+      return DualN(a[b]) + Tdef * DualN(a[b + 1] - a[b]);
+    };
+    const DualN g1 = DualN(1.0) / (lin(G_GH2_1) - DualN(1.0));
+    const DualN g2 = DualN(1.0) / (lin(G_GH2_2) - DualN(1.0));
+    const DualN nsum = yd[0] * g1 + yd[1] * g2 +
+                       DualN(gm1) * (yd[2] + yd[3] + yd[4] + yd[5] + yd[6] + yd[7] + yd[8]);
+    const DualN density = DualN(2.0) * (yd[0] + yd[1]) +
+                          DualN(1.0079400000000001) * (yd[2] + yd[3] + yd[4]) +
+                          DualN(4.0026020000000004) * (yd[5] + yd[6] + yd[7]);
+    const DualN gg = Td * DualN(kb) * nsum / (density * DualN(mh)) - yd[9];
+    for (int q = 0; q < NSP; q++) dTdy[q] = -gg.d[q] / gg.d[NSP];
+  }
+  for (int s = 0; s < NSP; s++)
+    for (int q = 0; q < NSP; q++) J[s][q] = f[s].d[q] + f[s].d[NSP] * dTdy[q];
+}
+
 extern "C" int orc_chem_vjp(void *h, const double *y, const double *v, double *ybar,
                             int fixed_T) {
   Chem *c = (Chem *)h;
-  const TableGrid &g = c->g;
   for (long i = 0; i < c->n; i++) {
     const long j = NSP * i;
     double ya[NSP];
     for (int s = 0; s < NSP; s++) ya[s] = y[j + s] * c->scale[j + s];
-    double T = c->Ts[i], dT = 0.0;
-    temperature(g, ya, T, dT);
+    double T = c->Ts[i], dT = 0.0, J[NSP][NSP];
+    exact_jac_cell(c, i, ya, T, dT, fixed_T, J);
     c->Ts[i] = T;
     c->dTs_ge[i] = dT;
-    double rs[NRATE], drs[NRATE];
-    interp_rates(g, T, rs, drs);
-    DualN yd[NSP], kd[NRATE], f[NSP];
-    for (int s = 0; s < NSP; s++) { yd[s] = DualN(ya[s]); yd[s].d[s] = 1.0; }
-    for (int r = 0; r < NRATE; r++) { kd[r] = DualN(rs[r]); kd[r].d[NSP] = drs[r]; }
-    DualN Td(T);
-    Td.d[NSP] = 1.0;
-    rhs_physical(yd, kd, Td, DualN(c->z), aux_of(c, i), f);
-    double dTdy[NSP] = {};
-    const bool clamped = !(T > g.tmin && T < g.tmax);
-    if (!fixed_T && !clamped) {
-      // g(T, y) with the interpolated gamma_H2 tables
-      const double kb = 1.3806504e-16, mh = 1.67e-24, gm1 = 1.0 / (5.0 / 3.0 - 1.0);
-      const int b = table_bin(g, std::log(T));
-      const double t1 = g.lb + double(b) * g.dbin, t2 = g.lb + double(b + 1) * g.dbin;
-      const DualN Tdef = (log(Td) - DualN(t1)) / DualN(t2 - t1);
-      auto lin = [&](int tab) {
-        const double *a = g.tab + tab * TLEN;
-        return DualN(a[b]) + Tdef * DualN(a[b + 1] - a[b]);
-      };
-      const DualN g1 = DualN(1.0) / (lin(G_GH2_1) - DualN(1.0));
-      const DualN g2 = DualN(1.0) / (lin(G_GH2_2) - DualN(1.0));
-      const DualN nsum = yd[0] * g1 + yd[1] * g2 +
-                         DualN(gm1) * (yd[2] + yd[3] + yd[4] + yd[5] + yd[6] + yd[7] + yd[8]);
-      const DualN density = DualN(2.0) * (yd[0] + yd[1]) +
-                            DualN(1.0079400000000001) * (yd[2] + yd[3] + yd[4]) +
-                            DualN(4.0026020000000004) * (yd[5] + yd[6] + yd[7]);
-      const DualN gg = Td * DualN(kb) * nsum / (density * DualN(mh)) - yd[9];
-      for (int q = 0; q < NSP; q++) dTdy[q] = -gg.d[q] / gg.d[NSP];
-    }
     for (int q = 0; q < NSP; q++) {
       double acc = 0.0;
       for (int s = 0; s < NSP; s++) {
-        const double Jsq = f[s].d[q] + f[s].d[NSP] * dTdy[q];     // physical d f_s / d y_q
         double w = v[j + s] * c->inv_scale[j + s];
         if (s == NSP - 1) w *= c->inv_mdensity[i];
-        acc += w * Jsq;
+        acc += w * J[s][q];
       }
       ybar[j + q] = acc * c->scale[j + q];
     }
+  }
+  return 0;
+}
+
+// The scaled Jacobian dF/dy of one cell (F = calculate_rhs_cvklu in scaled variables, the
+// mvd_chem_rhs output), exact, at the scaled state y; the temperature is solved from the
+// warm start *T (updated). J row-major [s][q].
+extern "C" int orc_chem_exact_jac_cell(void *h, long i, const double *y, double *T,
+                                       double *Jout) {
+  Chem *c = (Chem *)h;
+  const long j = NSP * i;
+  double ya[NSP], J[NSP][NSP], dT = 0.0;
+  for (int s = 0; s < NSP; s++) ya[s] = y[s] * c->scale[j + s];
+  exact_jac_cell(c, i, ya, *T, dT, 0, J);
+  for (int s = 0; s < NSP; s++) {
+    double rs = c->inv_scale[j + s];
+    if (s == NSP - 1) rs *= c->inv_mdensity[i];
+    for (int q = 0; q < NSP; q++) Jout[s * NSP + q] = J[s][q] * rs * c->scale[j + q];
   }
   return 0;
 }

@@ -126,66 +126,166 @@ extern "C" int orc_lu(const double *Min, double *LUout, int *piv, const double *
   return 0;
 }
 
+// One DIRK step of one cell; y (10, scaled) is overwritten with y_{n+1}, the cell's Ts/dTs
+// follow its RHS evaluations. Y (may be null) receives the stage values. Returns false when
+// the Newton budget ran out; nit counts the Newton iterations.
+static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[NSP],
+                      double (*Y)[NSP], int &nit) {
+  const int ns = opt->nstage;
+  double yn[NSP], z[NSP], F[4][NSP], f[NSP];
+  for (int s = 0; s < NSP; s++) yn[s] = z[s] = y[s];
+  cell_rhs(cl, yn, opt->rhs_scale, f);               // refresh Ts at y_n for J
+  double M[NSP][NSP];
+  int piv[NSP];
+  bool ok_cell = true;
+  nit = 0;
+  for (int st = 0; st < ns && ok_cell; st++) {
+    const double hg = hstep * opt->A[st * 4 + st];
+    if (st == 0) {
+      cell_matrix(cl, yn, hg, opt->rhs_scale, M);
+      lu_factor(M, piv);
+    }
+    double r[NSP];
+    for (int s = 0; s < NSP; s++) {
+      double acc = 0.0;
+      for (int j = 0; j < st; j++) acc += opt->A[st * 4 + j] * F[j][s];
+      r[s] = yn[s] + hstep * acc;
+    }
+    int setups = 0;
+    bool conv = false;
+    while (!conv) {
+      for (int it = 0; it < opt->max_iters; it++) {
+        cell_rhs(cl, z, opt->rhs_scale, f);
+        double d[NSP];
+        for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
+        lu_solve(M, piv, d);
+        for (int s = 0; s < NSP; s++) z[s] += d[s];
+        nit++;
+        if (wrms(d, z, opt->rtol, opt->atol) <= opt->newton_tol) { conv = true; break; }
+      }
+      if (conv) break;
+      if (++setups > opt->max_setups) { ok_cell = false; break; }
+      cell_rhs(cl, z, opt->rhs_scale, f);             // temperature at the iterate
+      cell_matrix(cl, z, hg, opt->rhs_scale, M);
+      lu_factor(M, piv);
+    }
+    for (int s = 0; s < NSP; s++) F[st][s] = (z[s] - r[s]) / hg;
+    if (Y)
+      for (int s = 0; s < NSP; s++) Y[st][s] = z[s];
+  }
+  for (int s = 0; s < NSP; s++) y[s] = z[s];
+  return ok_cell;
+}
+
+static Cell cell_of(ChemState *c, long i) {
+  Cell cl{c, i, c->Ts[i], c->dTs_ge[i], {}, {}, {c->mdensity[i], c->inv_mdensity[i],
+                                                c->cie_oda[i], c->h2_oda[i]}};
+  for (int s = 0; s < NSP; s++) {
+    cl.sc[s] = c->scale[NSP * i + s];
+    cl.isc[s] = c->inv_scale[NSP * i + s];
+  }
+  return cl;
+}
+
 // One DIRK step of every cell; y (n x 10, scaled) is overwritten with y_{n+1}; Ts is left
 // at the temperature of the last RHS evaluation of each cell. iters[i] receives the total
 // Newton iterations of cell i (may be null). Returns the number of failed cells.
 extern "C" int orc_chem_dirk_step(void *h, const orc_dirk *opt, double hstep, double *y,
                                   int *iters) {
   ChemState *c = (ChemState *)h;
-  const int ns = opt->nstage;
   int failed = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : failed)
   for (long i = 0; i < c->n; i++) {
-    Cell cl{c, i, c->Ts[i], c->dTs_ge[i], {}, {}, {c->mdensity[i], c->inv_mdensity[i],
-                                                  c->cie_oda[i], c->h2_oda[i]}};
-    for (int s = 0; s < NSP; s++) {
-      cl.sc[s] = c->scale[NSP * i + s];
-      cl.isc[s] = c->inv_scale[NSP * i + s];
-    }
-    double yn[NSP], z[NSP], F[4][NSP], f[NSP];
-    for (int s = 0; s < NSP; s++) yn[s] = z[s] = y[NSP * i + s];
-    cell_rhs(cl, yn, opt->rhs_scale, f);               // refresh Ts at y_n for J
-    double M[NSP][NSP];
-    int piv[NSP];
-    bool ok_cell = true;
+    Cell cl = cell_of(c, i);
+    double z[NSP];
+    for (int s = 0; s < NSP; s++) z[s] = y[NSP * i + s];
     int nit = 0;
-    for (int st = 0; st < ns && ok_cell; st++) {
-      const double hg = hstep * opt->A[st * 4 + st];
-      if (st == 0) {
-        cell_matrix(cl, yn, hg, opt->rhs_scale, M);
-        lu_factor(M, piv);
-      }
-      double r[NSP];
-      for (int s = 0; s < NSP; s++) {
-        double acc = 0.0;
-        for (int j = 0; j < st; j++) acc += opt->A[st * 4 + j] * F[j][s];
-        r[s] = yn[s] + hstep * acc;
-      }
-      int setups = 0;
-      bool conv = false;
-      while (!conv) {
-        for (int it = 0; it < opt->max_iters; it++) {
-          cell_rhs(cl, z, opt->rhs_scale, f);
-          double d[NSP];
-          for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
-          lu_solve(M, piv, d);
-          for (int s = 0; s < NSP; s++) z[s] += d[s];
-          nit++;
-          if (wrms(d, z, opt->rtol, opt->atol) <= opt->newton_tol) { conv = true; break; }
-        }
-        if (conv) break;
-        if (++setups > opt->max_setups) { ok_cell = false; break; }
-        cell_rhs(cl, z, opt->rhs_scale, f);             // temperature at the iterate
-        cell_matrix(cl, z, hg, opt->rhs_scale, M);
-        lu_factor(M, piv);
-      }
-      for (int s = 0; s < NSP; s++) F[st][s] = (z[s] - r[s]) / hg;
-    }
+    const bool ok_cell = step_cell(cl, opt, hstep, z, nullptr, nit);
     if (!ok_cell) failed++;
     for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
     c->Ts[i] = cl.Ts;
     c->dTs_ge[i] = cl.dTs;
     if (iters) iters[i] = ok_cell ? nit : -1;
+  }
+  return failed;
+}
+
+extern "C" int orc_chem_exact_jac_cell(void *h, long i, const double *y, double *T,
+                                       double *Jout);
+
+// Discrete adjoint of nsteps DIRK steps, per cell: lam0 = (d y_N / d y_0)^T lamT for the
+// stage equations solved exactly (the derivative of the idealised method along the computed
+// trajectory). Every step's starting state is kept (no checkpointing here); backwards, the
+// step is recomputed from its state to get the stage values Y_k, and with the exact scaled
+// stage Jacobians J_k = s_rhs dF/dy(Y_k):
+//   for k = s..1: Ybar_k = h J_k^T (b_k lam + sum_{m>k} a_mk nu_m),
+//                 nu_k = (I - h a_kk J_k)^{-T} Ybar_k;      lam <- lam + sum_k nu_k,
+// with b the last row of A (stiffly accurate tables). y is advanced to y_N, Ts/dTs left at
+// the forward run's end. Returns the number of cells whose forward Newton failed.
+extern "C" int orc_chem_dirk_adjoint(void *h, const orc_dirk *opt, double hstep, int nsteps,
+                                     double *y, const double *lamT, double *lam0) {
+  ChemState *c = (ChemState *)h;
+  const int ns = opt->nstage;
+  int failed = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : failed)
+  for (long i = 0; i < c->n; i++) {
+    Cell cl = cell_of(c, i);
+    std::vector<double> st((size_t)(nsteps + 1) * (NSP + 2));
+    double z[NSP];
+    for (int s = 0; s < NSP; s++) z[s] = y[NSP * i + s];
+    bool ok = true;
+    for (int n = 0; n < nsteps; n++) {
+      double *rec = &st[(size_t)n * (NSP + 2)];
+      for (int s = 0; s < NSP; s++) rec[s] = z[s];
+      rec[NSP] = cl.Ts;
+      rec[NSP + 1] = cl.dTs;
+      int nit = 0;
+      ok = step_cell(cl, opt, hstep, z, nullptr, nit) && ok;
+    }
+    if (!ok) failed++;
+    for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
+    c->Ts[i] = cl.Ts;
+    c->dTs_ge[i] = cl.dTs;
+    double lam[NSP];
+    for (int s = 0; s < NSP; s++) lam[s] = lamT[NSP * i + s];
+    for (int n = nsteps - 1; n >= 0; n--) {
+      const double *rec = &st[(size_t)n * (NSP + 2)];
+      Cell cr = cl;
+      cr.Ts = rec[NSP];
+      cr.dTs = rec[NSP + 1];
+      double yy[NSP], Y[4][NSP];
+      for (int s = 0; s < NSP; s++) yy[s] = rec[s];
+      int nit = 0;
+      step_cell(cr, opt, hstep, yy, Y, nit);
+      double nu[4][NSP] = {};
+      for (int k = ns - 1; k >= 0; k--) {
+        double w[NSP];
+        for (int s = 0; s < NSP; s++) {
+          double acc = opt->A[(ns - 1) * 4 + k] * lam[s];
+          for (int m = k + 1; m < ns; m++) acc += opt->A[m * 4 + k] * nu[m][s];
+          w[s] = acc;
+        }
+        double J[NSP * NSP], T = cr.Ts;
+        orc_chem_exact_jac_cell(c, i, Y[k], &T, J);
+        double yb[NSP], Mt[NSP][NSP];
+        for (int q = 0; q < NSP; q++) {
+          double acc = 0.0;
+          for (int s = 0; s < NSP; s++) acc += (opt->rhs_scale * J[s * NSP + q]) * w[s];
+          yb[q] = hstep * acc;
+        }
+        const double hg = hstep * opt->A[k * 4 + k];
+        for (int r = 0; r < NSP; r++)
+          for (int q = 0; q < NSP; q++)
+            Mt[r][q] = (r == q ? 1.0 : 0.0) - hg * (opt->rhs_scale * J[q * NSP + r]);
+        int piv[NSP];
+        lu_factor(Mt, piv);
+        lu_solve(Mt, piv, yb);
+        for (int s = 0; s < NSP; s++) nu[k][s] = yb[s];
+      }
+      for (int k = 0; k < ns; k++)
+        for (int s = 0; s < NSP; s++) lam[s] += nu[k][s];
+    }
+    for (int s = 0; s < NSP; s++) lam0[NSP * i + s] = lam[s];
   }
   return failed;
 }

@@ -25,6 +25,8 @@ struct mvd_context {
   int *ls_nsing = nullptr;
   bool ls_ready = false;
   mvd::ErkState erk;                  // fixed-step ERK evolution (first use)
+  double *adj_buf = nullptr;          // chemistry adjoint checkpoints and scratch
+  size_t adj_len = 0;
 };
 
 namespace {
@@ -146,6 +148,7 @@ void mvd_destroy(mvd_context *c) {
   if (c->ls_piv) (void)hipFree(c->ls_piv);
   if (c->ls_nsing) (void)hipFree(c->ls_nsing);
   mvd::erk_free(c->erk);
+  if (c->adj_buf) (void)hipFree(c->adj_buf);
   mvd::chem_free(c->chem);
   delete c;
 }
@@ -506,6 +509,33 @@ int mvd_chem_dirk_step(mvd_context *ctx, double, double h, const mvd_dirk_opts *
     stats->ms = ms;
   }
   return hc[mvd::STAT_FAILED];
+}
+
+int mvd_chem_dirk_adjoint(mvd_context *ctx, double, double h, int nsteps, int ckpt_every,
+                          const mvd_dirk_opts *o, double *y, const double *lamT, double *lam0) {
+  if (!ctx || !y || !lamT || !lam0 || !(h > 0) || nsteps < 1 || ckpt_every < 1)
+    return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  mvd::DirkDev d;
+  int rc = dirk_check(o, d);
+  if (rc) return rc;
+  // stiffly accurate tables only (y_{n+1} is the last stage): c_s = sum_j a_sj = 1
+  double cs = 0.0;
+  for (int j = 0; j < d.nstage; j++) cs += d.A[(d.nstage - 1) * 4 + j];
+  if (std::fabs(cs - 1.0) > 1e-12) return MVD_BAD_ARGUMENT;
+  const size_t need = mvd::chem_adjoint_scratch_doubles(ctx->chem.n, nsteps, ckpt_every);
+  if (need > ctx->adj_len) {
+    if (ctx->adj_buf) (void)hipFree(ctx->adj_buf);
+    ctx->adj_buf = nullptr;
+    ctx->adj_len = 0;
+    if (!dev_ok(hipMalloc(&ctx->adj_buf, sizeof(double) * need))) return MVD_DEVICE_ERROR;
+    ctx->adj_len = need;
+  }
+  int nfail = 0;
+  if (mvd::chem_adjoint_run(ctx->chem, d, h, nsteps, ckpt_every, y, lamT, lam0, ctx->adj_buf,
+                            ctx->stream, &nfail))
+    return MVD_DEVICE_ERROR;
+  return nfail;
 }
 
 int mvd_chem_dirk_tune(mvd_context *ctx, double h, const mvd_dirk_opts *o, const double *y,

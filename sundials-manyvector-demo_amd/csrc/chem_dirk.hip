@@ -747,4 +747,244 @@ int chem_ls_solve(long n, const double *LU, const unsigned long long *piv, const
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ---------------------------------------------------------------------------------
+// Checkpointed discrete adjoint of nsteps DIRK steps (no counterpart in the reference; for
+// sensitivity drivers): lam0 = (d y_N / d y_0)^T lamT, per cell, for the stage equations
+// solved exactly along the computed trajectory. One thread per cell:
+//   forward  - advances the cell nsteps steps with the per-cell step below, storing
+//              (y, Ts, dTs) every K steps;
+//   backward - segment by segment, last first: recomputes the segment's steps from its
+//              checkpoint (the per-step states kept in scratch), then walks them backwards;
+//              each step is recomputed from its state to get the stage values Y_k, and the
+//              exact stage Jacobians J_k = s_rhs dF/dy(Y_k) (rows from the exact
+//              vector-Jacobian product of the RHS) enter
+//                for k = s..1: Ybar_k = h J_k^T (b_k lam + sum_{m>k} a_mk nu_m),
+//                              nu_k = (I - h a_kk J_k)^{-T} Ybar_k;   lam += sum_k nu_k,
+//              b = the last row of A (the stiffly accurate tables of mvd_chem_dirk_step).
+// The per-cell step is the plain solver's algorithm (modified Newton, J at y_n, refactor
+// after max_iters), thread-local, so that recomputation repeats the forward run bit for bit.
+// Scratch per cell (element-major, coalesced): (nckpt + K) x 12 + MAXSTAGE x 10 doubles.
+// ---------------------------------------------------------------------------------
+__device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, double &dTs,
+                          const DirkDev &o, double h, double (&y)[NSP], double *Ys, long n,
+                          long i) {
+#pragma clang fp contract(off)
+  double yn[NSP], z[NSP], f[NSP], F[MAXSTAGE][NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) yn[s] = z[s] = y[s];
+  cell_rhs(cc, c, Ts, dTs, yn, o.rhs_scale, f);      // refresh Ts at y_n for J
+  double a[NSP][NSP];
+  unsigned long long pv = 0;
+  bool ok = true;
+#pragma unroll
+  for (int st = 0; st < MAXSTAGE; st++) {
+    if (st < o.nstage && ok) {
+      const double hg = h * o.A[st * 4 + st];
+      if (st == 0) {
+        cell_matrix_regs(cc, c, Ts, dTs, yn, hg, o.rhs_scale, a);
+        pv = lu_factor_regs(a);
+      }
+      double r[NSP];
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < st; j++) acc += o.A[st * 4 + j] * F[j][s];
+        r[s] = yn[s] + h * acc;
+      }
+      int setups = 0;
+      bool conv = false;
+      while (!conv) {
+        for (int it = 0; it < o.max_iters; it++) {
+          cell_rhs(cc, c, Ts, dTs, z, o.rhs_scale, f);
+          double d[NSP];
+#pragma unroll
+          for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
+          lu_solve_regs(a, pv, d);
+#pragma unroll
+          for (int s = 0; s < NSP; s++) z[s] += d[s];
+          if (wrms_norm(d, z, o.rtol, o.atol) <= o.ntol) { conv = true; break; }
+        }
+        if (conv) break;
+        if (++setups > o.max_setups) { ok = false; break; }
+        cell_rhs(cc, c, Ts, dTs, z, o.rhs_scale, f);   // temperature at the iterate
+        cell_matrix_regs(cc, c, Ts, dTs, z, hg, o.rhs_scale, a);
+        pv = lu_factor_regs(a);
+      }
+#pragma unroll
+      for (int s = 0; s < NSP; s++) F[st][s] = (z[s] - r[s]) / hg;
+      if (Ys)
+#pragma unroll
+        for (int s = 0; s < NSP; s++) Ys[(long)(st * NSP + s) * n + i] = z[s];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NSP; s++) y[s] = z[s];
+  return ok;
+}
+
+constexpr int ADJ_REC = NSP + 2;     // y, Ts, dTs
+
+__global__ void __launch_bounds__(256)
+chem_adjoint_forward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, int nsteps, int K,
+                     double *y, double *ck, int *nfail) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CellIn c = load_cell(p, i);
+  double Ts = p.Ts[i], dTs = p.dTs[i];
+  double z[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) z[s] = y[NSP * i + s];
+  bool ok = true;
+  for (int st = 0; st < nsteps; st++) {
+    if (st % K == 0) {
+      double *rec = ck + (long)(st / K) * ADJ_REC * n;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) rec[s * n + i] = z[s];
+      rec[NSP * n + i] = Ts;
+      rec[(NSP + 1) * n + i] = dTs;
+    }
+    ok = cell_step(cc, c, Ts, dTs, o, h, z, nullptr, n, i) && ok;
+  }
+#pragma unroll
+  for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
+  p.Ts[i] = Ts;
+  p.dTs[i] = dTs;
+  if (!ok) atomicAdd(nfail, 1);
+}
+
+// M^T = (I - hg J)^T of one stage from the exact Jacobian rows at the scaled state Y
+// (temperature solved from the warm start T); also yb = h J^T w.
+__device__ inline void stage_adjoint_matrix(const ChemConst &cc, const CellIn &c, double T,
+                                            const double (&Y)[NSP], double srhs, double h,
+                                            double hg, const double (&w)[NSP],
+                                            double (&Mt)[NSP][NSP], double (&yb)[NSP]) {
+#pragma clang fp contract(off)
+  double ya[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) ya[s] = Y[s] * c.s(s);
+  double dT;
+  chem_temperature(cc, ya, T, dT);
+  const bool clamped = !(T > cc.tmin && T < cc.tmax);
+  double acc[NSP];
+#pragma unroll
+  for (int q = 0; q < NSP; q++) acc[q] = 0.0;
+#pragma unroll
+  for (int r = 0; r < NSP; r++) {
+    double vp[NSP], row[NSP];
+#pragma unroll
+    for (int s = 0; s < NSP; s++) vp[s] = 0.0;
+    vp[r] = c.is(r);
+    if (r == NSP - 1) vp[r] *= c.ax.inv_mdensity;
+    chem_vjp_physical(cc, ya, T, clamped, c.ax, vp, row);
+#pragma unroll
+    for (int q = 0; q < NSP; q++) {
+      const double J = row[q] * c.s(q);                 // scaled dF_r/dy_q
+      acc[q] += (srhs * J) * w[r];
+      Mt[q][r] = (q == r ? 1.0 : 0.0) - hg * (srhs * J);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NSP; q++) yb[q] = h * acc[q];
+}
+
+__global__ void __launch_bounds__(256)
+chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, int nsteps, int K,
+                      const double *ck, double *seg, double *Ys, const double *lamT,
+                      double *lam0) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CellIn c = load_cell(p, i);
+  double lam[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) lam[s] = lamT[NSP * i + s];
+  const int ns = o.nstage;
+  const int nseg = (nsteps + K - 1) / K;
+  for (int sg = nseg - 1; sg >= 0; sg--) {
+    const int n0 = sg * K, n1 = n0 + K < nsteps ? n0 + K : nsteps;
+    const double *rec = ck + (long)sg * ADJ_REC * n;
+    double z[NSP];
+#pragma unroll
+    for (int s = 0; s < NSP; s++) z[s] = rec[s * n + i];
+    double Ts = rec[NSP * n + i], dTs = rec[(NSP + 1) * n + i];
+    for (int st = n0; st < n1; st++) {
+      double *sr = seg + (long)(st - n0) * ADJ_REC * n;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) sr[s * n + i] = z[s];
+      sr[NSP * n + i] = Ts;
+      sr[(NSP + 1) * n + i] = dTs;
+      if (st + 1 < n1) (void)cell_step(cc, c, Ts, dTs, o, h, z, nullptr, n, i);
+    }
+    for (int st = n1 - 1; st >= n0; st--) {
+      const double *sr = seg + (long)(st - n0) * ADJ_REC * n;
+#pragma unroll
+      for (int s = 0; s < NSP; s++) z[s] = sr[s * n + i];
+      Ts = sr[NSP * n + i];
+      dTs = sr[(NSP + 1) * n + i];
+      (void)cell_step(cc, c, Ts, dTs, o, h, z, Ys, n, i);
+      double nu[MAXSTAGE][NSP];
+#pragma unroll
+      for (int k = 0; k < MAXSTAGE; k++)
+#pragma unroll
+        for (int s = 0; s < NSP; s++) nu[k][s] = 0.0;
+#pragma unroll
+      for (int k = MAXSTAGE - 1; k >= 0; k--) {
+        if (k < ns) {
+          double w[NSP], Y[NSP];
+#pragma unroll
+          for (int s = 0; s < NSP; s++) {
+            double acc = dirk_coef(o, ns - 1, k) * lam[s];
+            if (ns - 1 == 0) acc = o.A[k] * lam[s];
+#pragma unroll
+            for (int m = k + 1; m < MAXSTAGE; m++)
+              if (m < ns) acc += dirk_coef(o, m, k) * nu[m][s];
+            w[s] = acc;
+            Y[s] = Ys[(long)(k * NSP + s) * n + i];
+          }
+          double Mt[NSP][NSP], yb[NSP];
+          stage_adjoint_matrix(cc, c, Ts, Y, o.rhs_scale, h, h * o.A[k * 4 + k], w, Mt, yb);
+          const unsigned long long pv = lu_factor_regs(Mt);
+          lu_solve_regs(Mt, pv, yb);
+#pragma unroll
+          for (int s = 0; s < NSP; s++) nu[k][s] = yb[s];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < MAXSTAGE; k++)
+        if (k < ns)
+#pragma unroll
+          for (int s = 0; s < NSP; s++) lam[s] += nu[k][s];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NSP; s++) lam0[NSP * i + s] = lam[s];
+}
+
+size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K) {
+  const long nckpt = (nsteps + K - 1) / K;
+  return (size_t)n * ((nckpt + K) * ADJ_REC + MAXSTAGE * NSP);
+}
+
+int chem_adjoint_run(ChemState &s, const DirkDev &o, double h, int nsteps, int K, double *y,
+                     const double *lamT, double *lam0, double *scratch, hipStream_t st,
+                     int *host_fail) {
+  const long n = s.n;
+  const unsigned nbk = (unsigned)((n + 255) / 256);
+  const long nckpt = (nsteps + K - 1) / K;
+  double *ck = scratch, *seg = ck + nckpt * ADJ_REC * n, *Ys = seg + (long)K * ADJ_REC * n;
+  const ChemPtrsD p = ptrs_d(s);
+  int *nfail = s.counters;
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, 1L, nfail, 0);
+  hipLaunchKernelGGL(chem_adjoint_forward, dim3(nbk), dim3(256), 0, st, n, s.cc, p, o, h, nsteps,
+                     K, y, ck, nfail);
+  hipLaunchKernelGGL(chem_adjoint_backward, dim3(nbk), dim3(256), 0, st, n, s.cc, p, o, h,
+                     nsteps, K, (const double *)ck, seg, Ys, lamT, lam0);
+  if (hipGetLastError() != hipSuccess) return -1;
+  if (hipMemcpyAsync(host_fail, nfail, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return -1;
+  return 0;
+}
+
 }  // namespace mvd

@@ -28,6 +28,11 @@ struct DirkDev {
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
                         int grid_waves, int block);
 int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st);
+// checkpointed discrete adjoint of nsteps DIRK steps (checkpoint every K steps)
+size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K);
+int chem_adjoint_run(ChemState &s, const DirkDev &o, double h, int nsteps, int K, double *y,
+                     const double *lamT, double *lam0, double *scratch, hipStream_t st,
+                     int *host_fail);
 // block linear solver of the chemistry subvector (A = I - gamma J, batched LU)
 int chem_ls_setup(long n, double gamma, const double *J, double *LU, unsigned long long *piv,
                   int *nsing, hipStream_t st);

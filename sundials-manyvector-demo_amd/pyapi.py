@@ -102,6 +102,8 @@ def lib():
         "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
                                 C.POINTER(DirkStats)], C.c_int),
         "mvd_chem_dirk_tune": ([vp, C.c_double, C.POINTER(DirkOpts), _P, _P], C.c_int),
+        "mvd_chem_dirk_adjoint": ([vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                   C.POINTER(DirkOpts), _P, _P, _P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -268,6 +270,16 @@ class Context:
 
     def chem_jac(self, y, J, t=0.0):
         _check("mvd_chem_jac", lib().mvd_chem_jac(self.h, t, dptr(y), dptr(J)))
+
+    def dirk_adjoint(self, y, h, nsteps, ckpt_every, lamT, lam0, opts=None, t=0.0):
+        """Checkpointed adjoint of nsteps DIRK steps: y advanced in place, lam0 filled with
+        (dy_N/dy_0)^T lamT. Returns the number of cells whose forward Newton failed."""
+        o = opts if opts is not None else sdirk3()
+        rc = lib().mvd_chem_dirk_adjoint(self.h, t, h, int(nsteps), int(ckpt_every), C.byref(o),
+                                         dptr(y), dptr(lamT), dptr(lam0))
+        if rc < 0:
+            _check("mvd_chem_dirk_adjoint", rc)
+        return rc
 
     def chem_lsetup(self, gamma, J):
         """Factor A = I - gamma*J per cell; returns the number of singular blocks (0: ok)."""

@@ -84,7 +84,7 @@ def erk(A, b, c):
 
 def erk_evolve(g, tab, h, nsteps, w):
     """Fixed-step explicit RK of fe() on the oracle, in place on w; returns the flag."""
-    return lib().orc_erk_evolve(C.byref(g), C.byref(tab), C.c_double(h), C.c_int(nsteps), _pp(w))
+    return lib().orc_erk_evolve(C.byref(g), C.byref(tab), h, nsteps, _pp(w))
 
 
 def dirk_opts(A=None, c=None, nstage=3, rtol=1e-5, atol=1e-9, newton_tol=0.1, max_iters=5,
@@ -138,6 +138,12 @@ def lib():
         L.orc_stability_dt.argtypes = [C.POINTER(Grid), PP, C.c_double]
         L.orc_stability_dt.restype = C.c_double
         L.orc_face_flux_opcount.argtypes = [C.c_int, C.POINTER(C.c_ulonglong)]
+        L.orc_chem_dirk_adjoint.argtypes = [C.c_void_p, C.POINTER(Dirk), C.c_double, C.c_int,
+                                            P, P, P]
+        L.orc_chem_dirk_adjoint.restype = C.c_int
+        L.orc_chem_exact_jac_cell.argtypes = [C.c_void_p, C.c_long, P, P, P]
+        L.orc_erk_evolve.argtypes = [C.POINTER(Grid), C.POINTER(Erk), C.c_double, C.c_int, PP]
+        L.orc_erk_evolve.restype = C.c_int
         L.orc_chem_create.argtypes = [C.c_long, P, C.c_double]
         L.orc_chem_create.restype = C.c_void_p
         L.orc_chem_destroy.argtypes = [C.c_void_p]
@@ -326,6 +332,25 @@ def dirk_step(chem, opts, h, y):
     failed = lib().orc_chem_dirk_step(chem.h, C.byref(opts), h, _p(y),
                                       iters.ctypes.data_as(C.POINTER(C.c_int)))
     return failed, iters
+
+
+def dirk_adjoint(chem, opts, h, nsteps, y, lamT):
+    """Oracle discrete adjoint of nsteps DIRK steps: y (n*10, scaled) is advanced in place;
+    returns (failed cells, lam0 = (dy_N/dy_0)^T lamT)."""
+    lam0 = np.zeros_like(lamT)
+    lamT = np.ascontiguousarray(lamT, dtype=np.float64)
+    failed = lib().orc_chem_dirk_adjoint(chem.h, C.byref(opts), h, nsteps, _p(y), _p(lamT),
+                                         _p(lam0))
+    return failed, lam0
+
+
+def exact_jac_cell(chem, i, y, T):
+    """Exact scaled Jacobian (10x10, row-major) of cell i at the scaled state y."""
+    J = np.zeros(100)
+    Tv = np.array([T], dtype=np.float64)
+    lib().orc_chem_exact_jac_cell(chem.h, i, _p(np.ascontiguousarray(y, dtype=np.float64)),
+                                  _p(Tv), _p(J))
+    return J.reshape(10, 10), float(Tv[0])
 
 
 def lu(M, b):
