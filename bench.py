@@ -257,6 +257,48 @@ def bench_vjp(api, problems, torch, n, seed, reps):
     }
 
 
+def bench_sod(api, problems, torch, cpu=True):
+    """Config-1 Sod shock tube: 128x3x3 cells, Neumann, 2000 fixed steps of 1e-4 to t=0.2 with
+    Bogacki-Shampine 3 (mvd_erk_evolve, one captured step replayed); the oracle's ERK on one
+    host core for the same run, and the largest field-relative difference between the two."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    nx, nsteps, h = 128, 2000, 1e-4
+    w0 = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in problems.sod(nx)[:5]]
+    ctx = api.Context(api.make_params(nx, 3, 3, 0, (1,) * 6))
+    wd = [torch.from_numpy(a.copy()).to("cuda") for a in w0] + [None]
+    ctx.erk_evolve(wd, h, 10)                     # capture the step graph
+    for q in range(5):
+        wd[q].copy_(torch.from_numpy(w0[q]))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    rc = ctx.erk_evolve(wd, h, nsteps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    out = {"workload": f"Sod {nx}x3x3, Neumann, ERK3 (Bogacki-Shampine) h={h:g}, {nsteps} steps "
+                       f"to t=0.2",
+           "rc": rc, "gpu_ms": ms, "steps_per_s": nsteps / (ms * 1e-3),
+           "cell_steps_per_s": nx * 9 * nsteps / (ms * 1e-3)}
+    if cpu:
+        orc.set_threads(1)
+        wc = w0 + [np.zeros(1)]
+        wc = [a.copy() for a in wc]
+        t0 = time.perf_counter()
+        assert orc.erk_evolve(orc.make_grid(nx, 3, 3, 0, (1,) * 6),
+                              orc.erk(*orc.bs3_table()), h, nsteps, wc) == 0
+        out["cpu_1core_s"] = time.perf_counter() - t0
+        diffs = []
+        for q in range(5):
+            den = float(np.abs(wc[q]).max())
+            num = float(np.abs(wd[q].cpu().numpy() - wc[q]).max())
+            diffs.append(num / den if den > 0 else num)      # my = mz = 0 stay zero
+        out["max_rel_diff_vs_oracle"] = max(diffs)
+    ctx.close()
+    return out
+
+
 def cpu_chem_baseline(n_sample, seed, threads, reps, h=1e3):
     """The CPU oracle's per-cell SDIRK3 step on config-3 cells, `reps` times from the same
     initial state; returns cells/s and the total seconds."""
@@ -295,6 +337,9 @@ def main():
     ap.add_argument("--chem-cpu-reps", type=int, default=8)
     ap.add_argument("--no-chem", action="store_true")
     ap.add_argument("--no-vjp", action="store_true")
+    ap.add_argument("--no-sod", action="store_true")
+    ap.add_argument("--only", choices=["vjp", "sod"], default=None,
+                    help="run one extra workload alone and print its JSON (profiling runs)")
     ap.add_argument("--vjp-n", type=int, default=128)
     ap.add_argument("--vjp-reps", type=int, default=10)
     ap.add_argument("--profile-fe", action="store_true",
@@ -310,6 +355,13 @@ def main():
 
     import torch
     api = _load("mvd_pyapi", os.path.join(PKG, "pyapi.py"))
+    if args.only:
+        problems = _load("mvd_problems", os.path.join(PKG, "problems.py"))
+        if args.only == "vjp":
+            print(json.dumps(bench_vjp(api, problems, torch, args.vjp_n, 777, args.vjp_reps)))
+        else:
+            print(json.dumps(bench_sod(api, problems, torch, cpu=not args.no_cpu)))
+        return
     problems = _load("mvd_problems", os.path.join(PKG, "problems.py"))
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     n = args.n
@@ -394,6 +446,8 @@ def main():
         extra["chem"] = bench_chem(api, problems, torch, args.chem_n, 4321, args.chem_steps)
     if not args.no_vjp:
         extra["vjp"] = bench_vjp(api, problems, torch, args.vjp_n, 777, args.vjp_reps)
+    if not args.no_sod:
+        extra["sod"] = bench_sod(api, problems, torch, cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
