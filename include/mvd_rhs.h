@@ -194,6 +194,24 @@ int mvd_dirk_sdirk3(mvd_dirk_opts *o);
  * multi-kernel path. Returns 0, or the number of failed cells (> 0, recoverable). */
 int mvd_chem_dirk_step(mvd_context *ctx, double t, double h, const mvd_dirk_opts *o, double *y,
                        int plain, mvd_dirk_stats *stats);
+/* The same step of y' = rhs_scale*f(y) + force with a constant forcing force ([cell][10],
+ * scaled; the forcing MRIStep hands its fast stepper, applied by ffast at
+ * multirate_chem_hydro_main.cpp:922-946), batched kernel only. */
+int mvd_chem_dirk_step_forced(mvd_context *ctx, double t, double h, const mvd_dirk_opts *o,
+                              double *y, const double *force, mvd_dirk_stats *stats);
+/* Multirate evolution of the reacting flow, device-resident (configuration 5; the driver
+ * multirate_chem_hydro_main.cpp:529-555 with MRIStep's default third-order coupling, MIS
+ * with Knoth-Wolke 3, and a fixed slow step): nslow slow steps of size h on w (the six
+ * ManyVector subvectors, chemistry scaled by mvd_chem_prepare, nchem = 10). Each slow stage
+ * evaluates fslow at the running state (its total-energy overwrite included), forms the
+ * constant MIS forcing and takes ceil((c_i - c_{i-1}) h / hmax_fast) equal fixed fast steps
+ * (the inner stepper's maximum step, input hmax): fluid += dt*forcing, chemistry by a
+ * forced batched DIRK step with the options fast (rhs_scale = TimeUnits gives ffast). One
+ * slow step is captured as a device graph and replayed. Returns 0, the number (> 0) of
+ * failed fast cell steps, MVD_ILLEGAL_STATE if any slow RHS met an illegal state, or an
+ * error code. */
+int mvd_mri_evolve(mvd_context *ctx, double t0, double h, int nslow, double hmax_fast,
+                   const mvd_dirk_opts *fast, double *const *w);
 /* Checkpointed discrete adjoint of nsteps chemistry DIRK steps (no counterpart in the
  * reference, which has no adjoint; for sensitivity drivers, e.g. of the domain-integrated
  * energy to the initial species). y (scaled, [cell][10]) is advanced nsteps steps of size h

@@ -28,6 +28,13 @@ int orc_chem_dirk_step(void *chem, const orc_dirk *opt, double h, double *y, int
 int orc_chem_dirk_adjoint(void *chem, const orc_dirk *opt, double h, int nsteps, double *y,
                           const double *lamT, double *lam0);
 int orc_chem_exact_jac_cell(void *chem, long i, const double *y, double *T, double *J);
+int orc_chem_dirk_step_forced(void *chem, const orc_dirk *opt, double h, double *y,
+                              const double *force, int *iters);
+/* multirate evolution, MIS form of Knoth-Wolke 3 slow / forced DIRK fast (orc_mri.cpp) */
+void orc_mri_kw3_table(double *c, double *A);
+int orc_mri_evolve(const orc_grid *g, void *chem, double energy_units, const orc_dirk *fast,
+                   double h, int nslow, double hmax, double *const *w);
+int orc_mri_fast_steps(int i, double h, double hmax);
 /* fixed-step explicit Runge-Kutta evolution of fe() (orc_erk.cpp) */
 typedef struct {
   int stages;

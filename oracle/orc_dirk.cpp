@@ -130,7 +130,7 @@ extern "C" int orc_lu(const double *Min, double *LUout, int *piv, const double *
 // follow its RHS evaluations. Y (may be null) receives the stage values. Returns false when
 // the Newton budget ran out; nit counts the Newton iterations.
 static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[NSP],
-                      double (*Y)[NSP], int &nit) {
+                      double (*Y)[NSP], int &nit, const double *force = nullptr) {
   const int ns = opt->nstage;
   double yn[NSP], z[NSP], F[4][NSP], f[NSP];
   for (int s = 0; s < NSP; s++) yn[s] = z[s] = y[s];
@@ -157,7 +157,10 @@ static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[N
       for (int it = 0; it < opt->max_iters; it++) {
         cell_rhs(cl, z, opt->rhs_scale, f);
         double d[NSP];
-        for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
+        if (force)
+          for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * (f[s] + force[s]) - r[s]);
+        else
+          for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
         lu_solve(M, piv, d);
         for (int s = 0; s < NSP; s++) z[s] += d[s];
         nit++;
@@ -187,11 +190,13 @@ static Cell cell_of(ChemState *c, long i) {
   return cl;
 }
 
-// One DIRK step of every cell; y (n x 10, scaled) is overwritten with y_{n+1}; Ts is left
-// at the temperature of the last RHS evaluation of each cell. iters[i] receives the total
-// Newton iterations of cell i (may be null). Returns the number of failed cells.
-extern "C" int orc_chem_dirk_step(void *h, const orc_dirk *opt, double hstep, double *y,
-                                  int *iters) {
+// One DIRK step of every cell with a constant forcing added to the scaled RHS (force
+// [cell][10] or null; the forcing an MRI method hands its fast stepper); y (n x 10, scaled)
+// is overwritten with y_{n+1}; Ts is left at the temperature of the last RHS evaluation of
+// each cell. iters[i] receives the total Newton iterations of cell i (may be null). Returns
+// the number of failed cells.
+extern "C" int orc_chem_dirk_step_forced(void *h, const orc_dirk *opt, double hstep, double *y,
+                                         const double *force, int *iters) {
   ChemState *c = (ChemState *)h;
   int failed = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : failed)
@@ -200,7 +205,8 @@ extern "C" int orc_chem_dirk_step(void *h, const orc_dirk *opt, double hstep, do
     double z[NSP];
     for (int s = 0; s < NSP; s++) z[s] = y[NSP * i + s];
     int nit = 0;
-    const bool ok_cell = step_cell(cl, opt, hstep, z, nullptr, nit);
+    const bool ok_cell = step_cell(cl, opt, hstep, z, nullptr, nit,
+                                   force ? force + NSP * i : nullptr);
     if (!ok_cell) failed++;
     for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
     c->Ts[i] = cl.Ts;
@@ -208,6 +214,12 @@ extern "C" int orc_chem_dirk_step(void *h, const orc_dirk *opt, double hstep, do
     if (iters) iters[i] = ok_cell ? nit : -1;
   }
   return failed;
+}
+
+// The unforced step.
+extern "C" int orc_chem_dirk_step(void *h, const orc_dirk *opt, double hstep, double *y,
+                                  int *iters) {
+  return orc_chem_dirk_step_forced(h, opt, hstep, y, nullptr, iters);
 }
 
 extern "C" int orc_chem_exact_jac_cell(void *h, long i, const double *y, double *T,

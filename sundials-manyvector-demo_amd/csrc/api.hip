@@ -6,6 +6,7 @@
 #include "mvd_internal.hpp"
 #include "mvd_chem.hpp"
 #include "mvd_dirk.hpp"
+#include "mvd_mri.hpp"
 #include "../../include/mvd_rhs.h"
 
 struct mvd_context {
@@ -27,6 +28,7 @@ struct mvd_context {
   mvd::ErkState erk;                  // fixed-step ERK evolution (first use)
   double *adj_buf = nullptr;          // chemistry adjoint checkpoints and scratch
   size_t adj_len = 0;
+  mvd::MriState mri;                  // multirate evolution (first use)
 };
 
 namespace {
@@ -149,6 +151,7 @@ void mvd_destroy(mvd_context *c) {
   if (c->ls_nsing) (void)hipFree(c->ls_nsing);
   mvd::erk_free(c->erk);
   if (c->adj_buf) (void)hipFree(c->adj_buf);
+  mvd::mri_free(c->mri);
   mvd::chem_free(c->chem);
   delete c;
 }
@@ -476,10 +479,11 @@ int default_waves(mvd_context *ctx) {
 
 }  // namespace
 
-int mvd_chem_dirk_step(mvd_context *ctx, double, double h, const mvd_dirk_opts *o, double *y,
-                       int plain, mvd_dirk_stats *stats) {
+static int dirk_step_common(mvd_context *ctx, double h, const mvd_dirk_opts *o, double *y,
+                            int plain, const double *force, mvd_dirk_stats *stats) {
   if (!ctx || !y || !(h > 0)) return MVD_BAD_ARGUMENT;
   if (!ctx->chem.ready) return MVD_NOT_READY;
+  if (plain && force) return MVD_UNSUPPORTED;
   mvd::DirkDev d;
   int rc = dirk_check(o, d);
   if (rc) return rc;
@@ -488,7 +492,7 @@ int mvd_chem_dirk_step(mvd_context *ctx, double, double h, const mvd_dirk_opts *
   (void)hipEventRecord(e0, ctx->stream);
   rc = plain ? mvd::dirk_plain_run(ctx->chem, d, h, y, ctx->stream)
              : mvd::dirk_batched_launch(ctx->chem, d, h, y, ctx->stream, default_waves(ctx),
-                                        ctx->dirk_block);
+                                        ctx->dirk_block, force);
   (void)hipEventRecord(e1, ctx->stream);
   if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return MVD_DEVICE_ERROR; }
   int *hc = ctx->chem.host_counters;
@@ -509,6 +513,40 @@ int mvd_chem_dirk_step(mvd_context *ctx, double, double h, const mvd_dirk_opts *
     stats->ms = ms;
   }
   return hc[mvd::STAT_FAILED];
+}
+
+int mvd_chem_dirk_step(mvd_context *ctx, double, double h, const mvd_dirk_opts *o, double *y,
+                       int plain, mvd_dirk_stats *stats) {
+  return dirk_step_common(ctx, h, o, y, plain, nullptr, stats);
+}
+
+int mvd_chem_dirk_step_forced(mvd_context *ctx, double, double h, const mvd_dirk_opts *o,
+                              double *y, const double *force, mvd_dirk_stats *stats) {
+  if (!force) return MVD_BAD_ARGUMENT;
+  return dirk_step_common(ctx, h, o, y, 0, force, stats);
+}
+
+int mvd_mri_evolve(mvd_context *ctx, double, double h, int nslow, double hmax_fast,
+                   const mvd_dirk_opts *fast, double *const *w) {
+  if (!ctx || !(h > 0) || nslow < 0 || !(hmax_fast > 0)) return MVD_BAD_ARGUMENT;
+  if (h / hmax_fast > 1e6) return MVD_BAD_ARGUMENT;
+  if (ctx->g.nchem != mvd::NSP) return MVD_UNSUPPORTED;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  if (ctx->chem.n != (long)ctx->g.nx * ctx->g.ny * ctx->g.nz) return MVD_BAD_ARGUMENT;
+  mvd::DirkDev d;
+  int rc = dirk_check(fast, d);
+  if (rc) return rc;
+  rc = check_ptrs(ctx, (const double *const *)w);
+  if (rc) return rc;
+  if (nslow == 0) return MVD_SUCCESS;
+  int nfail = 0;
+  rc = mvd::mri_evolve(ctx->g, ctx->fe, ctx->chem, 1.0 / ctx->energy_units, ctx->mri, d, h,
+                       nslow, hmax_fast, default_waves(ctx), ctx->dirk_block, w, ctx->stream,
+                       ctx->host_flag, &nfail);
+  if (rc == -2) return MVD_UNSUPPORTED;
+  if (rc) return MVD_DEVICE_ERROR;
+  if (*ctx->host_flag) return MVD_ILLEGAL_STATE;
+  return nfail;
 }
 
 int mvd_chem_dirk_adjoint(mvd_context *ctx, double, double h, int nsteps, int ckpt_every,

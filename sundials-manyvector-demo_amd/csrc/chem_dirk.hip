@@ -322,9 +322,12 @@ __device__ inline void lu_solve_regs(const double (&a)[NSP][NSP], unsigned long 
 // Stage derivatives of a 64-cell chunk, element-major (F_j[s] of lane l at (j*10+s)*64 + l).
 constexpr int STAGE_WORDS = (MAXSTAGE - 1) * NSP;
 
+// FORCED: y' = s_rhs f(y) + force, force constant over the step ([cell][10], the forcing an
+// MRI method hands its fast stepper, multirate_chem_hydro_main.cpp:922-946)
+template <bool FORCED>
 __global__ void __launch_bounds__(256)
 chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double *y,
-                  double *work, int *queue, int *rows) {
+                  const double *force, double *work, int *queue, int *rows) {
 #pragma clang fp contract(off)
   // block statistics in LDS (the per-cell updates are LDS atomics; one row per block)
   __shared__ int bs[NSTAT];
@@ -373,8 +376,13 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
         continue;
       }
       double d[NSP];
+      if (FORCED) {
 #pragma unroll
-      for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
+        for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * (f[s] + force[NSP * i + s]) - r[s]);
+      } else {
+#pragma unroll
+        for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
+      }
       lu_solve_regs(lu, pv, d);
 #pragma unroll
       for (int s = 0; s < NSP; s++) z[s] += d[s];
@@ -585,7 +593,7 @@ static int ensure_work(ChemState &s, size_t bytes) {
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
-                        int grid_waves, int block) {
+                        int grid_waves, int block, const double *force) {
   const long n = s.n;
   if (n <= 0 || n > (1L << 30)) return -1;
   if (block != 64 && block != 128 && block != 256) return -1;
@@ -602,8 +610,12 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   char *w = (char *)s.work;
   int *rows = (int *)(w + b_blk);
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
-  hipLaunchKernelGGL(chem_dirk_batched, dim3((unsigned)grid), dim3(block), 0, st, n, s.cc,
-                     ptrs_d(s), o, h, y, (double *)w, s.counters, rows);
+  if (force)
+    hipLaunchKernelGGL(chem_dirk_batched<true>, dim3((unsigned)grid), dim3(block), 0, st, n, s.cc,
+                       ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
+  else
+    hipLaunchKernelGGL(chem_dirk_batched<false>, dim3((unsigned)grid), dim3(block), 0, st, n,
+                       s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
   hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(nrow)), dim3(256), 0, st,
                      (const int *)rows, nrow, s.counters + 1);
   return hipGetLastError() == hipSuccess ? 0 : -1;

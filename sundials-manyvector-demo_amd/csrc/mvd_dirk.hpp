@@ -25,8 +25,9 @@ struct DirkDev {
   double rhs_scale;
 };
 
+// force (nullable): constant forcing added to the scaled RHS, [cell][10]
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
-                        int grid_waves, int block);
+                        int grid_waves, int block, const double *force = nullptr);
 int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st);
 // checkpointed discrete adjoint of nsteps DIRK steps (checkpoint every K steps)
 size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K);

@@ -107,3 +107,52 @@ def chem_cells(n, seed=4321, alpha=(1.0,) * 8, ntot=(1e-2, 1e2), temp=(10.0, 1e4
     y[:, 8] = de / M_AMU
     y[:, 9] = ge
     return np.ascontiguousarray(y.reshape(-1)), T
+
+
+def primordial_blast_mri(nx, ny=None, nz=None, bump=100.0, width=0.05, t_in=1e4, t_amb=10.0,
+                         gamma=5.0 / 3.0, units=(3.0e70, 3.0857e30, 1.0e11)):
+    """Multirate primordial blast (configuration 5): gas at rest on [0,1]^3 with a Gaussian
+    overdensity of `bump` times the background density0 = 1e2*mH and width `width` centred
+    in the box, temperature t_amb outside rising to t_in at the centre, both with the
+    reference's profile shape exp(-2 r^2 / width^2) (primordial_blast.cpp:218-231); species
+    composition as primordial_blast.cpp:238-271 (trace 1e-3 fractions outside r^2/w^2 < 2,
+    tiny = 1e-40 / small = 1e-12 inside, Hfrac = 0.76), ge = kB T n / (rho (gamma - 1)).
+    Units (mass, length, time) of tests/primordial_blast/input_primordial_blast_mr.txt:34-37.
+    Returns (fluid: 5 flat code-unit arrays, chem: AoS physical, 10 per cell, as
+    initial_conditions stores them before prepare_Dengo_structures)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    mu, lu, tu = units
+    density_units = mu / lu / lu / lu
+    energy_units = mu / lu / tu / tu
+    tiny, small, hfrac = 1e-40, 1e-12, 0.76
+    hi_w, he_w = 1.00794 * MH, 4.002602 * MH
+    density0 = 1e2 * MH
+    zz, yy, xx = np.meshgrid(cell_centres(nz), cell_centres(ny), cell_centres(nx), indexing="ij")
+    rsq = (xx - 0.5) ** 2 + (yy - 0.5) ** 2 + (zz - 0.5) ** 2
+    prof = np.exp(-2.0 * rsq / width / width)
+    density = density0 * (1.0 + (bump - 1.0) * prof)
+    T = t_amb + (t_in - t_amb) * prof
+    inside = rsq / width / width < 2.0
+    H2I = np.where(inside, tiny, 1e-3) * density
+    H2II = np.where(inside, tiny, 1e-3) * density
+    HII = np.where(inside, small, 1e-3) * density
+    HM = np.where(inside, tiny, 1e-3) * density
+    HeII = np.where(inside, small, 1e-3) * density
+    HeIII = np.where(inside, small, 1e-3) * density
+    HeI = (1.0 - hfrac) * density - HeII - HeIII
+    HI = density - (H2I + H2II + HII + HM + HeI + HeII + HeIII)
+    n = np.stack([H2I / (2 * hi_w), H2II / (2 * hi_w), HI / hi_w, HII / hi_w, HM / hi_w,
+                  HeI / he_w, HeII / he_w, HeIII / he_w], axis=-1)
+    ndens = n.sum(axis=-1)
+    de = (n[..., 3] + n[..., 6] + 2.0 * n[..., 7] - n[..., 4] + n[..., 1]) * MH
+    ge = KBOLTZ * T * ndens / (density * (gamma - 1.0))
+    chem = np.empty(density.shape + (10,))
+    chem[..., :8] = n
+    chem[..., 8] = de / M_AMU
+    chem[..., 9] = ge
+    rho = density / density_units
+    zero = np.zeros_like(rho)
+    et = ge / energy_units                   # initial_conditions: (ge + 0.5 rho v^2)/EnergyUnits, v = 0
+    fluid = [np.ascontiguousarray(a.reshape(-1)) for a in (rho, zero, zero, zero, et)]
+    return fluid, np.ascontiguousarray(chem.reshape(-1))

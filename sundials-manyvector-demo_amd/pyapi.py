@@ -102,6 +102,10 @@ def lib():
         "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
                                 C.POINTER(DirkStats)], C.c_int),
         "mvd_chem_dirk_tune": ([vp, C.c_double, C.POINTER(DirkOpts), _P, _P], C.c_int),
+        "mvd_chem_dirk_step_forced": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, _P,
+                                       C.POINTER(DirkStats)], C.c_int),
+        "mvd_mri_evolve": ([vp, C.c_double, C.c_double, C.c_int, C.c_double,
+                            C.POINTER(DirkOpts), C.POINTER(_P)], C.c_int),
         "mvd_chem_dirk_adjoint": ([vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.POINTER(DirkOpts), _P, _P, _P], C.c_int),
     }
@@ -270,6 +274,25 @@ class Context:
 
     def chem_jac(self, y, J, t=0.0):
         _check("mvd_chem_jac", lib().mvd_chem_jac(self.h, t, dptr(y), dptr(J)))
+
+    def mri_evolve(self, w, h, nslow, hmax_fast, fast_opts, t0=0.0):
+        """Multirate evolution (MIS Knoth-Wolke 3 slow, forced batched DIRK fast with steps of
+        at most hmax_fast) in place on w; returns the failed fast cell steps (0: all
+        converged)."""
+        rc = lib().mvd_mri_evolve(self.h, t0, h, int(nslow), hmax_fast, C.byref(fast_opts),
+                                  dptrs(w))
+        if rc < 0:
+            _check("mvd_mri_evolve", rc)
+        return rc
+
+    def dirk_step_forced(self, y, h, force, opts=None, t=0.0):
+        o = opts if opts is not None else sdirk3()
+        st = DirkStats()
+        rc = lib().mvd_chem_dirk_step_forced(self.h, t, h, C.byref(o), dptr(y), dptr(force),
+                                             C.byref(st))
+        if rc < 0:
+            _check("mvd_chem_dirk_step_forced", rc)
+        return rc
 
     def dirk_adjoint(self, y, h, nsteps, ckpt_every, lamT, lam0, opts=None, t=0.0):
         """Checkpointed adjoint of nsteps DIRK steps: y advanced in place, lam0 filled with

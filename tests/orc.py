@@ -142,6 +142,15 @@ def lib():
                                             P, P, P]
         L.orc_chem_dirk_adjoint.restype = C.c_int
         L.orc_chem_exact_jac_cell.argtypes = [C.c_void_p, C.c_long, P, P, P]
+        L.orc_chem_dirk_step_forced.argtypes = [C.c_void_p, C.POINTER(Dirk), C.c_double, P, P,
+                                                C.POINTER(C.c_int)]
+        L.orc_chem_dirk_step_forced.restype = C.c_int
+        L.orc_mri_evolve.argtypes = [C.POINTER(Grid), C.c_void_p, C.c_double, C.POINTER(Dirk),
+                                     C.c_double, C.c_int, C.c_double, PP]
+        L.orc_mri_evolve.restype = C.c_int
+        L.orc_mri_kw3_table.argtypes = [P, P]
+        L.orc_mri_fast_steps.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.orc_mri_fast_steps.restype = C.c_int
         L.orc_erk_evolve.argtypes = [C.POINTER(Grid), C.POINTER(Erk), C.c_double, C.c_int, PP]
         L.orc_erk_evolve.restype = C.c_int
         L.orc_chem_create.argtypes = [C.c_long, P, C.c_double]
@@ -351,6 +360,24 @@ def exact_jac_cell(chem, i, y, T):
     lib().orc_chem_exact_jac_cell(chem.h, i, _p(np.ascontiguousarray(y, dtype=np.float64)),
                                   _p(Tv), _p(J))
     return J.reshape(10, 10), float(Tv[0])
+
+
+def dirk_step_forced(chem, opts, h, y, force):
+    """One oracle DIRK step of y' = s_rhs f(y) + force (force [cell][10], constant)."""
+    force = np.ascontiguousarray(force, dtype=np.float64)
+    return lib().orc_chem_dirk_step_forced(chem.h, C.byref(opts), h, _p(y), _p(force), None)
+
+
+def mri_kw3_table():
+    c, A = np.zeros(4), np.zeros(16)
+    lib().orc_mri_kw3_table(_p(c), _p(A))
+    return c, A.reshape(4, 4)
+
+
+def mri_evolve(g, chem, energy_units, fast, h, nslow, hmax, w):
+    """Oracle multirate evolution in place on w (6 subvectors, chemistry scaled)."""
+    return lib().orc_mri_evolve(C.byref(g), chem.h, energy_units, C.byref(fast), h, nslow, hmax,
+                                _pp(w))
 
 
 def lu(M, b):
