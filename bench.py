@@ -299,6 +299,80 @@ def bench_sod(api, problems, torch, cpu=True):
     return out
 
 
+def bench_mri(api, problems, torch, n, nslow, cpu=True, cpu_n=16):
+    """Config-5 multirate primordial blast: n^3 cells, 10 species, reflecting boundaries,
+    gas at rest with a x100 Gaussian overdensity of width 0.05 at 1e4 K in 10 K gas (the
+    reference's species recipe), units of input_primordial_blast_mr.txt. Slow step =
+    min(CFL 0.3 bound, the input's h0 = 0.05); fast steps no longer than the input's
+    hmax = 1e-4 (501 per slow step); SDIRK3 fast stepper with ffast's TimeUnits scaling and
+    per-cell step rejection; chemistry forcing converted to the scaled fast variables. The
+    oracle runs the same options on a cpu_n^3 sample for one slow step (host baseline), and
+    the device is compared with it there."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    units = (3.0e70, 3.0857e30, 1.0e11)
+    eu = units[0] / units[1] / units[2] / units[2]
+    h0, hmax = 0.05, 1e-4
+
+    def setup(m):
+        fluid, chem = problems.primordial_blast_mri(m)
+        ctx = api.Context(api.make_params(m, m, m, 10, (3,) * 6, gamma=5 / 3, cfl=0.3,
+                                          units=units))
+        ctx.chem_setup(orc.load_tables(), m ** 3)
+        wd = [torch.from_numpy(a.copy()).to("cuda") for a in fluid] + \
+             [torch.from_numpy(chem.copy()).to("cuda")]
+        ctx.chem_prepare(wd[5])
+        return ctx, wd, fluid, chem
+
+    fast = api.sdirk3(rhs_scale=units[2], max_halvings=10)
+    ctx, wd, _, _ = setup(n)
+    h = min(ctx.stability(wd), h0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    failed = ctx.mri_evolve(wd, h, nslow, hmax, fast, consistent=True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    ok = all(bool(torch.isfinite(a).all()) for a in wd)
+    ctx.close()
+    nfast = sum(orc.lib().orc_mri_fast_steps(i, h, hmax) for i in (1, 2, 3))
+    out = {"workload": f"multirate primordial blast {n}^3, 10 species, reflecting, "
+                       f"{nslow} slow steps of {h:g} (MIS Knoth-Wolke 3), {nfast} SDIRK3 fast "
+                       f"steps each (hmax {hmax:g}), consistent forcing units",
+           "slow_step": h, "fast_steps_per_slow_step": nfast, "failed_fast_cell_steps": failed,
+           "finite": ok, "gpu_ms": ms,
+           "cell_slow_steps_per_s": n ** 3 * nslow / (ms * 1e-3)}
+    if cpu:
+        m = cpu_n
+        ctx, wd, fluid, chem = setup(m)
+        hs = min(ctx.stability(wd), h0)
+        ctx.mri_evolve(wd, hs, 1, hmax, fast, consistent=True)
+        oc = orc.Chem(m ** 3)
+        y = chem.copy()
+        oc.prepare(y)
+        oc.extra()
+        wo = [a.copy() for a in fluid] + [y]
+        g = orc.make_grid(m, m, m, 10, (3,) * 6, gamma=5 / 3)
+        orc.set_threads(1)
+        t0 = time.perf_counter()
+        orc.mri_evolve(g, oc, eu, orc.dirk_opts(rhs_scale=units[2], max_halvings=10), hs, 1, hmax,
+                       wo, consistent=True)
+        dt = time.perf_counter() - t0
+        diffs = []
+        for q in range(6):
+            den = float(np.abs(wo[q]).max())
+            num = float(np.abs(wd[q].cpu().numpy() - wo[q]).max())
+            diffs.append(num / den if den > 0 else num)
+        ctx.close()
+        out["cpu_baseline"] = {"value": m ** 3 / dt, "unit": "cell*slow-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle multirate run, {m}^3 cells, 1 slow step, same "
+                                         f"options, {dt:.1f} s"}
+        out["sample_max_rel_diff_vs_oracle"] = max(diffs)
+    return out
+
+
 def cpu_chem_baseline(n_sample, seed, threads, reps, h=1e3):
     """The CPU oracle's per-cell SDIRK3 step on config-3 cells, `reps` times from the same
     initial state; returns cells/s and the total seconds."""
@@ -338,7 +412,10 @@ def main():
     ap.add_argument("--no-chem", action="store_true")
     ap.add_argument("--no-vjp", action="store_true")
     ap.add_argument("--no-sod", action="store_true")
-    ap.add_argument("--only", choices=["vjp", "sod"], default=None,
+    ap.add_argument("--no-mri", action="store_true")
+    ap.add_argument("--mri-n", type=int, default=128)
+    ap.add_argument("--mri-steps", type=int, default=20)
+    ap.add_argument("--only", choices=["vjp", "sod", "mri"], default=None,
                     help="run one extra workload alone and print its JSON (profiling runs)")
     ap.add_argument("--vjp-n", type=int, default=128)
     ap.add_argument("--vjp-reps", type=int, default=10)
@@ -359,6 +436,9 @@ def main():
         problems = _load("mvd_problems", os.path.join(PKG, "problems.py"))
         if args.only == "vjp":
             print(json.dumps(bench_vjp(api, problems, torch, args.vjp_n, 777, args.vjp_reps)))
+        elif args.only == "mri":
+            print(json.dumps(bench_mri(api, problems, torch, args.mri_n, args.mri_steps,
+                                       cpu=not args.no_cpu)))
         else:
             print(json.dumps(bench_sod(api, problems, torch, cpu=not args.no_cpu)))
         return
@@ -448,6 +528,9 @@ def main():
         extra["vjp"] = bench_vjp(api, problems, torch, args.vjp_n, 777, args.vjp_reps)
     if not args.no_sod:
         extra["sod"] = bench_sod(api, problems, torch, cpu=not args.no_cpu)
+    if not args.no_mri:
+        extra["mri"] = bench_mri(api, problems, torch, args.mri_n, args.mri_steps,
+                                 cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:

@@ -176,6 +176,11 @@ typedef struct {
   int max_iters;           /* Newton iterations before the Jacobian is re-evaluated (1..100) */
   int max_setups;          /* Jacobian re-evaluations per stage before a cell fails (0..20) */
   double rhs_scale;        /* 1 for frhs, TimeUnits for ffast/fimpl */
+  int max_halvings;        /* per-cell step rejection (batched solver only; 0..20, 0 = off):
+                              a cell whose Newton budget runs out retries its step in halves,
+                              up to this many times, the stand-in for the fast integrator's
+                              step-size control; a cell that fails at the finest size keeps
+                              its last accepted state */
 } mvd_dirk_opts;
 
 typedef struct {
@@ -207,11 +212,16 @@ int mvd_chem_dirk_step_forced(mvd_context *ctx, double t, double h, const mvd_di
  * constant MIS forcing and takes ceil((c_i - c_{i-1}) h / hmax_fast) equal fixed fast steps
  * (the inner stepper's maximum step, input hmax): fluid += dt*forcing, chemistry by a
  * forced batched DIRK step with the options fast (rhs_scale = TimeUnits gives ffast). One
- * slow step is captured as a device graph and replayed. Returns 0, the number (> 0) of
- * failed fast cell steps, MVD_ILLEGAL_STATE if any slow RHS met an illegal state, or an
- * error code. */
+ * slow step is captured as a device graph and replayed. flags: 0 keeps the reference's
+ * forcing units (fslow's chemistry rates are physical and its gas-energy slot holds the
+ * code-unit total-energy rate, added unchanged to the Dengo-scaled fast variables);
+ * MVD_MRI_CONSISTENT_FORCING converts them to the scaled variables first (species rates
+ * divided by scale, the energy slot times EnergyUnits divided by the ge scale). Returns 0,
+ * the number (> 0) of failed fast cell steps, MVD_ILLEGAL_STATE if any slow RHS met an
+ * illegal state, or an error code. */
+#define MVD_MRI_CONSISTENT_FORCING 1
 int mvd_mri_evolve(mvd_context *ctx, double t0, double h, int nslow, double hmax_fast,
-                   const mvd_dirk_opts *fast, double *const *w);
+                   const mvd_dirk_opts *fast, int flags, double *const *w);
 /* Checkpointed discrete adjoint of nsteps chemistry DIRK steps (no counterpart in the
  * reference, which has no adjoint; for sensitivity drivers, e.g. of the domain-integrated
  * energy to the initial species). y (scaled, [cell][10]) is advanced nsteps steps of size h

@@ -21,6 +21,7 @@ typedef struct {
   double rtol, atol, newton_tol;
   int max_iters, max_setups;
   double rhs_scale;      /* 1 (frhs) or TimeUnits (ffast / fimpl) */
+  int max_halvings;      /* per-cell step rejection (0: off) */
 } orc_dirk;
 int orc_chem_dirk_step(void *chem, const orc_dirk *opt, double h, double *y, int *iters);
 /* discrete adjoint of nsteps DIRK steps (orc_dirk.cpp); exact scaled Jacobian of one cell
@@ -33,7 +34,7 @@ int orc_chem_dirk_step_forced(void *chem, const orc_dirk *opt, double h, double 
 /* multirate evolution, MIS form of Knoth-Wolke 3 slow / forced DIRK fast (orc_mri.cpp) */
 void orc_mri_kw3_table(double *c, double *A);
 int orc_mri_evolve(const orc_grid *g, void *chem, double energy_units, const orc_dirk *fast,
-                   double h, int nslow, double hmax, double *const *w);
+                   double h, int nslow, double hmax, int consistent, double *const *w);
 int orc_mri_fast_steps(int i, double h, double hmax);
 /* fixed-step explicit Runge-Kutta evolution of fe() (orc_erk.cpp) */
 typedef struct {

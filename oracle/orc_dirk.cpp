@@ -180,6 +180,37 @@ static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[N
   return ok_cell;
 }
 
+// The step with per-cell step rejection when opt->max_halvings > 0 (the batched solver's
+// option): a failed attempt returns the cell to its substep start (state and temperature
+// warm start) and retries in halves, up to max_halvings times; accepted substeps count in
+// units of h / 2^max_halvings. A cell that fails at the finest size keeps its last accepted
+// state. Without it, the plain step (a failed cell keeps its last Newton iterate).
+static bool step_cell_sub(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[NSP],
+                          int &nit, const double *force) {
+  if (opt->max_halvings <= 0) return step_cell(cl, opt, hstep, y, nullptr, nit, force);
+  const int mh = opt->max_halvings;
+  int halv = 0, done = 0;
+  nit = 0;
+  while (done < (1 << mh)) {
+    double z[NSP];
+    for (int s = 0; s < NSP; s++) z[s] = y[s];
+    const double Ts0 = cl.Ts, dTs0 = cl.dTs;
+    int k = 0;
+    const bool ok = step_cell(cl, opt, std::ldexp(hstep, -halv), z, nullptr, k, force);
+    nit += k;
+    if (ok) {
+      for (int s = 0; s < NSP; s++) y[s] = z[s];
+      done += 1 << (mh - halv);
+    } else {
+      cl.Ts = Ts0;
+      cl.dTs = dTs0;
+      if (halv == mh) return false;
+      halv++;
+    }
+  }
+  return true;
+}
+
 static Cell cell_of(ChemState *c, long i) {
   Cell cl{c, i, c->Ts[i], c->dTs_ge[i], {}, {}, {c->mdensity[i], c->inv_mdensity[i],
                                                 c->cie_oda[i], c->h2_oda[i]}};
@@ -205,8 +236,7 @@ extern "C" int orc_chem_dirk_step_forced(void *h, const orc_dirk *opt, double hs
     double z[NSP];
     for (int s = 0; s < NSP; s++) z[s] = y[NSP * i + s];
     int nit = 0;
-    const bool ok_cell = step_cell(cl, opt, hstep, z, nullptr, nit,
-                                   force ? force + NSP * i : nullptr);
+    const bool ok_cell = step_cell_sub(cl, opt, hstep, z, nit, force ? force + NSP * i : nullptr);
     if (!ok_cell) failed++;
     for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
     c->Ts[i] = cl.Ts;

@@ -12,7 +12,11 @@
 // forcing r_i = sum_{j<i} ((a_ij - a_{i-1,j}) / (c_i - c_{i-1})) F_j. The fast integration
 // here takes ceil((c_i - c_{i-1}) h / hmax) equal fixed steps (hmax: the inner stepper's
 // maximum step): the fluid fields (no fast dynamics) advance by dt * r_i, the chemistry by
-// one forced DIRK step of every cell.
+// one forced DIRK step of every cell. With `consistent`, the chemistry part of each slow RHS
+// is converted to the scaled variables the fast stepper integrates (species rates divided by
+// their scale, the gas-energy slot -- fslow's total-energy rate in code units -- times
+// EnergyUnits divided by the ge scale); without it the reference's units are kept (its
+// fslow rates are added unchanged to the scaled fast variables).
 #include <cstring>
 #include <vector>
 #include "oracle.h"
@@ -21,6 +25,7 @@ extern "C" int orc_fslow(const orc_grid *g, void *chem, double energy_units, dou
                          double *const *wdot);
 extern "C" int orc_chem_dirk_step_forced(void *h, const orc_dirk *opt, double hstep, double *y,
                                          const double *force, int *iters);
+extern "C" double *orc_chem_array(void *h, int which);
 
 // Knoth-Wolke 3 with b appended: c, A (row-major, 4 x 4)
 static const double KW_C[4] = {0.0, 1.0 / 3.0, 3.0 / 4.0, 1.0};
@@ -50,7 +55,8 @@ extern "C" int orc_mri_fast_steps(int i, double h, double hmax) {
 // failed fast cell steps.
 extern "C" int orc_mri_evolve(const orc_grid *g, void *chem, double energy_units,
                               const orc_dirk *fast, double h, int nslow, double hmax,
-                              double *const *w) {
+                              int consistent, double *const *w) {
+  const double *scale = orc_chem_array(chem, 0);
   const long n = g->nx * g->ny * g->nz;
   const int nc = g->nchem;
   const long len[6] = {n, n, n, n, n, (long)nc * n};
@@ -66,6 +72,11 @@ extern "C" int orc_mri_evolve(const orc_grid *g, void *chem, double energy_units
       double *Fo[6];
       for (int q = 0; q < 6; q++) Fo[q] = F[(i - 1) * 6 + q].data();
       if (orc_fslow(g, chem, energy_units, w, Fo) != 0) illegal = true;
+      if (consistent)
+        for (long c = 0; c < n; c++) {
+          for (int s = 0; s < nc - 1; s++) Fo[5][nc * c + s] /= scale[nc * c + s];
+          Fo[5][nc * c + nc - 1] = Fo[5][nc * c + nc - 1] * energy_units / scale[nc * c + nc - 1];
+        }
       const double dc = KW_C[i] - KW_C[i - 1];
       for (int q = 0; q < 6; q++)
         for (long k = 0; k < len[q]; k++) {

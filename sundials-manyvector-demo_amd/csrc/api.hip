@@ -452,7 +452,8 @@ namespace {
 int dirk_check(const mvd_dirk_opts *o, mvd::DirkDev &d) {
   if (!o || o->nstage < 1 || o->nstage > mvd::MAXSTAGE) return MVD_BAD_ARGUMENT;
   if (o->max_iters < 1 || o->max_iters > 100 || o->max_setups < 0 || o->max_setups > 20 ||
-      !(o->newton_tol > 0) || !(o->rtol >= 0) || !(o->atol > 0) || !(o->rhs_scale > 0))
+      !(o->newton_tol > 0) || !(o->rtol >= 0) || !(o->atol > 0) || !(o->rhs_scale > 0) ||
+      o->max_halvings < 0 || o->max_halvings > 20)
     return MVD_BAD_ARGUMENT;
   for (int i = 0; i < o->nstage; i++) {
     if (!(o->A[i * 4 + i] > 0) || o->A[i * 4 + i] != o->A[0]) return MVD_UNSUPPORTED;  // SDIRK
@@ -464,6 +465,7 @@ int dirk_check(const mvd_dirk_opts *o, mvd::DirkDev &d) {
   d.rtol = o->rtol; d.atol = o->atol; d.ntol = o->newton_tol;
   d.max_iters = o->max_iters; d.max_setups = o->max_setups;
   d.rhs_scale = o->rhs_scale;
+  d.max_halvings = o->max_halvings;
   return MVD_SUCCESS;
 }
 
@@ -487,6 +489,7 @@ static int dirk_step_common(mvd_context *ctx, double h, const mvd_dirk_opts *o, 
   mvd::DirkDev d;
   int rc = dirk_check(o, d);
   if (rc) return rc;
+  if (plain && d.max_halvings > 0) return MVD_UNSUPPORTED;
   hipEvent_t e0, e1;
   if (!dev_ok(hipEventCreate(&e0)) || !dev_ok(hipEventCreate(&e1))) return MVD_DEVICE_ERROR;
   (void)hipEventRecord(e0, ctx->stream);
@@ -527,7 +530,8 @@ int mvd_chem_dirk_step_forced(mvd_context *ctx, double, double h, const mvd_dirk
 }
 
 int mvd_mri_evolve(mvd_context *ctx, double, double h, int nslow, double hmax_fast,
-                   const mvd_dirk_opts *fast, double *const *w) {
+                   const mvd_dirk_opts *fast, int flags, double *const *w) {
+  if (flags & ~MVD_MRI_CONSISTENT_FORCING) return MVD_BAD_ARGUMENT;
   if (!ctx || !(h > 0) || nslow < 0 || !(hmax_fast > 0)) return MVD_BAD_ARGUMENT;
   if (h / hmax_fast > 1e6) return MVD_BAD_ARGUMENT;
   if (ctx->g.nchem != mvd::NSP) return MVD_UNSUPPORTED;
@@ -541,7 +545,8 @@ int mvd_mri_evolve(mvd_context *ctx, double, double h, int nslow, double hmax_fa
   if (nslow == 0) return MVD_SUCCESS;
   int nfail = 0;
   rc = mvd::mri_evolve(ctx->g, ctx->fe, ctx->chem, 1.0 / ctx->energy_units, ctx->mri, d, h,
-                       nslow, hmax_fast, default_waves(ctx), ctx->dirk_block, w, ctx->stream,
+                       nslow, hmax_fast, (flags & MVD_MRI_CONSISTENT_FORCING) != 0,
+                       default_waves(ctx), ctx->dirk_block, w, ctx->stream,
                        ctx->host_flag, &nfail);
   if (rc == -2) return MVD_UNSUPPORTED;
   if (rc) return MVD_DEVICE_ERROR;
@@ -557,6 +562,7 @@ int mvd_chem_dirk_adjoint(mvd_context *ctx, double, double h, int nsteps, int ck
   mvd::DirkDev d;
   int rc = dirk_check(o, d);
   if (rc) return rc;
+  if (d.max_halvings > 0) return MVD_UNSUPPORTED;     // the adjoint replays fixed steps
   // stiffly accurate tables only (y_{n+1} is the last stage): c_s = sum_j a_sj = 1
   double cs = 0.0;
   for (int j = 0; j < d.nstage; j++) cs += d.A[(d.nstage - 1) * 4 + j];

@@ -323,8 +323,15 @@ __device__ inline void lu_solve_regs(const double (&a)[NSP][NSP], unsigned long 
 constexpr int STAGE_WORDS = (MAXSTAGE - 1) * NSP;
 
 // FORCED: y' = s_rhs f(y) + force, force constant over the step ([cell][10], the forcing an
-// MRI method hands its fast stepper, multirate_chem_hydro_main.cpp:922-946)
-template <bool FORCED>
+// MRI method hands its fast stepper, multirate_chem_hydro_main.cpp:922-946).
+// SUBSTEP (o.max_halvings > 0): per-cell step rejection. When a cell's Newton budget runs
+// out, the cell returns to the start of its current substep (state and temperature warm
+// start) and retries it in halves, up to max_halvings times; accepted substeps are
+// counted in units of the finest one (h / 2^max_halvings), so their sizes add up to h
+// exactly. A cell that fails at the finest size keeps its last accepted state and counts
+// as failed. Without SUBSTEP a failed cell keeps its last Newton iterate (the plain
+// solver's behaviour).
+template <bool FORCED, bool SUBSTEP>
 __global__ void __launch_bounds__(256)
 chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double *y,
                   const double *force, double *work, int *queue, int *rows) {
@@ -355,7 +362,10 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     //   convergence: RHS(z), setup, and the stage's Newton loop restarts.
     bool setup = true, ok = true;
     int st = 0, it = 0, setups = 0, nit = 0, nsetup = 0;
-    const double hg = h * o.A[0];                          // SDIRK: one diagonal value
+    double hs = h;                                         // current (sub)step
+    double hg = h * o.A[0];                                // SDIRK: one diagonal value
+    double Ts0 = Ts, dTs0 = dTs;                           // at the substep start
+    int halv = 0, done = 0;                                // substep level, units done
     unsigned long long pv = 0;
     for (;;) {
       // The lane offset is made opaque on every pass so that the per-lane addresses of the
@@ -389,7 +399,21 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
       nit++;
       it++;
       if (wrms_norm(d, z, o.rtol, o.atol) <= o.ntol) {
-        if (++st == o.nstage) break;
+        if (++st == o.nstage) {
+          if (!SUBSTEP) break;
+          done += 1 << (o.max_halvings - halv);
+          if (done >= (1 << o.max_halvings)) break;
+          // substep accepted: it becomes the start of the next one
+#pragma unroll
+          for (int s = 0; s < NSP; s++) { y[NSP * i + s] = z[s]; r[s] = z[s] + hs * 0.0; }
+          Ts0 = Ts;
+          dTs0 = dTs;
+          st = 0;
+          it = 0;
+          setups = 0;
+          setup = true;
+          continue;
+        }
         // F_{st-1} = (z - r)/hg (ARKODE's implicit-stage derivative); the next stage's r
 #pragma unroll
         for (int s = 0; s < NSP; s++) F[((st - 1) * NSP + s) * 64] = (z[s] - r[s]) / hg;
@@ -399,12 +423,34 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
 #pragma unroll
           for (int j = 0; j < MAXSTAGE - 1; j++)
             if (j < st) a += dirk_coef(o, st, j) * F[(j * NSP + s) * 64];
-          r[s] = y[NSP * i + s] + h * a;                   // y still holds y_n
+          r[s] = y[NSP * i + s] + hs * a;                  // y holds the (sub)step start
         }
         it = 0;
         setups = 0;
       } else if (it == o.max_iters) {
-        if (++setups > o.max_setups) { ok = false; break; }
+        if (++setups > o.max_setups) {
+          if (SUBSTEP) {
+            // back to the substep start (y holds it)
+#pragma unroll
+            for (int s = 0; s < NSP; s++) z[s] = y[NSP * i + s];
+            Ts = Ts0;
+            dTs = dTs0;
+            if (halv < o.max_halvings) {
+              halv++;
+              hs = ldexp(h, -halv);
+              hg = hs * o.A[0];
+#pragma unroll
+              for (int s = 0; s < NSP; s++) r[s] = z[s] + hs * 0.0;
+              st = 0;
+              it = 0;
+              setups = 0;
+              setup = true;
+              continue;
+            }
+          }
+          ok = false;
+          break;
+        }
         setup = true;
       }
     }
@@ -610,12 +656,19 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   char *w = (char *)s.work;
   int *rows = (int *)(w + b_blk);
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
-  if (force)
-    hipLaunchKernelGGL(chem_dirk_batched<true>, dim3((unsigned)grid), dim3(block), 0, st, n, s.cc,
-                       ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
+  const bool sub = o.max_halvings > 0;
+  if (force && sub)
+    hipLaunchKernelGGL((chem_dirk_batched<true, true>), dim3((unsigned)grid), dim3(block), 0, st,
+                       n, s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
+  else if (force)
+    hipLaunchKernelGGL((chem_dirk_batched<true, false>), dim3((unsigned)grid), dim3(block), 0, st,
+                       n, s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
+  else if (sub)
+    hipLaunchKernelGGL((chem_dirk_batched<false, true>), dim3((unsigned)grid), dim3(block), 0, st,
+                       n, s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
   else
-    hipLaunchKernelGGL(chem_dirk_batched<false>, dim3((unsigned)grid), dim3(block), 0, st, n,
-                       s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
+    hipLaunchKernelGGL((chem_dirk_batched<false, false>), dim3((unsigned)grid), dim3(block), 0,
+                       st, n, s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
   hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(nrow)), dim3(256), 0, st,
                      (const int *)rows, nrow, s.counters + 1);
   return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -18,6 +18,14 @@
 // combinations and 3*nfast fast steps -- is captured as one hipGraph and replayed. fe()'s
 // legal-state bits and the failed-cell counts of every fast step accumulate on the device
 // and are read back once.
+//
+// Forcing units. The reference's fslow returns the chemistry rates in physical units (the
+// advection of the unscaled species) and the total-energy rate, in code units, in the gas
+// energy slot, and MRIStep adds them unchanged to the fast stepper's Dengo-scaled
+// variables. That is reproduced by default. With `consistent` the
+// chemistry part of each slow RHS is first converted to the scaled variables the fast
+// stepper integrates: species rates divided by their scale, the gas-energy slot times
+// EnergyUnits (fslow sets et = ge / EnergyUnits) divided by the ge scale.
 #include "mvd_mri.hpp"
 
 namespace mvd {
@@ -33,6 +41,16 @@ constexpr double KW_A[4][4] = {{0.0, 0.0, 0.0, 0.0},
 
 __global__ void mri_or_flag(const unsigned *src, unsigned *dst) { *dst |= *src; }
 __global__ void mri_add_int(const int *src, int *dst) { *dst += *src; }
+// chemistry rates of one slow RHS into the fast stepper's scaled variables
+__global__ void __launch_bounds__(256)
+mri_scale_forcing(long n, double *Fc, const double *scale, double eunits) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int s = 0; s < NSP - 1; s++) Fc[NSP * i + s] /= scale[NSP * i + s];
+  Fc[NSP * i + NSP - 1] = Fc[NSP * i + NSP - 1] * eunits / scale[NSP * i + NSP - 1];
+}
+
 __global__ void mri_zero(unsigned *flag, int *nfail) {
   *flag = 0u;
   *nfail = 0;
@@ -60,7 +78,7 @@ namespace {
 // one slow step on st
 int enqueue_slow_step(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv,
                       MriState &m, const DirkDev &o, double h, double hmax, int waves,
-                      int block, hipStream_t st) {
+                      int block, bool consistent, hipStream_t st) {
   const long n = (long)g.nx * g.ny * g.nz;
   const long len = (5L + g.nchem) * n;
   double *Y[6];
@@ -81,6 +99,9 @@ int enqueue_slow_step(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_
     hipLaunchKernelGGL(mri_or_flag, dim3(1), dim3(1), 0, st, (const unsigned *)fe.flag, m.flag);
     if (fslow_finish_launch(g, Fi, st)) return -1;
     if (chem_scale_launch(n * g.nchem, Y[5], ch.scale, 0, st)) return -1;
+    if (consistent)
+      hipLaunchKernelGGL(mri_scale_forcing, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                         n, Fi[5], (const double *)ch.scale, 1.0 / eunit_inv);
     // forcing r_i over the whole ManyVector
     const double dc = KW_C[i] - KW_C[i - 1];
     double coef[3];
@@ -124,8 +145,9 @@ void mri_free(MriState &m) {
 }
 
 int mri_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, MriState &m,
-               const DirkDev &o, double h, int nslow, double hmax, int waves, int block,
-               double *const *w, hipStream_t st, unsigned *host_flag, int *host_fail) {
+               const DirkDev &o, double h, int nslow, double hmax, bool consistent, int waves,
+               int block, double *const *w, hipStream_t st, unsigned *host_flag,
+               int *host_fail) {
   const long n = (long)g.nx * g.ny * g.nz;
   const long len = (5L + g.nchem) * n;
   if (!m.Y) {
@@ -147,7 +169,8 @@ int mri_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, Mr
   hipLaunchKernelGGL(mri_zero, dim3(1), dim3(1), 0, st, m.flag, m.nfail);
   const bool timer_on = fe.timer.on;
   fe.timer.on = false;                    // no event records inside the graph
-  bool rebuild = !m.exec || m.h != h || m.hmax != hmax || !same_opts(m.o, o) ||
+  bool rebuild = !m.exec || m.h != h || m.hmax != hmax || m.consistent != consistent ||
+                 !same_opts(m.o, o) ||
                  m.waves != waves || m.block != block;
   int rc = 0;
   for (int step = 0; step < nslow && rc == 0; step++) {
@@ -157,7 +180,7 @@ int mri_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, Mr
     }
     // eager step (allocates what the fast kernels need), then capture the step for the
     // remaining ones
-    rc = enqueue_slow_step(g, fe, ch, eunit_inv, m, o, h, hmax, waves, block, st);
+    rc = enqueue_slow_step(g, fe, ch, eunit_inv, m, o, h, hmax, waves, block, consistent, st);
     if (rc || step + 1 == nslow) break;
     if (m.exec) { (void)hipGraphExecDestroy(m.exec); m.exec = nullptr; }
     hipGraph_t graph = nullptr;
@@ -165,7 +188,8 @@ int mri_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, Mr
       rc = -1;
       break;
     }
-    const int rcc = enqueue_slow_step(g, fe, ch, eunit_inv, m, o, h, hmax, waves, block, m.cap);
+    const int rcc =
+        enqueue_slow_step(g, fe, ch, eunit_inv, m, o, h, hmax, waves, block, consistent, m.cap);
     const hipError_t ec = hipStreamEndCapture(m.cap, &graph);
     if (rcc || ec != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -177,6 +201,7 @@ int mri_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, Mr
     if (ei != hipSuccess) { m.exec = nullptr; rc = -1; break; }
     m.h = h;
     m.hmax = hmax;
+    m.consistent = consistent;
     m.o = o;
     m.waves = waves;
     m.block = block;

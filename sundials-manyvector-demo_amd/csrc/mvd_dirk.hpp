@@ -23,6 +23,7 @@ struct DirkDev {
   double rtol, atol, ntol;
   int max_iters, max_setups;
   double rhs_scale;
+  int max_halvings;     // per-cell step rejection (batched solver), 0: off
 };
 
 // force (nullable): constant forcing added to the scaled RHS, [cell][10]

@@ -110,13 +110,14 @@ def chem_cells(n, seed=4321, alpha=(1.0,) * 8, ntot=(1e-2, 1e2), temp=(10.0, 1e4
 
 
 def primordial_blast_mri(nx, ny=None, nz=None, bump=100.0, width=0.05, t_in=1e4, t_amb=10.0,
-                         gamma=5.0 / 3.0, units=(3.0e70, 3.0857e30, 1.0e11)):
+                         gamma=5.0 / 3.0, units=(3.0e70, 3.0857e30, 1.0e11), core_trace=True):
     """Multirate primordial blast (configuration 5): gas at rest on [0,1]^3 with a Gaussian
     overdensity of `bump` times the background density0 = 1e2*mH and width `width` centred
     in the box, temperature t_amb outside rising to t_in at the centre, both with the
     reference's profile shape exp(-2 r^2 / width^2) (primordial_blast.cpp:218-231); species
     composition as primordial_blast.cpp:238-271 (trace 1e-3 fractions outside r^2/w^2 < 2,
-    tiny = 1e-40 / small = 1e-12 inside, Hfrac = 0.76), ge = kB T n / (rho (gamma - 1)).
+    tiny = 1e-40 / small = 1e-12 inside, Hfrac = 0.76; core_trace=False uses the outside
+    fractions everywhere), ge = kB T n / (rho (gamma - 1)).
     Units (mass, length, time) of tests/primordial_blast/input_primordial_blast_mr.txt:34-37.
     Returns (fluid: 5 flat code-unit arrays, chem: AoS physical, 10 per cell, as
     initial_conditions stores them before prepare_Dengo_structures)."""
@@ -133,7 +134,7 @@ def primordial_blast_mri(nx, ny=None, nz=None, bump=100.0, width=0.05, t_in=1e4,
     prof = np.exp(-2.0 * rsq / width / width)
     density = density0 * (1.0 + (bump - 1.0) * prof)
     T = t_amb + (t_in - t_amb) * prof
-    inside = rsq / width / width < 2.0
+    inside = (rsq / width / width < 2.0) & core_trace
     H2I = np.where(inside, tiny, 1e-3) * density
     H2II = np.where(inside, tiny, 1e-3) * density
     HII = np.where(inside, small, 1e-3) * density

@@ -33,7 +33,8 @@ class DirkOpts(C.Structure):
     """mvd_dirk_opts: SDIRK table (row-major 4x4 A) and Newton controls."""
     _fields_ = [("nstage", C.c_int), ("A", C.c_double * 16), ("c", C.c_double * 4),
                 ("rtol", C.c_double), ("atol", C.c_double), ("newton_tol", C.c_double),
-                ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double)]
+                ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double),
+                ("max_halvings", C.c_int)]
 
 
 class ErkTable(C.Structure):
@@ -105,7 +106,7 @@ def lib():
         "mvd_chem_dirk_step_forced": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, _P,
                                        C.POINTER(DirkStats)], C.c_int),
         "mvd_mri_evolve": ([vp, C.c_double, C.c_double, C.c_int, C.c_double,
-                            C.POINTER(DirkOpts), C.POINTER(_P)], C.c_int),
+                            C.POINTER(DirkOpts), C.c_int, C.POINTER(_P)], C.c_int),
         "mvd_chem_dirk_adjoint": ([vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.POINTER(DirkOpts), _P, _P, _P], C.c_int),
     }
@@ -275,12 +276,12 @@ class Context:
     def chem_jac(self, y, J, t=0.0):
         _check("mvd_chem_jac", lib().mvd_chem_jac(self.h, t, dptr(y), dptr(J)))
 
-    def mri_evolve(self, w, h, nslow, hmax_fast, fast_opts, t0=0.0):
+    def mri_evolve(self, w, h, nslow, hmax_fast, fast_opts, consistent=False, t0=0.0):
         """Multirate evolution (MIS Knoth-Wolke 3 slow, forced batched DIRK fast with steps of
         at most hmax_fast) in place on w; returns the failed fast cell steps (0: all
-        converged)."""
+        converged). consistent: chemistry forcing in the fast stepper's scaled variables."""
         rc = lib().mvd_mri_evolve(self.h, t0, h, int(nslow), hmax_fast, C.byref(fast_opts),
-                                  dptrs(w))
+                                  1 if consistent else 0, dptrs(w))
         if rc < 0:
             _check("mvd_mri_evolve", rc)
         return rc

@@ -35,7 +35,8 @@ class Dirk(C.Structure):
     """orc_dirk (same layout as the library's mvd_dirk_opts)."""
     _fields_ = [("nstage", C.c_int), ("A", C.c_double * 16), ("c", C.c_double * 4),
                 ("rtol", C.c_double), ("atol", C.c_double), ("newton_tol", C.c_double),
-                ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double)]
+                ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double),
+                ("max_halvings", C.c_int)]
 
 
 def sdirk3_table():
@@ -88,7 +89,7 @@ def erk_evolve(g, tab, h, nsteps, w):
 
 
 def dirk_opts(A=None, c=None, nstage=3, rtol=1e-5, atol=1e-9, newton_tol=0.1, max_iters=5,
-              max_setups=2, rhs_scale=1.0):
+              max_setups=2, rhs_scale=1.0, max_halvings=0):
     if A is None:
         A, c = sdirk3_table()
     o = Dirk()
@@ -99,6 +100,7 @@ def dirk_opts(A=None, c=None, nstage=3, rtol=1e-5, atol=1e-9, newton_tol=0.1, ma
         o.c[k] = v
     o.rtol, o.atol, o.newton_tol = rtol, atol, newton_tol
     o.max_iters, o.max_setups, o.rhs_scale = max_iters, max_setups, rhs_scale
+    o.max_halvings = max_halvings
     return o
 
 
@@ -146,7 +148,7 @@ def lib():
                                                 C.POINTER(C.c_int)]
         L.orc_chem_dirk_step_forced.restype = C.c_int
         L.orc_mri_evolve.argtypes = [C.POINTER(Grid), C.c_void_p, C.c_double, C.POINTER(Dirk),
-                                     C.c_double, C.c_int, C.c_double, PP]
+                                     C.c_double, C.c_int, C.c_double, C.c_int, PP]
         L.orc_mri_evolve.restype = C.c_int
         L.orc_mri_kw3_table.argtypes = [P, P]
         L.orc_mri_fast_steps.argtypes = [C.c_int, C.c_double, C.c_double]
@@ -374,10 +376,10 @@ def mri_kw3_table():
     return c, A.reshape(4, 4)
 
 
-def mri_evolve(g, chem, energy_units, fast, h, nslow, hmax, w):
+def mri_evolve(g, chem, energy_units, fast, h, nslow, hmax, w, consistent=False):
     """Oracle multirate evolution in place on w (6 subvectors, chemistry scaled)."""
     return lib().orc_mri_evolve(C.byref(g), chem.h, energy_units, C.byref(fast), h, nslow, hmax,
-                                _pp(w))
+                                1 if consistent else 0, _pp(w))
 
 
 def lu(M, b):

@@ -66,16 +66,18 @@ def mri_pair(mvd, torch, dims, bc):
     return ctx, wd, oc, wo
 
 
-def test_mri_matches_oracle(mvd, torch_gpu):
+@pytest.mark.parametrize("consistent", [False, True])
+def test_mri_matches_oracle(mvd, torch_gpu, consistent):
     torch = torch_gpu
     dims, bc = (12, 10, 8), (3,) * 6
     n = 12 * 10 * 8
-    print(f"[mri] dims={dims} 3 slow steps", flush=True)
+    print(f"[mri] dims={dims} 3 slow steps consistent={consistent}", flush=True)
     ctx, wd, oc, wo = mri_pair(mvd, torch, dims, bc)
     g = orc.make_grid(*dims, 10, bc, gamma=5 / 3)
-    assert orc.mri_evolve(g, oc, EU, orc.dirk_opts(rhs_scale=UNITS[2]), 0.05, 3, 0.01, wo) == 0
+    assert orc.mri_evolve(g, oc, EU, orc.dirk_opts(rhs_scale=UNITS[2]), 0.05, 3, 0.01, wo,
+                          consistent) == 0
     fast = mvd.sdirk3(rhs_scale=UNITS[2])
-    assert ctx.mri_evolve(wd, 0.05, 3, 0.01, fast) == 0
+    assert ctx.mri_evolve(wd, 0.05, 3, 0.01, fast, consistent) == 0
     for q in range(5):
         assert normwise_err(wd[q].cpu().numpy(), wo[q]) <= 1e-8, q
     got = wd[5].cpu().numpy().reshape(n, 10)
@@ -107,4 +109,43 @@ def test_mri_graph_replay_is_exact(mvd, torch_gpu):
     ctx.chem_state_set("Ts", Ts0)
     assert ctx.mri_evolve(c, 0.05, 2, 0.02, fast) == 0
     assert not torch.equal(c[5], a[5])
+    ctx.close()
+
+
+def test_step_rejection_matches_oracle(mvd, torch_gpu):
+    """Per-cell step rejection (max_halvings): with a Newton budget too small for the full
+    step many cells fail; retried in halves most converge, the rest keep their last accepted
+    state. Device and oracle take the same substeps (states to 1e-10, the same failed
+    cells)."""
+    torch = torch_gpu
+    n = 4096
+    print(f"[step rejection] n={n}", flush=True)
+    y, _ = problems.chem_cells(n, seed=71)
+    ctx = mvd.Context(mvd.make_params(4, 4, 4, 10))
+    ctx.chem_setup(orc.load_tables(), n)
+    y0 = torch.from_numpy(y.copy()).to("cuda")
+    ctx.chem_prepare(y0)
+    Ts0 = ctx.chem_state("Ts", torch).clone()
+    results = {}
+    for mh in (0, 6):
+        oc = orc.Chem(n)
+        yo = y.copy()
+        oc.prepare(yo)
+        oc.extra()
+        opts_o = orc.dirk_opts(max_iters=2, max_setups=0, max_halvings=mh)
+        fo, _ = orc.dirk_step(oc, opts_o, 1e5, yo)
+        yd = y0.clone()
+        ctx.chem_state_set("Ts", Ts0)
+        fd, _ = ctx.dirk_step(yd, 1e5, mvd.sdirk3(max_iters=2, max_setups=0, max_halvings=mh))
+        results[mh] = (fo, fd)
+        assert fd == fo
+        if mh:
+            got = yd.cpu().numpy().reshape(n, 10)
+            ref = yo.reshape(n, 10)
+            for s in range(10):
+                assert normwise_err(got[:, s], ref[:, s]) <= 1e-10, s
+            Tg = ctx.chem_state("Ts", torch).cpu().numpy()
+            assert np.max(np.abs(Tg - oc.array("Ts")) / oc.array("Ts")) <= 1e-10
+    # h = 1e5 s: most cells fail without halving, a few even at the finest size
+    assert results[0][0] > 500 and 0 < results[6][0] < results[0][0]
     ctx.close()
