@@ -158,6 +158,15 @@ int mvd_chem_lsolve(mvd_context *ctx, const double *b, double *x);
  * log T). Updates Ts/dTs like mvd_chem_rhs; ydot (may be NULL) receives the RHS itself. */
 int mvd_chem_vjp(mvd_context *ctx, double t, const double *y, const double *v, double *ybar,
                  double *ydot);
+/* Self-check of the device temperature solve (no counterpart in the reference; for tests
+ * and diagnostics): the ten Newton steps of cvklu_calculate_temperature
+ * (raja_primordial_network.cpp:908-1011) end early at the first bitwise repeat of the
+ * iteration (a cycle of period <= 4, whose remaining steps are known). Runs the early form
+ * and the full ten steps from the warm starts Ts0 ([cell]) for every cell of y (scaled,
+ * [cell][10]) and stores the number of cells whose (T, dTs_ge) differ bitwise; the
+ * per-cell chemistry state is not changed. */
+int mvd_chem_temperature_check(mvd_context *ctx, const double *y, const double *Ts0,
+                               long *mismatches);
 /* device pointers of per-cell chemistry state: 0 scale, 1 inv_scale, 2 Ts, 3 dTs_ge,
  * 4 mdensity, 5 inv_mdensity, 6 cie_optical_depth, 7 h2_optical_depth */
 double *mvd_chem_state(mvd_context *ctx, int which);

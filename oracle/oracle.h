@@ -46,7 +46,8 @@ typedef struct {
 int orc_erk_evolve(const orc_grid *g, const orc_erk *tab, double h, int nsteps, double *const *w);
 int orc_lu(const double *M, double *LU, int *piv, const double *b, double *x);
 /* bitwise check of the early exit of the device temperature solve (orc_chem.cpp) */
-long orc_temperature_exit_check(void *chem, const double *y, const double *Ts0, int *steps);
+long orc_temperature_exit_check(void *chem, const double *y, const double *Ts0, int max_period,
+                                int *steps, int *period);
 /* vector-Jacobian products of fe() and of one face flux (orc_vjp.cpp) */
 int orc_face_flux_vjp(const double *stencil, int dir, int nchem, double gamma,
                       const double *Fbar, double *sbar);

@@ -207,14 +207,16 @@ extern "C" void orc_chem_opcount(void *h, unsigned long long *out) {
   out[9] = OpCounter::sqrt_; out[10] = OpCounter::cmp; out[11] = OpCounter::trans;
 }
 
-// The early exit of the device temperature solve, restated: the step map depends only on its
-// input, so a fixed point (x' == x) or a 2-cycle (x' == previous input) ends the ten steps
-// with the values they would have produced (parity of the step picks the member of the
-// cycle). Runs both forms from the warm starts Ts0 for every cell of y (scaled) and returns
-// the number of cells whose (T, dT/dge) differ bitwise; steps[i] gets the steps the early
-// form took (may be null).
+// The early exit of the device temperature solve, restated: with x_0 = Ts0,
+// x_{k+1} = S(x_k) and d_k = dge_dT(x_k), the ten steps give (x_10, d_9). S and d depend
+// only on their input, so at the first repeat x_m == x_{m-p} (p <= max_period) the
+// sequence is periodic from index m-p and x_10 = x_{m-p + (10-m) mod p},
+// d_9 = d_{m-p + (9-m) mod p}. Runs both forms from the warm starts Ts0 for every cell of
+// y (scaled) and returns the number of cells whose (T, dT/dge) differ bitwise; steps[i]
+// gets the steps the early form took and period[i] the period it stopped on (0: none),
+// either may be null.
 extern "C" long orc_temperature_exit_check(void *h, const double *y, const double *Ts0,
-                                           int *steps) {
+                                           int max_period, int *steps, int *period) {
   Chem *c = (Chem *)h;
   long bad = 0;
   for (long i = 0; i < c->n; i++) {
@@ -223,30 +225,36 @@ extern "C" long orc_temperature_exit_check(void *h, const double *y, const doubl
     double T10 = Ts0[i], d10 = 0.0;
     temperature(c->g, ya, T10, d10);
     const double density = temperature_density(ya);
-    double Ts = Ts0[i], Tnew = Ts * 1.1, dge_dT = 0.0, x_prev = 0.0, d_prev = 0.0;
-    int it = 0;
-    for (; it < 10; it++) {
-      const double T = Ts;
-      Tnew = temperature_step(c->g, ya, density, T, dge_dT);
-      Ts = Tnew;
-      if (std::memcmp(&Tnew, &T, sizeof(double)) == 0) break;
-      if (it > 0 && std::memcmp(&Tnew, &x_prev, sizeof(double)) == 0) {
-        if ((it & 1) == 0) {
-          Tnew = T;
-          dge_dT = d_prev;
+    double x[11], d[10];
+    x[0] = Ts0[i];
+    int m = 10, per = 0;
+    for (int k = 0; k < 10; k++) {
+      x[k + 1] = temperature_step(c->g, ya, density, x[k], d[k]);
+      for (int q = 1; q <= max_period && q <= k + 1; q++)
+        if (std::memcmp(&x[k + 1], &x[k + 1 - q], sizeof(double)) == 0) {
+          per = q;
+          break;
         }
+      if (per) {
+        m = k + 1;
         break;
       }
-      x_prev = T;
-      d_prev = dge_dT;
     }
-    Ts = Tnew;
+    double Ts, dge_dT;
+    if (per) {
+      Ts = x[m - per + (10 - m) % per];
+      dge_dT = d[m - per + ((9 - m) % per + per) % per];
+    } else {
+      Ts = x[10];
+      dge_dT = d[9];
+    }
     if (Ts < c->g.tmin) Ts = c->g.tmin;
     else if (Ts > c->g.tmax) Ts = c->g.tmax;
     const double dTs = 1.0 / dge_dT;
     if (std::memcmp(&Ts, &T10, sizeof(double)) != 0 || std::memcmp(&dTs, &d10, sizeof(double)) != 0)
       bad++;
-    if (steps) steps[i] = it < 10 ? it + 1 : 10;
+    if (steps) steps[i] = m;
+    if (period) period[i] = per;
   }
   return bad;
 }

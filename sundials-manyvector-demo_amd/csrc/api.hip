@@ -313,6 +313,13 @@ int mvd_chem_vjp(mvd_context *ctx, double, const double *y, const double *v, dou
   return mvd::chem_vjp(ctx->chem, y, v, ybar, ydot, ctx->stream, true);
 }
 
+int mvd_chem_temperature_check(mvd_context *ctx, const double *y, const double *Ts0,
+                               long *mismatches) {
+  if (!ctx || !y || !Ts0 || !mismatches) return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  return mvd::chem_temperature_check(ctx->chem, y, Ts0, mismatches, ctx->stream);
+}
+
 int mvd_chem_jac(mvd_context *ctx, double, const double *y, double *J) {
   if (!ctx || !y || !J) return MVD_BAD_ARGUMENT;
   if (!ctx->chem.ready) return MVD_NOT_READY;

@@ -119,6 +119,26 @@ chem_vjp_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, const double
   for (int s = 0; s < NSP; s++) ybar[NSP * i + s] = yb[s] * p.scale[NSP * i + s];
 }
 
+// Self-check of the temperature solve's early exit: the early form and the ten steps from
+// the same warm start Ts0 for every cell of y (scaled); counts the cells whose (T, dTs_ge)
+// differ bitwise. The per-cell state (Ts, dTs_ge) is not touched.
+__global__ void __launch_bounds__(256)
+chem_temperature_check_kernel(long n, ChemConst cc, const double *scale, const double *yd,
+                              const double *Ts0, int *bad) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double y[NSP];
+#pragma unroll
+  for (int v = 0; v < NSP; v++) y[v] = yd[NSP * i + v] * scale[NSP * i + v];
+  double Ta = Ts0[i], da, Tb = Ts0[i], db;
+  chem_temperature(cc, y, Ta, da);
+  chem_temperature_ten(cc, y, Tb, db);
+  if (__double_as_longlong(Ta) != __double_as_longlong(Tb) ||
+      __double_as_longlong(da) != __double_as_longlong(db))
+    atomicAdd(bad, 1);
+}
+
 // calculate_jacobian_cvklu (raja_primordial_network.cpp:1393-1965): dense column-major
 // blocks, each entry times inv_scale[row]*scale[col].
 __global__ void __launch_bounds__(256)
@@ -232,6 +252,22 @@ int chem_vjp(ChemState &s, const double *y, const double *v, double *ybar, doubl
                      v, ybar, ydot);
   if (hipGetLastError() != hipSuccess) return MVD_DEVICE_ERROR;
   if (sync && hipStreamSynchronize(st) != hipSuccess) return MVD_DEVICE_ERROR;
+  return MVD_SUCCESS;
+}
+
+int chem_temperature_check(ChemState &s, const double *y, const double *Ts0, long *bad,
+                           hipStream_t st) {
+  if (!s.ready) return MVD_NOT_READY;
+  int *d = s.counters + 63;      // last word of the 64: the DIRK uses the first 1 + 20
+  int h = 0;
+  if (hipMemsetAsync(d, 0, sizeof(int), st) != hipSuccess) return MVD_DEVICE_ERROR;
+  hipLaunchKernelGGL(chem_temperature_check_kernel, dim3(nb(s.n)), dim3(256), 0, st, s.n, s.cc,
+                     (const double *)s.scale, y, Ts0, d);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return MVD_DEVICE_ERROR;
+  *bad = h;
   return MVD_SUCCESS;
 }
 

@@ -40,14 +40,16 @@ __device__ inline CellIn load_cell(const ChemPtrsD &p, long i) {
                 CellAux{p.md[i], p.imd[i], p.cie[i], p.h2o[i]}};
 }
 
-// calculate_rhs_cvklu for one cell in scaled variables, times srhs; updates Ts/dTs
-__device__ inline void cell_rhs(const ChemConst &cc, const CellIn &c, double &Ts, double &dTs,
-                                const double (&y)[NSP], double srhs, double (&f)[NSP]) {
+// calculate_rhs_cvklu for one cell in scaled variables, times srhs; updates Ts/dTs. The
+// temperature solve reads the gamma tables from gt (global memory or an LDS copy).
+__device__ inline void cell_rhs(const ChemConst &cc, const GammaTabs &gt, const CellIn &c,
+                                double &Ts, double &dTs, const double (&y)[NSP], double srhs,
+                                double (&f)[NSP]) {
 #pragma clang fp contract(off)
   double ya[NSP];
 #pragma unroll
   for (int s = 0; s < NSP; s++) ya[s] = y[s] * c.s(s);
-  chem_temperature(cc, ya, Ts, dTs);
+  chem_temperature(cc, gt, ya, Ts, dTs);
   // species rows from the reaction tables, then the energy row from the cooling tables;
   // the fence keeps the two groups of table loads from being scheduled together
   double k[NRATE];
@@ -63,6 +65,11 @@ __device__ inline void cell_rhs(const ChemConst &cc, const CellIn &c, double &Ts
   f[9] *= c.ax.inv_mdensity;
 #pragma unroll
   for (int s = 0; s < NSP; s++) f[s] *= srhs;
+}
+
+__device__ inline void cell_rhs(const ChemConst &cc, const CellIn &c, double &Ts, double &dTs,
+                                const double (&y)[NSP], double srhs, double (&f)[NSP]) {
+  cell_rhs(cc, gamma_tabs(cc), c, Ts, dTs, y, srhs, f);
 }
 
 // A cell's 10x10 matrix in memory: element (r, c) at b[r*rs + c*cs]. The batched solver
@@ -336,10 +343,15 @@ __global__ void __launch_bounds__(256)
 chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double *y,
                   const double *force, double *work, int *queue, int *rows) {
 #pragma clang fp contract(off)
-  // block statistics in LDS (the per-cell updates are LDS atomics; one row per block)
+  // block statistics in LDS (the per-cell updates are LDS atomics; one row per block);
+  // the gamma tables of the temperature solve (32 KB) copied to LDS: the ten-step Newton
+  // loop gathers from them at every step of every RHS evaluation
   __shared__ int bs[NSTAT];
+  __shared__ double gtab[4 * TLEN];
   if (threadIdx.x < NSTAT) bs[threadIdx.x] = 0;
+  for (int e = threadIdx.x; e < 4 * TLEN; e += blockDim.x) gtab[e] = cc.tab[G_GH2_1 * TLEN + e];
   __syncthreads();
+  const GammaTabs gt = gamma_tabs(gtab);
   const int lane = threadIdx.x & 63;
   for (;;) {
     int base = 0;
@@ -376,7 +388,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
       const CellIn c = load_cell(p, i);
       double *F = work + (long)STAGE_WORDS * base + lo;     // F_j[s] at (j*10 + s)*64
       double f[NSP];
-      cell_rhs(cc, c, Ts, dTs, z, o.rhs_scale, f);
+      cell_rhs(cc, gt, c, Ts, dTs, z, o.rhs_scale, f);
       if (setup) {
         cell_matrix_regs(cc, c, Ts, dTs, z, hg, o.rhs_scale, lu);
         pv = lu_factor_regs(lu);

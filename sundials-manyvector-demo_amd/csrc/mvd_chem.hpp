@@ -54,6 +54,8 @@ int chem_rhs(ChemState &s, const double *y, double *ydot, hipStream_t st, bool s
 int chem_jac(ChemState &s, const double *y, double *J, hipStream_t st, bool sync);
 int chem_vjp(ChemState &s, const double *y, const double *v, double *ybar, double *ydot,
              hipStream_t st, bool sync);
+int chem_temperature_check(ChemState &s, const double *y, const double *Ts0, long *bad,
+                           hipStream_t st);
 double *chem_state_ptr(ChemState &s, int which);
 void chem_free(ChemState &s);
 
@@ -70,65 +72,135 @@ __device__ inline int chem_bin(const ChemConst &c, double logT) {
   return b;
 }
 
+// The four gamma tables of the temperature solve (gH2_1, dgH2_1, gH2_2, dgH2_2), read
+// from the table blob in global memory or from a copy the caller staged in LDS.
+struct GammaTabs {
+  const double *g1, *dg1, *g2, *dg2;
+};
+
+__device__ inline GammaTabs gamma_tabs(const double *g4) {   // 4 consecutive tables
+  return GammaTabs{g4, g4 + TLEN, g4 + 2 * TLEN, g4 + 3 * TLEN};
+}
+
+__device__ inline GammaTabs gamma_tabs(const ChemConst &c) {
+  return gamma_tabs(c.tab + G_GH2_1 * TLEN);
+}
+
+// Mass density (in hydrogen masses) of the temperature solve
+__device__ inline double temperature_density(const double (&y)[NSP]) {
+#pragma clang fp contract(off)
+  return 2.0 * y[0] + 2.0 * y[1] + 1.0079400000000001 * y[2] + 1.0079400000000001 * y[3] +
+         1.0079400000000001 * y[4] + 4.0026020000000004 * y[5] + 4.0026020000000004 * y[6] +
+         4.0026020000000004 * y[7];
+}
+
+// One Newton step of the temperature solve from the input T (density `dm` = density * mH):
+// returns the next iterate and sets dge_dT at T (raja_primordial_network.cpp:943-990).
+__device__ inline double temperature_step(const ChemConst &c, const GammaTabs &gt,
+                                          const double (&y)[NSP], double dm, double T,
+                                          double &dge_dT) {
+#pragma clang fp contract(off)
+  const double kb = 1.3806504e-16;
+  const double gamma = 5.e0 / 3.e0;
+  const double gm1 = 1.0 / (gamma - 1);
+  const double lT = log(T);
+  const int b = chem_bin(c, lT);
+  const double t1 = c.lb + (double)b * c.dbin;
+  const double t2 = c.lb + (double)(b + 1) * c.dbin;
+  const double Tdef = (lT - t1) / (t2 - t1);
+  const double gH2_2 = gt.g2[b] + Tdef * (gt.g2[b + 1] - gt.g2[b]);
+  const double dgH2_2 = gt.dg2[b] + Tdef * (gt.dg2[b + 1] - gt.dg2[b]);
+  const double gH2_1 = gt.g1[b] + Tdef * (gt.g1[b + 1] - gt.g1[b]);
+  const double dgH2_1 = gt.dg1[b] + Tdef * (gt.dg1[b + 1] - gt.dg1[b]);
+  const double g1 = 1.0 / (gH2_1 - 1.0);
+  const double g2 = 1.0 / (gH2_2 - 1.0);
+  const double nsum = y[0] * g1 + y[1] * g2 + y[2] * gm1 + y[3] * gm1 + y[4] * gm1 +
+                      y[5] * gm1 + y[6] * gm1 + y[7] * gm1 + gm1 * y[8];
+  dge_dT = T * kb * ((-y[0]) * g1 * g1 * dgH2_1 - y[1] * g2 * g2 * dgH2_2) / dm + kb * nsum / dm;
+  const double dge = T * kb * nsum / dm - y[9];
+  return T - dge / dge_dT;
+}
+
+// Longest cycle of the temperature iteration that ends the loop early.
+constexpr int TCYCLE = 4;
+
 // cvklu_calculate_temperature (raja_primordial_network.cpp:908-1011): 10 Newton steps
 // from the warm start Ts, clamp to [1,1e5], dTs_ge = 1/dge_dT.
 //
-// The step map x -> x' (and its dge_dT) depends only on the input x and the cell, so once
-// an iterate repeats bitwise the rest of the ten steps is known: a fixed point (x' == x)
-// keeps x and its dge_dT; a 2-cycle (x' == the previous input) alternates, and the tenth
-// output and ninth dge_dT follow from the parity of the step. Leaving the loop there gives
-// bitwise the values of all ten steps, with fewer evaluations (most warm-started solves
-// settle in 2-4 steps).
-__device__ inline void chem_temperature(const ChemConst &c, const double (&y)[NSP], double &Ts,
-                                        double &dTs_ge) {
+// With x_0 = Ts, x_{k+1} = S(x_k) and d_k = dge_dT(x_k), the result is (x_10, 1/d_9). The
+// step map S and d depend only on the input x and the cell, so once an iterate repeats
+// bitwise, x_m == x_{m-p}, the sequence is periodic with period p from index m-p on and
+// the rest of the ten steps is known: x_10 = x_{m-p + (10-m) mod p} and
+// d_9 = d_{m-p + (9-m) mod p}, both among the last p inputs. Leaving the loop at the first
+// repeat with p <= TCYCLE (the inputs and d of the last TCYCLE steps are kept) gives
+// bitwise the values of all ten steps with fewer evaluations. Fixed points and 2-cycles are
+// the common exits of warm-started solves (2-4 steps); 3- and 4-cycles, which rounding
+// makes near some roots, otherwise run the full ten steps and hold up their whole wave.
+__device__ inline void chem_temperature(const ChemConst &c, const GammaTabs &gt,
+                                        const double (&y)[NSP], double &Ts, double &dTs_ge) {
 #pragma clang fp contract(off)
-  const double kb = 1.3806504e-16, mh = 1.67e-24;
-  const double gamma = 5.e0 / 3.e0;
-  const double gm1 = 1.0 / (gamma - 1);
-  const double density = 2.0 * y[0] + 2.0 * y[1] + 1.0079400000000001 * y[2] +
-                         1.0079400000000001 * y[3] + 1.0079400000000001 * y[4] +
-                         4.0026020000000004 * y[5] + 4.0026020000000004 * y[6] +
-                         4.0026020000000004 * y[7];
-  const double dm = density * mh;
-  double T = Ts, Tnew = T * 1.1, dge_dT = 0.0;
-  const double *g1t = c.tab + G_GH2_1 * TLEN, *dg1t = c.tab + G_DGH2_1 * TLEN;
-  const double *g2t = c.tab + G_GH2_2 * TLEN, *dg2t = c.tab + G_DGH2_2 * TLEN;
-  double x_prev = 0.0, d_prev = 0.0;    // input and dge_dT of the previous step
+  const double dm = temperature_density(y) * 1.67e-24;
+  double Tnew = Ts * 1.1, dge_dT = 0.0;
+  double xh[TCYCLE], dh[TCYCLE];        // x_k, d_k of the last steps, newest first
+#pragma unroll
+  for (int q = 0; q < TCYCLE; q++) xh[q] = dh[q] = 0.0;
   for (int it = 0; it < 10; it++) {
-    T = Ts;
-    const double lT = log(Ts);
-    const int b = chem_bin(c, lT);
-    const double t1 = c.lb + (double)b * c.dbin;
-    const double t2 = c.lb + (double)(b + 1) * c.dbin;
-    const double Tdef = (lT - t1) / (t2 - t1);
-    const double gH2_2 = g2t[b] + Tdef * (g2t[b + 1] - g2t[b]);
-    const double dgH2_2 = dg2t[b] + Tdef * (dg2t[b + 1] - dg2t[b]);
-    const double gH2_1 = g1t[b] + Tdef * (g1t[b + 1] - g1t[b]);
-    const double dgH2_1 = dg1t[b] + Tdef * (dg1t[b + 1] - dg1t[b]);
-    const double g1 = 1.0 / (gH2_1 - 1.0);
-    const double g2 = 1.0 / (gH2_2 - 1.0);
-    const double nsum = y[0] * g1 + y[1] * g2 + y[2] * gm1 + y[3] * gm1 + y[4] * gm1 +
-                        y[5] * gm1 + y[6] * gm1 + y[7] * gm1 + gm1 * y[8];
-    dge_dT = T * kb * ((-y[0]) * g1 * g1 * dgH2_1 - y[1] * g2 * g2 * dgH2_2) / dm + kb * nsum / dm;
-    const double dge = T * kb * nsum / dm - y[9];
-    Tnew = T - dge / dge_dT;
+    const double T = Ts;
+    Tnew = temperature_step(c, gt, y, dm, T, dge_dT);
     Ts = Tnew;
+#pragma unroll
+    for (int q = TCYCLE - 1; q > 0; q--) {
+      xh[q] = xh[q - 1];
+      dh[q] = dh[q - 1];
+    }
+    xh[0] = T;                          // x_it
+    dh[0] = dge_dT;                     // d_it
+    // x_{it+1} == x_{it+1-p}: xh[p-1], which exists for p <= it + 1
     const long long bn = __double_as_longlong(Tnew);
-    if (bn == __double_as_longlong(T)) break;                  // fixed point
-    if (it > 0 && bn == __double_as_longlong(x_prev)) {        // 2-cycle x_{j+1} == x_{j-1}
-      if ((it & 1) == 0) {                                     // j even: x_10 = x_j, d_9 = d_{j-1}
-        Tnew = T;
-        dge_dT = d_prev;
-      }                                                        // j odd: x_10 = x_{j+1}, d_9 = d_j
+    int p = 0;
+#pragma unroll
+    for (int q = TCYCLE; q >= 1; q--)
+      if (q <= it + 1 && bn == __double_as_longlong(xh[q - 1])) p = q;
+    if (p) {
+      // m = it + 1; x_10 = xh[rx], d_9 = dh[rd] (offsets back from step it)
+      const int rx = p - 1 - (9 - it) % p;
+      const int rd = p - 1 - ((8 - it) % p + p) % p;
+      double xv = xh[0], dv = dh[0];
+#pragma unroll
+      for (int q = 1; q < TCYCLE; q++) {
+        if (rx == q) xv = xh[q];
+        if (rd == q) dv = dh[q];
+      }
+      Tnew = xv;
+      dge_dT = dv;
       break;
     }
-    x_prev = T;
-    d_prev = dge_dT;
   }
   Ts = Tnew;
   if (Ts < c.tmin) Ts = c.tmin;
   else if (Ts > c.tmax) Ts = c.tmax;
   dTs_ge = 1.0 / dge_dT;
+}
+
+// The ten steps without the early exit (the self-check of the exit compares the two)
+__device__ inline void chem_temperature_ten(const ChemConst &c, const double (&y)[NSP],
+                                            double &Ts, double &dTs_ge) {
+#pragma clang fp contract(off)
+  const GammaTabs gt = gamma_tabs(c);
+  const double dm = temperature_density(y) * 1.67e-24;
+  double Tnew = Ts * 1.1, dge_dT = 0.0;
+  for (int it = 0; it < 10; it++) {
+    Tnew = temperature_step(c, gt, y, dm, Ts, dge_dT);
+    Ts = Tnew;
+  }
+  if (Ts < c.tmin) Ts = c.tmin;
+  else if (Ts > c.tmax) Ts = c.tmax;
+  dTs_ge = 1.0 / dge_dT;
+}
+
+__device__ inline void chem_temperature(const ChemConst &c, const double (&y)[NSP], double &Ts,
+                                        double &dTs_ge) {
+  chem_temperature(c, gamma_tabs(c), y, Ts, dTs_ge);
 }
 
 // linear-in-log(T) interpolation of every rate table and its T derivative

@@ -99,6 +99,7 @@ def lib():
                            C.POINTER(_P)], C.c_int),
         "mvd_chem_lsolve": ([vp, _P, _P], C.c_int),
         "mvd_chem_vjp": ([vp, C.c_double, _P, _P, _P, _P], C.c_int),
+        "mvd_chem_temperature_check": ([vp, _P, _P, C.POINTER(C.c_long)], C.c_int),
         "mvd_dirk_sdirk3": ([C.POINTER(DirkOpts)], C.c_int),
         "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
                                 C.POINTER(DirkStats)], C.c_int),
@@ -318,6 +319,14 @@ class Context:
     def chem_vjp(self, y, v, ybar, ydot=None, t=0.0):
         _check("mvd_chem_vjp", lib().mvd_chem_vjp(
             self.h, t, dptr(y), dptr(v), dptr(ybar), dptr(ydot) if ydot is not None else _P()))
+
+    def temperature_check(self, y, Ts0):
+        """Cells whose early-exit device temperature solve differs bitwise from the ten
+        Newton steps, from the warm starts Ts0 (device arrays)."""
+        bad = C.c_long()
+        _check("mvd_chem_temperature_check", lib().mvd_chem_temperature_check(
+            self.h, dptr(y), dptr(Ts0), C.byref(bad)))
+        return bad.value
 
     def chem_state(self, which, torch_mod):
         """Copy of a per-cell state array (see mvd_chem_state) as a torch device tensor."""

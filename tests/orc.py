@@ -177,7 +177,8 @@ def lib():
         L.orc_chem_dirk_step.restype = C.c_int
         L.orc_lu.argtypes = [P, P, C.POINTER(C.c_int), P, P]
         L.orc_lu.restype = C.c_int
-        L.orc_temperature_exit_check.argtypes = [C.c_void_p, P, P, C.POINTER(C.c_int)]
+        L.orc_temperature_exit_check.argtypes = [C.c_void_p, P, P, C.c_int, C.POINTER(C.c_int),
+                                                     C.POINTER(C.c_int)]
         L.orc_temperature_exit_check.restype = C.c_long
         L.orc_face_flux_vjp.argtypes = [P, C.c_int, C.c_int, C.c_double, P, P]
         L.orc_fe_vjp.argtypes = [C.POINTER(Grid), PP, PP, PP]
@@ -314,13 +315,16 @@ class Chem:
                                    _p(y), _p(ydot))
         return ydot
 
-    def temperature_exit_check(self, y, Ts0):
-        """(cells whose early-exit temperature differs bitwise from ten steps, steps taken)"""
+    def temperature_exit_check(self, y, Ts0, max_period=4):
+        """(cells whose early-exit temperature differs bitwise from ten steps, steps taken,
+        period of the repeat that ended each solve (0: none))"""
         steps = np.zeros(self.n, dtype=np.int32)
+        period = np.zeros(self.n, dtype=np.int32)
+        ip = C.POINTER(C.c_int)
         bad = lib().orc_temperature_exit_check(
-            self.h, _p(y), _p(np.ascontiguousarray(Ts0, dtype=np.float64)),
-            steps.ctypes.data_as(C.POINTER(C.c_int)))
-        return bad, steps
+            self.h, _p(y), _p(np.ascontiguousarray(Ts0, dtype=np.float64)), int(max_period),
+            steps.ctypes.data_as(ip), period.ctypes.data_as(ip))
+        return bad, steps, period
 
     def vjp(self, y, v, fixed_T=False):
         """(d rhs / d y)^T v in the scaled variables (temperature solved and stored)."""

@@ -65,6 +65,33 @@ def test_chem_prepare_rhs_jac(mvd, torch_gpu, n, seed):
     ctx.close()
 
 
+def test_temperature_early_exit_on_device(mvd, torch_gpu):
+    """The device temperature solve ends its ten Newton steps at the first bitwise repeat of
+    the iteration (period <= 4); the self-check entry point runs that form and the full ten
+    steps side by side on the device and must find no cell whose (T, dTs_ge) differ, from
+    cold, true, converged and perturbed warm starts. The oracle's restatement of the exit
+    shows that these inputs hold repeats of every period up to 4."""
+    torch = torch_gpu
+    n = 4096
+    print(f"[temperature exit] n={n}", flush=True)
+    y, T_true = problems.chem_cells(n, seed=21)
+    ctx, yd = gpu_chem(mvd, torch, n, y)
+    rd = torch.empty_like(yd)
+    ctx.chem_rhs(yd, rd)
+    Tw = ctx.chem_state("Ts", torch).cpu().numpy()
+    oc = orc.Chem(n)
+    yo = y.copy()
+    oc.prepare(yo)
+    periods = np.zeros(5, dtype=np.int64)
+    for Ts0 in (np.full(n, 1000.0), T_true, Tw, Tw * (1 + 1e-12), Tw * (1 - 1e-9), Tw * 1.01):
+        T0 = torch.from_numpy(np.ascontiguousarray(Ts0)).to("cuda")
+        assert ctx.temperature_check(yd, T0) == 0
+        _, _, per = oc.temperature_exit_check(yo, np.ascontiguousarray(Ts0), 4)
+        periods += np.bincount(per, minlength=5)
+    assert np.all(periods[1:] > 0), periods
+    ctx.close()
+
+
 def test_chem_scaling_roundtrip(mvd, torch_gpu):
     torch = torch_gpu
     n = 512

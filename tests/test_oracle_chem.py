@@ -216,10 +216,12 @@ def test_stateful_warm_start():
 
 
 def test_temperature_early_exit_is_exact():
-    """The device temperature solve leaves its ten Newton steps at a bitwise fixed point or
-    2-cycle of the step map. Restated in the oracle, that early form must give bitwise the
-    (T, dT/dge) of the ten steps, from cold starts, converged warm starts (where 2-cycles
-    appear) and slightly perturbed warm starts."""
+    """The device temperature solve leaves its ten Newton steps at the first bitwise repeat
+    x_m == x_{m-p} of the iteration with period p <= 4 (fixed points, 2-, 3- and 4-cycles).
+    Restated in the oracle, that early form must give bitwise the (T, dT/dge) of the ten
+    steps, from cold starts, converged warm starts and slightly perturbed warm starts. The
+    data hold every period up to 4, so a wrong member of a cycle (an off-by-one in the
+    index arithmetic) changes T or dT/dge and shows up here."""
     n = 4096
     y, T_true = problems.chem_cells(n, seed=21)
     ch = orc.Chem(n)
@@ -232,8 +234,12 @@ def test_temperature_early_exit_is_exact():
     starts += [Tw, Tw * (1 + 1e-12), Tw * (1 - 1e-9), Tw * 1.01]
     mean_steps = []
     for Ts0 in starts:
-        bad, steps = ch.temperature_exit_check(ys, Ts0)
-        assert bad == 0
+        for P in (1, 2, 3, 4, 8):
+            bad, steps, period = ch.temperature_exit_check(ys, Ts0, P)
+            assert bad == 0, P
+            assert period.max() <= P
+        counts = np.bincount(period, minlength=5)
+        assert np.all(counts[1:5] > 0), counts          # every period 1..4 occurs
         mean_steps.append(steps.mean())
     # converged warm starts settle within a few steps
     assert mean_steps[2] < 4.0, mean_steps
