@@ -328,6 +328,14 @@ __device__ inline void lu_solve_regs(const double (&a)[NSP][NSP], unsigned long 
 
 // Stage derivatives of a 64-cell chunk, element-major (F_j[s] of lane l at (j*10+s)*64 + l).
 constexpr int STAGE_WORDS = (MAXSTAGE - 1) * NSP;
+// Per-cell constants a wave keeps in LDS: scale, inv_scale, mdensity, inv_mdensity and the
+// two optical-depth factors.
+constexpr int CELL_WORDS = 2 * NSP + 4;
+
+// dynamic LDS of the batched kernel for `wpb` waves per block
+inline size_t dirk_lds_bytes(int wpb) {
+  return sizeof(double) * ((size_t)4 * TLEN + (size_t)wpb * CELL_WORDS * 64);
+}
 
 // FORCED: y' = s_rhs f(y) + force, force constant over the step ([cell][10], the forcing an
 // MRI method hands its fast stepper, multirate_chem_hydro_main.cpp:922-946).
@@ -341,32 +349,46 @@ constexpr int STAGE_WORDS = (MAXSTAGE - 1) * NSP;
 template <bool FORCED, bool SUBSTEP>
 __global__ void __launch_bounds__(256)
 chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double *y,
-                  const double *force, double *work, int *queue, int *rows) {
+                  const double *force, const int *order, double *work, int *queue, int *rows) {
 #pragma clang fp contract(off)
-  // block statistics in LDS (the per-cell updates are LDS atomics; one row per block);
-  // the gamma tables of the temperature solve (32 KB) copied to LDS: the ten-step Newton
-  // loop gathers from them at every step of every RHS evaluation
+  // block statistics in LDS (the per-cell updates are LDS atomics; one row per block).
+  // Dynamic LDS: the gamma tables of the temperature solve (32 KB; the ten-step Newton loop
+  // gathers from them at every step of every RHS evaluation), then per wave the constants
+  // of its 64 cells, element-major (scale, inv_scale, then the four auxiliary values):
+  // read at every RHS evaluation, they are loaded from HBM once per cell.
   __shared__ int bs[NSTAT];
-  __shared__ double gtab[4 * TLEN];
+  extern __shared__ double lds[];
   if (threadIdx.x < NSTAT) bs[threadIdx.x] = 0;
-  for (int e = threadIdx.x; e < 4 * TLEN; e += blockDim.x) gtab[e] = cc.tab[G_GH2_1 * TLEN + e];
+  for (int e = threadIdx.x; e < 4 * TLEN; e += blockDim.x) lds[e] = cc.tab[G_GH2_1 * TLEN + e];
   __syncthreads();
-  const GammaTabs gt = gamma_tabs(gtab);
+  const GammaTabs gt = gamma_tabs(lds);
   const int lane = threadIdx.x & 63;
+  double *const cst = lds + 4 * TLEN + (threadIdx.x >> 6) * (CELL_WORDS * 64);
   for (;;) {
     int base = 0;
     if (lane == 0) base = atomicAdd(queue, 64);
     base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
     if ((long)base >= n) break;
     if ((long)base + lane >= n) continue;
+    // the lane's cell: queue positions run over the cells in temperature order
+    const int cell = order ? order[base + lane] : base + lane;
     // per-lane state for the whole step: iterate, stage right-hand side, LU factors
     double Ts, dTs, z[NSP], r[NSP], lu[NSP][NSP];
     {
-      const long i = (long)base + lane;
+      const long i = cell;
       Ts = p.Ts[i];
       dTs = p.dTs[i];
 #pragma unroll
       for (int s = 0; s < NSP; s++) { z[s] = y[NSP * i + s]; r[s] = z[s] + h * 0.0; }
+#pragma unroll
+      for (int s = 0; s < NSP; s++) {
+        cst[s * 64 + lane] = p.scale[NSP * i + s];
+        cst[(NSP + s) * 64 + lane] = p.iscale[NSP * i + s];
+      }
+      cst[(2 * NSP) * 64 + lane] = p.md[i];
+      cst[(2 * NSP + 1) * 64 + lane] = p.imd[i];
+      cst[(2 * NSP + 2) * 64 + lane] = p.cie[i];
+      cst[(2 * NSP + 3) * 64 + lane] = p.h2o[i];
     }
     // One loop, one RHS evaluation per pass; the sequence of evaluations (and so of the
     // temperature warm starts) is that of the stage/Newton/re-setup loops:
@@ -380,12 +402,17 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     int halv = 0, done = 0;                                // substep level, units done
     unsigned long long pv = 0;
     for (;;) {
-      // The lane offset is made opaque on every pass so that the per-lane addresses of the
-      // cell's inputs are formed where they are used instead of being held in registers.
+      // The lane offset and cell index are made opaque on every pass so that the per-lane
+      // addresses of the cell's inputs are formed where they are used instead of being held
+      // in registers.
       long lo = lane;
-      __asm__ volatile("" : "+v"(lo));
-      const long i = (long)base + lo;
-      const CellIn c = load_cell(p, i);
+      int ci = cell;
+      __asm__ volatile("" : "+v"(lo), "+v"(ci));
+      const long i = ci;
+      const double *cl = cst + lo;
+      const CellIn c{cl, cl + NSP * 64, 64,
+                     CellAux{cl[(2 * NSP) * 64], cl[(2 * NSP + 1) * 64], cl[(2 * NSP + 2) * 64],
+                             cl[(2 * NSP + 3) * 64]}};
       double *F = work + (long)STAGE_WORDS * base + lo;     // F_j[s] at (j*10 + s)*64
       double f[NSP];
       cell_rhs(cc, gt, c, Ts, dTs, z, o.rhs_scale, f);
@@ -467,7 +494,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
       }
     }
     {
-      const long i = (long)base + lane;
+      const long i = cell;
 #pragma unroll
       for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
       p.Ts[i] = Ts;
@@ -650,6 +677,80 @@ static int ensure_work(ChemState &s, size_t bytes) {
 
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// ------------------------------------------------------------------------------------
+// Temperature order of the batched step: a counting sort of the cells by the table bin of
+// their temperature, in groups of 8 bins, so that the 64 lanes of a wave gather from the
+// same few lines of every rate table (and meet similar stiffness). The queue then hands
+// out positions of this order. Per-cell results do not depend on the order. The key
+// temperature is one step of the temperature iteration from the warm start (the warm
+// start itself may be a common guess such as the 1000 K of a fresh preparation).
+// ------------------------------------------------------------------------------------
+constexpr int TORD_NBK = 128;          // keys: table bin (0..1022) / 8
+constexpr int TORD_CELLS = 4096;       // cells per counting block
+constexpr long TORD_MIN = 4096;        // smaller batches keep the natural order
+
+// per-block bucket counts, bucket-major: hist[k * nblk + b]; the cells' keys to key[]
+__global__ void __launch_bounds__(256)
+tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, unsigned char *key, int *hist) {
+#pragma clang fp contract(off)
+  __shared__ int hc[TORD_NBK];
+  for (int t = threadIdx.x; t < TORD_NBK; t += blockDim.x) hc[t] = 0;
+  __syncthreads();
+  const long c0 = (long)blockIdx.x * TORD_CELLS;
+  for (int t = threadIdx.x; t < TORD_CELLS; t += blockDim.x) {
+    const long i = c0 + t;
+    if (i >= n) break;
+    double ya[NSP], d;
+#pragma unroll
+    for (int s = 0; s < NSP; s++) ya[s] = y[NSP * i + s] * p.scale[NSP * i + s];
+    const double T1 =
+        temperature_step(cc, gamma_tabs(cc), ya, temperature_density(ya) * 1.67e-24, p.Ts[i], d);
+    const int k = chem_bin(cc, log(T1 > cc.tmin ? (T1 < cc.tmax ? T1 : cc.tmax) : cc.tmin)) >> 3;
+    key[i] = (unsigned char)k;
+    atomicAdd(&hc[k], 1);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < TORD_NBK; t += blockDim.x)
+    hist[(long)t * gridDim.x + blockIdx.x] = hc[t];
+}
+
+// exclusive prefix sum of a[0..m) in place, one block of 1024 threads
+__global__ void __launch_bounds__(1024)
+tord_scan(long m, int *a) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const long per = (m + 1023) / 1024;
+  const long e0 = (long)t * per, e1 = e0 + per < m ? e0 + per : m;
+  int sum = 0;
+  for (long e = e0; e < e1; e++) sum += a[e];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t ? part[t - 1] : 0;
+  for (long e = e0; e < e1; e++) {
+    const int v = a[e];
+    a[e] = run;
+    run += v;
+  }
+}
+
+// order[start of (bucket, block) + rank] = cell; the same keys as tord_count
+__global__ void __launch_bounds__(256)
+tord_scatter(long n, const unsigned char *key, const int *off, int *order) {
+  __shared__ int pos[TORD_NBK];
+  for (int t = threadIdx.x; t < TORD_NBK; t += blockDim.x)
+    pos[t] = off[(long)t * gridDim.x + blockIdx.x];
+  __syncthreads();
+  const long c0 = (long)blockIdx.x * TORD_CELLS;
+  for (int t = threadIdx.x; t < TORD_CELLS; t += blockDim.x)
+    if (c0 + t < n) order[atomicAdd(&pos[key[c0 + t]], 1)] = (int)(c0 + t);
+}
+
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
                         int grid_waves, int block, const double *force) {
   const long n = s.n;
@@ -661,26 +762,47 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   if (waves > (long)chunks) waves = (long)chunks;
   const long grid = (waves + wpb - 1) / wpb;
   const long nrow = grid;                                  // one statistics row per block
-  // work: stage-derivative blocks | per-block statistics rows
+  // work: stage-derivative blocks | per-block statistics rows | cell order | bucket
+  // counts | keys
+  const bool sorted = n >= TORD_MIN;
+  const long nblk = (n + TORD_CELLS - 1) / TORD_CELLS;
   const size_t b_blk = align256(chunks * 64 * STAGE_WORDS * sizeof(double));
   const size_t b_rows = align256((size_t)nrow * NSTAT * sizeof(int));
-  if (ensure_work(s, b_blk + b_rows)) return -1;
+  const size_t b_ord = sorted ? align256((size_t)n * sizeof(int)) : 0;
+  const size_t b_hist = sorted ? align256((size_t)TORD_NBK * nblk * sizeof(int)) : 0;
+  const size_t b_key = sorted ? align256((size_t)n) : 0;
+  if (ensure_work(s, b_blk + b_rows + b_ord + b_hist + b_key)) return -1;
   char *w = (char *)s.work;
   int *rows = (int *)(w + b_blk);
+  int *order = sorted ? (int *)(w + b_blk + b_rows) : nullptr;
+  if (sorted) {
+    int *hist = (int *)(w + b_blk + b_rows + b_ord);
+    unsigned char *key = (unsigned char *)(w + b_blk + b_rows + b_ord + b_hist);
+    hipLaunchKernelGGL(tord_count, dim3((unsigned)nblk), dim3(256), 0, st, n, s.cc, ptrs_d(s),
+                       (const double *)y, key, hist);
+    hipLaunchKernelGGL(tord_scan, dim3(1), dim3(1024), 0, st, (long)TORD_NBK * nblk, hist);
+    hipLaunchKernelGGL(tord_scatter, dim3((unsigned)nblk), dim3(256), 0, st, n,
+                       (const unsigned char *)key, (const int *)hist, order);
+  }
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
   const bool sub = o.max_halvings > 0;
+  const unsigned lds = (unsigned)dirk_lds_bytes(wpb);
   if (force && sub)
-    hipLaunchKernelGGL((chem_dirk_batched<true, true>), dim3((unsigned)grid), dim3(block), 0, st,
-                       n, s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
+    hipLaunchKernelGGL((chem_dirk_batched<true, true>), dim3((unsigned)grid), dim3(block), lds, st,
+                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, (double *)w, s.counters,
+                       rows);
   else if (force)
-    hipLaunchKernelGGL((chem_dirk_batched<true, false>), dim3((unsigned)grid), dim3(block), 0, st,
-                       n, s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
+    hipLaunchKernelGGL((chem_dirk_batched<true, false>), dim3((unsigned)grid), dim3(block), lds, st,
+                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, (double *)w, s.counters,
+                       rows);
   else if (sub)
-    hipLaunchKernelGGL((chem_dirk_batched<false, true>), dim3((unsigned)grid), dim3(block), 0, st,
-                       n, s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
+    hipLaunchKernelGGL((chem_dirk_batched<false, true>), dim3((unsigned)grid), dim3(block), lds, st,
+                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, (double *)w, s.counters,
+                       rows);
   else
-    hipLaunchKernelGGL((chem_dirk_batched<false, false>), dim3((unsigned)grid), dim3(block), 0,
-                       st, n, s.cc, ptrs_d(s), o, h, y, force, (double *)w, s.counters, rows);
+    hipLaunchKernelGGL((chem_dirk_batched<false, false>), dim3((unsigned)grid), dim3(block), lds,
+                       st, n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, (double *)w, s.counters,
+                       rows);
   hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(nrow)), dim3(256), 0, st,
                      (const int *)rows, nrow, s.counters + 1);
   return hipGetLastError() == hipSuccess ? 0 : -1;
