@@ -328,13 +328,18 @@ __device__ inline void lu_solve_regs(const double (&a)[NSP][NSP], unsigned long 
 
 // Stage derivatives of a 64-cell chunk, element-major (F_j[s] of lane l at (j*10+s)*64 + l).
 constexpr int STAGE_WORDS = (MAXSTAGE - 1) * NSP;
-// Per-cell constants a wave keeps in LDS: scale, inv_scale, mdensity, inv_mdensity and the
-// two optical-depth factors.
+// Per-cell values a wave keeps in LDS, element-major ([word][64 lanes]): scale, inv_scale,
+// mdensity, inv_mdensity, the two optical-depth factors, the stage derivatives F_j of the
+// first nstage - 1 stages and the (sub)step start y_n.
 constexpr int CELL_WORDS = 2 * NSP + 4;
+__host__ __device__ inline int dirk_wave_words(int nstage) {
+  return CELL_WORDS + (nstage - 1) * NSP + NSP;
+}
+constexpr size_t LDS_LIMIT = 160 * 1024;   // LDS of a gfx950 CU, all of it one block's
 
 // dynamic LDS of the batched kernel for `wpb` waves per block
-inline size_t dirk_lds_bytes(int wpb) {
-  return sizeof(double) * ((size_t)4 * TLEN + (size_t)wpb * CELL_WORDS * 64);
+inline size_t dirk_lds_bytes(int wpb, int nstage) {
+  return sizeof(double) * ((size_t)4 * TLEN + (size_t)wpb * dirk_wave_words(nstage) * 64);
 }
 
 // FORCED: y' = s_rhs f(y) + force, force constant over the step ([cell][10], the forcing an
@@ -349,13 +354,13 @@ inline size_t dirk_lds_bytes(int wpb) {
 template <bool FORCED, bool SUBSTEP>
 __global__ void __launch_bounds__(256)
 chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double *y,
-                  const double *force, const int *order, double *work, int *queue, int *rows) {
+                  const double *force, const int *order, int *queue, int *rows) {
 #pragma clang fp contract(off)
   // block statistics in LDS (the per-cell updates are LDS atomics; one row per block).
   // Dynamic LDS: the gamma tables of the temperature solve (32 KB; the ten-step Newton loop
-  // gathers from them at every step of every RHS evaluation), then per wave the constants
-  // of its 64 cells, element-major (scale, inv_scale, then the four auxiliary values):
-  // read at every RHS evaluation, they are loaded from HBM once per cell.
+  // gathers from them at every step of every RHS evaluation), then per wave the values of
+  // its 64 cells (WAVE_WORDS): the constants read at every RHS evaluation, loaded from HBM
+  // once per cell, and the stage derivatives and step start read at stage changes.
   __shared__ int bs[NSTAT];
   extern __shared__ double lds[];
   if (threadIdx.x < NSTAT) bs[threadIdx.x] = 0;
@@ -363,12 +368,21 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
   __syncthreads();
   const GammaTabs gt = gamma_tabs(lds);
   const int lane = threadIdx.x & 63;
-  double *const cst = lds + 4 * TLEN + (threadIdx.x >> 6) * (CELL_WORDS * 64);
+  double *const cst = lds + 4 * TLEN + (threadIdx.x >> 6) * (dirk_wave_words(o.nstage) * 64);
+  double *const Fst = cst + CELL_WORDS * 64 + lane;        // F_j[s] at (j*10 + s)*64
+  double *const yst = Fst + (o.nstage - 1) * NSP * 64;     // y_n[s] at s*64
+  // The first chunk of each wave is its global wave index; later ones come from the queue
+  // (which starts after them), each fetched while the previous chunk is being solved.
+  const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
+  int next = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
   for (;;) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(queue, 64);
-    base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+    const int base = next;
     if ((long)base >= n) break;
+    {
+      int q = 0;
+      if (lane == 0) q = atomicAdd(queue, 64) + nwaves * 64;
+      next = __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));
+    }
     if ((long)base + lane >= n) continue;
     // the lane's cell: queue positions run over the cells in temperature order
     const int cell = order ? order[base + lane] : base + lane;
@@ -382,6 +396,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
       for (int s = 0; s < NSP; s++) { z[s] = y[NSP * i + s]; r[s] = z[s] + h * 0.0; }
 #pragma unroll
       for (int s = 0; s < NSP; s++) {
+        yst[s * 64] = z[s];
         cst[s * 64 + lane] = p.scale[NSP * i + s];
         cst[(NSP + s) * 64 + lane] = p.iscale[NSP * i + s];
       }
@@ -413,7 +428,6 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
       const CellIn c{cl, cl + NSP * 64, 64,
                      CellAux{cl[(2 * NSP) * 64], cl[(2 * NSP + 1) * 64], cl[(2 * NSP + 2) * 64],
                              cl[(2 * NSP + 3) * 64]}};
-      double *F = work + (long)STAGE_WORDS * base + lo;     // F_j[s] at (j*10 + s)*64
       double f[NSP];
       cell_rhs(cc, gt, c, Ts, dTs, z, o.rhs_scale, f);
       if (setup) {
@@ -444,7 +458,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
           if (done >= (1 << o.max_halvings)) break;
           // substep accepted: it becomes the start of the next one
 #pragma unroll
-          for (int s = 0; s < NSP; s++) { y[NSP * i + s] = z[s]; r[s] = z[s] + hs * 0.0; }
+          for (int s = 0; s < NSP; s++) { yst[s * 64] = z[s]; r[s] = z[s] + hs * 0.0; }
           Ts0 = Ts;
           dTs0 = dTs;
           st = 0;
@@ -455,23 +469,23 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
         }
         // F_{st-1} = (z - r)/hg (ARKODE's implicit-stage derivative); the next stage's r
 #pragma unroll
-        for (int s = 0; s < NSP; s++) F[((st - 1) * NSP + s) * 64] = (z[s] - r[s]) / hg;
+        for (int s = 0; s < NSP; s++) Fst[((st - 1) * NSP + s) * 64] = (z[s] - r[s]) / hg;
 #pragma unroll
         for (int s = 0; s < NSP; s++) {
           double a = 0.0;
 #pragma unroll
           for (int j = 0; j < MAXSTAGE - 1; j++)
-            if (j < st) a += dirk_coef(o, st, j) * F[(j * NSP + s) * 64];
-          r[s] = y[NSP * i + s] + hs * a;                  // y holds the (sub)step start
+            if (j < st) a += dirk_coef(o, st, j) * Fst[(j * NSP + s) * 64];
+          r[s] = yst[s * 64] + hs * a;                     // the (sub)step start
         }
         it = 0;
         setups = 0;
       } else if (it == o.max_iters) {
         if (++setups > o.max_setups) {
           if (SUBSTEP) {
-            // back to the substep start (y holds it)
+            // back to the substep start
 #pragma unroll
-            for (int s = 0; s < NSP; s++) z[s] = y[NSP * i + s];
+            for (int s = 0; s < NSP; s++) z[s] = yst[s * 64];
             Ts = Ts0;
             dTs = dTs0;
             if (halv < o.max_halvings) {
@@ -756,28 +770,30 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   const long n = s.n;
   if (n <= 0 || n > (1L << 30)) return -1;
   if (block != 64 && block != 128 && block != 256) return -1;
+  if (o.nstage < 1 || o.nstage > MAXSTAGE) return -1;
+  // blocks whose LDS would not fit one CU are split (fewer waves share the gamma tables)
+  while (block > 64 && dirk_lds_bytes(block / 64, o.nstage) + NSTAT * sizeof(int) > LDS_LIMIT)
+    block /= 2;
   const int wpb = block / 64;
   const size_t chunks = (size_t)(n + 63) / 64;
   long waves = grid_waves > 0 ? grid_waves : 1;
   if (waves > (long)chunks) waves = (long)chunks;
   const long grid = (waves + wpb - 1) / wpb;
   const long nrow = grid;                                  // one statistics row per block
-  // work: stage-derivative blocks | per-block statistics rows | cell order | bucket
-  // counts | keys
+  // work: per-block statistics rows | cell order | bucket counts | keys
   const bool sorted = n >= TORD_MIN;
   const long nblk = (n + TORD_CELLS - 1) / TORD_CELLS;
-  const size_t b_blk = align256(chunks * 64 * STAGE_WORDS * sizeof(double));
   const size_t b_rows = align256((size_t)nrow * NSTAT * sizeof(int));
   const size_t b_ord = sorted ? align256((size_t)n * sizeof(int)) : 0;
   const size_t b_hist = sorted ? align256((size_t)TORD_NBK * nblk * sizeof(int)) : 0;
   const size_t b_key = sorted ? align256((size_t)n) : 0;
-  if (ensure_work(s, b_blk + b_rows + b_ord + b_hist + b_key)) return -1;
+  if (ensure_work(s, b_rows + b_ord + b_hist + b_key)) return -1;
   char *w = (char *)s.work;
-  int *rows = (int *)(w + b_blk);
-  int *order = sorted ? (int *)(w + b_blk + b_rows) : nullptr;
+  int *rows = (int *)(w);
+  int *order = sorted ? (int *)(w + b_rows) : nullptr;
   if (sorted) {
-    int *hist = (int *)(w + b_blk + b_rows + b_ord);
-    unsigned char *key = (unsigned char *)(w + b_blk + b_rows + b_ord + b_hist);
+    int *hist = (int *)(w + b_rows + b_ord);
+    unsigned char *key = (unsigned char *)(w + b_rows + b_ord + b_hist);
     hipLaunchKernelGGL(tord_count, dim3((unsigned)nblk), dim3(256), 0, st, n, s.cc, ptrs_d(s),
                        (const double *)y, key, hist);
     hipLaunchKernelGGL(tord_scan, dim3(1), dim3(1024), 0, st, (long)TORD_NBK * nblk, hist);
@@ -786,23 +802,19 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   }
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
   const bool sub = o.max_halvings > 0;
-  const unsigned lds = (unsigned)dirk_lds_bytes(wpb);
+  const unsigned lds = (unsigned)dirk_lds_bytes(wpb, o.nstage);
   if (force && sub)
     hipLaunchKernelGGL((chem_dirk_batched<true, true>), dim3((unsigned)grid), dim3(block), lds, st,
-                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, (double *)w, s.counters,
-                       rows);
+                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, s.counters, rows);
   else if (force)
     hipLaunchKernelGGL((chem_dirk_batched<true, false>), dim3((unsigned)grid), dim3(block), lds, st,
-                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, (double *)w, s.counters,
-                       rows);
+                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, s.counters, rows);
   else if (sub)
     hipLaunchKernelGGL((chem_dirk_batched<false, true>), dim3((unsigned)grid), dim3(block), lds, st,
-                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, (double *)w, s.counters,
-                       rows);
+                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, s.counters, rows);
   else
     hipLaunchKernelGGL((chem_dirk_batched<false, false>), dim3((unsigned)grid), dim3(block), lds,
-                       st, n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, (double *)w, s.counters,
-                       rows);
+                       st, n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, s.counters, rows);
   hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(nrow)), dim3(256), 0, st,
                      (const int *)rows, nrow, s.counters + 1);
   return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -72,6 +72,37 @@ def test_dirk_batched_and_plain_match_oracle(mvd, torch_gpu, n, seed, h):
     assert st_b["max_cell_iters"] == st_p["max_cell_iters"]
 
 
+def test_dirk_four_stage_table(mvd, torch_gpu):
+    """A 4-stage SDIRK table (constant diagonal 1/4, stiffly accurate): the batched kernel
+    keeps three stage derivatives per cell in LDS, which for 256-thread blocks would exceed
+    a CU's LDS, so the launch runs 128-thread blocks. Batched, plain and oracle agree as
+    for the default table."""
+    torch = torch_gpu
+    n = 8192
+    print(f"[dirk 4-stage] n={n}", flush=True)
+    A = np.array([[0.25, 0, 0, 0], [0.25, 0.25, 0, 0], [0.2, 0.3, 0.25, 0],
+                  [0.1, 0.3, 0.35, 0.25]])
+    c = A.sum(axis=1)
+    oc, yo, ctx, yd = setup(mvd, torch, n, 7)
+    _, _, ctx_p, y_plain = setup(mvd, torch, n, 7)
+    failed_o, iters_o = orc.dirk_step(oc, orc.dirk_opts(A=A, c=c, nstage=4), 1e3, yo)
+    ok = iters_o >= 0
+    o = mvd.sdirk3()
+    o.nstage = 4
+    for k, v in enumerate(A.reshape(-1)):
+        o.A[k] = v
+    for k, v in enumerate(c):
+        o.c[k] = v
+    failed_b, st_b = ctx.dirk_step(yd, 1e3, o)
+    failed_p, st_p = ctx_p.dirk_step(y_plain, 1e3, o, plain=True)
+    assert failed_b == failed_o == failed_p
+    assert compare(yd.cpu().numpy(), yo, ok) <= TOL
+    assert np.array_equal(yd.cpu().numpy(), y_plain.cpu().numpy())
+    assert st_b["hist"] == st_p["hist"]
+    ctx.close()
+    ctx_p.close()
+
+
 def test_dirk_reports_newton_failure(mvd, torch_gpu):
     torch = torch_gpu
     n = 777
