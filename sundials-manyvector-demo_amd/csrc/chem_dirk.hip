@@ -372,18 +372,16 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
   double *const Fst = cst + CELL_WORDS * 64 + lane;        // F_j[s] at (j*10 + s)*64
   double *const yst = Fst + (o.nstage - 1) * NSP * 64;     // y_n[s] at s*64
   // The first chunk of each wave is its global wave index; later ones come from the queue
-  // (which starts after them), each fetched while the previous chunk is being solved.
+  // (which starts after them), each fetched while the previous chunk is being solved: lane
+  // 0 issues the atomic at the start of a chunk and the wave reads its result at the end.
   const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
   int next = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
   for (;;) {
     const int base = next;
     if ((long)base >= n) break;
-    {
-      int q = 0;
-      if (lane == 0) q = atomicAdd(queue, 64) + nwaves * 64;
-      next = __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));
-    }
-    if ((long)base + lane >= n) continue;
+    int q = 0;
+    if (lane == 0) q = atomicAdd(queue, 64) + nwaves * 64;
+    if ((long)base + lane < n) {   // lanes past the end idle to the end of the chunk
     // the lane's cell: queue positions run over the cells in temperature order
     const int cell = order ? order[base + lane] : base + lane;
     // per-lane state for the whole step: iterate, stage right-hand side, LU factors
@@ -519,6 +517,8 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     atomicAdd(&bs[STAT_SETUPS], nsetup);
     if (!ok) atomicAdd(&bs[STAT_FAILED], 1);
     atomicAdd(&bs[STAT_HIST + (nit < NHIST - 1 ? nit : NHIST - 1)], 1);
+    }
+    next = __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));
   }
   __syncthreads();
   if (threadIdx.x < NSTAT) rows[(long)blockIdx.x * NSTAT + threadIdx.x] = bs[threadIdx.x];
@@ -700,20 +700,23 @@ static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // start itself may be a common guess such as the 1000 K of a fresh preparation).
 // ------------------------------------------------------------------------------------
 constexpr int TORD_NBK = 128;          // keys: table bin (0..1022) / 8
-constexpr int TORD_CELLS = 4096;       // cells per counting block
 constexpr long TORD_MIN = 4096;        // smaller batches keep the natural order
+// Bucket counters sit 128 bytes apart (one cache line each) so that the blocks' atomics on
+// different buckets do not queue on the same line: total of bucket k at cnt[k * TORD_PAD],
+// its running offset at cnt[(TORD_NBK + k) * TORD_PAD].
+constexpr int TORD_PAD = 32;
 
-// per-block bucket counts, bucket-major: hist[k * nblk + b]; the cells' keys to key[]
-__global__ void __launch_bounds__(256)
-tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, unsigned char *key, int *hist) {
+// Keys (one byte per cell) and the bucket totals, accumulated per block in LDS first. One
+// cell per thread, blocks of TORD_BLOCK threads.
+constexpr int TORD_BLOCK = 1024;
+__global__ void __launch_bounds__(TORD_BLOCK)
+tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, unsigned char *key, int *cnt) {
 #pragma clang fp contract(off)
   __shared__ int hc[TORD_NBK];
-  for (int t = threadIdx.x; t < TORD_NBK; t += blockDim.x) hc[t] = 0;
+  if (threadIdx.x < TORD_NBK) hc[threadIdx.x] = 0;
   __syncthreads();
-  const long c0 = (long)blockIdx.x * TORD_CELLS;
-  for (int t = threadIdx.x; t < TORD_CELLS; t += blockDim.x) {
-    const long i = c0 + t;
-    if (i >= n) break;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
     double ya[NSP], d;
 #pragma unroll
     for (int s = 0; s < NSP; s++) ya[s] = y[NSP * i + s] * p.scale[NSP * i + s];
@@ -724,45 +727,38 @@ tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, unsigned char *ke
     atomicAdd(&hc[k], 1);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < TORD_NBK; t += blockDim.x)
-    hist[(long)t * gridDim.x + blockIdx.x] = hc[t];
+  if (threadIdx.x < TORD_NBK && hc[threadIdx.x])
+    atomicAdd(&cnt[threadIdx.x * TORD_PAD], hc[threadIdx.x]);
 }
 
-// exclusive prefix sum of a[0..m) in place, one block of 1024 threads
-__global__ void __launch_bounds__(1024)
-tord_scan(long m, int *a) {
-  __shared__ int part[1024];
+// order[start of bucket k + reserved range of this block + local rank] = cell. Each block
+// takes the exclusive prefix of the bucket totals, ranks its cells per bucket in LDS and
+// reserves one range per bucket from the running counters (zeroed beforehand).
+__global__ void __launch_bounds__(TORD_BLOCK)
+tord_scatter(long n, const unsigned char *key, int *cnt, int *order) {
+  __shared__ int pre[TORD_NBK], hc[TORD_NBK], res[TORD_NBK];
   const int t = threadIdx.x;
-  const long per = (m + 1023) / 1024;
-  const long e0 = (long)t * per, e1 = e0 + per < m ? e0 + per : m;
-  int sum = 0;
-  for (long e = e0; e < e1; e++) sum += a[e];
-  part[t] = sum;
+  if (t < TORD_NBK) {
+    pre[t] = cnt[t * TORD_PAD];
+    hc[t] = 0;
+  }
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = t >= off ? part[t - off] : 0;
+  for (int off = 1; off < TORD_NBK; off <<= 1) {           // inclusive scan of the totals
+    const int v = (t < TORD_NBK && t >= off) ? pre[t - off] : 0;
     __syncthreads();
-    part[t] += v;
+    if (t < TORD_NBK) pre[t] += v;
     __syncthreads();
   }
-  int run = t ? part[t - 1] : 0;
-  for (long e = e0; e < e1; e++) {
-    const int v = a[e];
-    a[e] = run;
-    run += v;
+  const long i = (long)blockIdx.x * blockDim.x + t;
+  int k = 0, r = 0;
+  if (i < n) {
+    k = key[i];
+    r = atomicAdd(&hc[k], 1);
   }
-}
-
-// order[start of (bucket, block) + rank] = cell; the same keys as tord_count
-__global__ void __launch_bounds__(256)
-tord_scatter(long n, const unsigned char *key, const int *off, int *order) {
-  __shared__ int pos[TORD_NBK];
-  for (int t = threadIdx.x; t < TORD_NBK; t += blockDim.x)
-    pos[t] = off[(long)t * gridDim.x + blockIdx.x];
   __syncthreads();
-  const long c0 = (long)blockIdx.x * TORD_CELLS;
-  for (int t = threadIdx.x; t < TORD_CELLS; t += blockDim.x)
-    if (c0 + t < n) order[atomicAdd(&pos[key[c0 + t]], 1)] = (int)(c0 + t);
+  if (t < TORD_NBK && hc[t]) res[t] = atomicAdd(&cnt[(TORD_NBK + t) * TORD_PAD], hc[t]);
+  __syncthreads();
+  if (i < n) order[(k ? pre[k - 1] : 0) + res[k] + r] = (int)i;
 }
 
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
@@ -780,25 +776,26 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   if (waves > (long)chunks) waves = (long)chunks;
   const long grid = (waves + wpb - 1) / wpb;
   const long nrow = grid;                                  // one statistics row per block
-  // work: per-block statistics rows | cell order | bucket counts | keys
+  // work: per-block statistics rows | cell order | bucket totals and running offsets | keys
   const bool sorted = n >= TORD_MIN;
-  const long nblk = (n + TORD_CELLS - 1) / TORD_CELLS;
   const size_t b_rows = align256((size_t)nrow * NSTAT * sizeof(int));
   const size_t b_ord = sorted ? align256((size_t)n * sizeof(int)) : 0;
-  const size_t b_hist = sorted ? align256((size_t)TORD_NBK * nblk * sizeof(int)) : 0;
+  const size_t b_cnt = sorted ? align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int)) : 0;
   const size_t b_key = sorted ? align256((size_t)n) : 0;
-  if (ensure_work(s, b_rows + b_ord + b_hist + b_key)) return -1;
+  if (ensure_work(s, b_rows + b_ord + b_cnt + b_key)) return -1;
   char *w = (char *)s.work;
   int *rows = (int *)(w);
   int *order = sorted ? (int *)(w + b_rows) : nullptr;
   if (sorted) {
-    int *hist = (int *)(w + b_rows + b_ord);
-    unsigned char *key = (unsigned char *)(w + b_rows + b_ord + b_hist);
-    hipLaunchKernelGGL(tord_count, dim3((unsigned)nblk), dim3(256), 0, st, n, s.cc, ptrs_d(s),
-                       (const double *)y, key, hist);
-    hipLaunchKernelGGL(tord_scan, dim3(1), dim3(1024), 0, st, (long)TORD_NBK * nblk, hist);
-    hipLaunchKernelGGL(tord_scatter, dim3((unsigned)nblk), dim3(256), 0, st, n,
-                       (const unsigned char *)key, (const int *)hist, order);
+    int *cnt = (int *)(w + b_rows + b_ord);
+    unsigned char *key = (unsigned char *)(w + b_rows + b_ord + b_cnt);
+    hipLaunchKernelGGL(fill_int, dim3(2 * TORD_NBK * TORD_PAD / 256), dim3(256), 0, st,
+                       (long)(2 * TORD_NBK * TORD_PAD), cnt, 0);
+    const unsigned tb = (unsigned)((n + TORD_BLOCK - 1) / TORD_BLOCK);
+    hipLaunchKernelGGL(tord_count, dim3(tb), dim3(TORD_BLOCK), 0, st, n, s.cc, ptrs_d(s),
+                       (const double *)y, key, cnt);
+    hipLaunchKernelGGL(tord_scatter, dim3(tb), dim3(TORD_BLOCK), 0, st, n,
+                       (const unsigned char *)key, cnt, order);
   }
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
   const bool sub = o.max_halvings > 0;
