@@ -71,6 +71,21 @@ int mvd_last_state_flags(mvd_context *ctx);
 int mvd_fe_vjp(mvd_context *ctx, double t, const double *const *w, const double *const *v,
                double *const *wbar);
 
+/* Ghost exchange of the single subdomain: the receive buffers of EulerData::ExchangeStart /
+ * ExchangeEnd (euler3D.hpp:577-1191) with the rank as its own periodic neighbour and the
+ * physical-boundary fills of euler3D.hpp:793-1166 (low faces mirror w[2-s], high faces copy
+ * w[n-3+s]; reflecting negates the normal momentum, Dirichlet every value). recv[6] =
+ * W, E, S, N, B, F device buffers (NULL entries are skipped) of 3*na*nb*(5+nchem) doubles
+ * in the reference's layout BUFINDX(v,s,a,b,NVAR,3,na,nb) = v + NVAR*(s + 3*(a + na*b)):
+ * W/E (s,j,k), S/N (s,i,k), B/F (s,i,j); slot s of a low face is the point s-3 outside it,
+ * of a high face the point n+s. */
+int mvd_exchange(mvd_context *ctx, const double *const *w, double *const *recv);
+/* pack1D_{x,y,z}[_bdry] (euler3D.hpp:1197-1378): for each of ncell cells (ijk: 3 ints per
+ * cell, device) the stencil w(c-3..c+2) along dir (0 x, 1 y, 2 z) with the ghost rules of
+ * mvd_exchange, out[cell][6][5+nchem] (device). Cells outside the grid: MVD_BAD_ARGUMENT. */
+int mvd_pack1d(mvd_context *ctx, const double *const *w, int dir, long ncell, const int *ijk,
+               double *out);
+
 /* stability (utilities.cpp:483-528; ARKStep stability callback, euler3D_main.cpp:272-275):
  * alpha = max over cells of |mx/rho| + c, dt_stab = cfl * min(dx,dy,dz) / alpha. */
 int mvd_max_wave_speed(mvd_context *ctx, const double *const *w, double *alpha);

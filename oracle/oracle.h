@@ -45,6 +45,8 @@ typedef struct {
 } orc_erk;
 int orc_erk_evolve(const orc_grid *g, const orc_erk *tab, double h, int nsteps, double *const *w);
 int orc_lu(const double *M, double *LU, int *piv, const double *b, double *x);
+/* single-rank ghost exchange buffers W, E, S, N, B, F (orc_fe.cpp) */
+void orc_exchange(const orc_grid *g, const double *const *w, double *const *recv);
 /* bitwise check of the early exit of the device temperature solve (orc_chem.cpp) */
 long orc_temperature_exit_check(void *chem, const double *y, const double *Ts0, int max_period,
                                 int *steps, int *period);

@@ -92,6 +92,31 @@ extern "C" void orc_gather(const orc_grid *g, const double *const *w, int dir, l
     for (int v = 0; v < nv; v++) out[l * nv + v] = s[l][v];
 }
 
+// EulerData::ExchangeStart/ExchangeEnd (euler3D.hpp:577-1191) for the single rank that owns
+// the grid: receive buffers W, E, S, N, B, F (null entries skipped) in the layout
+// BUFINDX(v,s,a,b,NVAR,3,na,nb) = v + NVAR*(s + 3*(a + na*b)) with (a, b) = (j, k) for W/E,
+// (i, k) for S/N, (i, j) for B/F; slot s holds the point at extended index s-3 (low faces)
+// or n+s (high faces), by the ghost rules of field_at.
+extern "C" void orc_exchange(const orc_grid *g, const double *const *w, double *const *recv) {
+  const int nv = 5 + g->nchem;
+  const long n[3] = {g->nx, g->ny, g->nz};
+  for (int face = 0; face < 6; face++) {
+    if (!recv[face]) continue;
+    const int d = face / 2;
+    const bool high = face % 2 == 1;
+    const long na = d == 0 ? g->ny : g->nx, nb = d == 2 ? g->ny : g->nz;
+    for (long b = 0; b < nb; b++)
+      for (long a = 0; a < na; a++)
+        for (int s = 0; s < 3; s++) {
+          const long e = high ? n[d] + s : s - 3;
+          const long i = d == 0 ? 0 : a, j = d == 0 ? a : (d == 1 ? 0 : b),
+                     k = d == 2 ? 0 : b;
+          for (int v = 0; v < nv; v++)
+            recv[face][v + nv * (s + 3 * (a + na * b))] = field_at(g, w, v, d, i, j, k, e);
+        }
+  }
+}
+
 // fEuler (utilities.cpp:17-253) with zero external forcing (the forcing used by every
 // configuration here: primordial_blast.cpp:312-317, sod.cpp:182-210, fluid_blast).
 // Fluxes are stored per face in the reference's AoS-by-component layout and the

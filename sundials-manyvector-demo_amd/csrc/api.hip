@@ -29,6 +29,7 @@ struct mvd_context {
   double *adj_buf = nullptr;          // chemistry adjoint checkpoints and scratch
   size_t adj_len = 0;
   mvd::MriState mri;                  // multirate evolution (first use)
+  int *pack_scratch = nullptr;        // bounds flag of mvd_pack1d (first use)
 };
 
 namespace {
@@ -146,6 +147,7 @@ void mvd_destroy(mvd_context *c) {
     for (int i = 0; i < 2 * mvd::PROF_NSLOT; i++) (void)hipEventDestroy(c->fe.timer.ev[i]);
   if (c->host_flag) (void)hipHostFree(c->host_flag);
   if (c->vjp_buf) (void)hipFree(c->vjp_buf);
+  if (c->pack_scratch) (void)hipFree(c->pack_scratch);
   if (c->ls_lu) (void)hipFree(c->ls_lu);
   if (c->ls_piv) (void)hipFree(c->ls_piv);
   if (c->ls_nsing) (void)hipFree(c->ls_nsing);
@@ -189,6 +191,32 @@ int mvd_fe_vjp(mvd_context *ctx, double, const double *const *w, const double *c
   if (rc == MVD_UNSUPPORTED) return rc;
   if (rc || !dev_ok(hipStreamSynchronize(ctx->stream))) return MVD_DEVICE_ERROR;
   return MVD_SUCCESS;
+}
+
+int mvd_exchange(mvd_context *ctx, const double *const *w, double *const *recv) {
+  if (!ctx || !recv) return MVD_BAD_ARGUMENT;
+  const int rc = check_ptrs(ctx, w);
+  if (rc) return rc;
+  if (mvd::exchange_launch(ctx->g, fields_of(ctx, w), recv, ctx->stream) ||
+      !dev_ok(hipStreamSynchronize(ctx->stream)))
+    return MVD_DEVICE_ERROR;
+  return MVD_SUCCESS;
+}
+
+int mvd_pack1d(mvd_context *ctx, const double *const *w, int dir, long ncell, const int *ijk,
+               double *out) {
+  if (!ctx || dir < 0 || dir > 2 || ncell < 0 || (ncell > 0 && (!ijk || !out)))
+    return MVD_BAD_ARGUMENT;
+  const int rc = check_ptrs(ctx, w);
+  if (rc) return rc;
+  if (!ctx->pack_scratch && !dev_ok(hipMalloc(&ctx->pack_scratch, sizeof(int))))
+    return MVD_DEVICE_ERROR;
+  int bad = 0;
+  if (mvd::pack1d_launch(ctx->g, fields_of(ctx, w), dir, ncell, ijk, out, ctx->pack_scratch,
+                         ctx->stream, &bad))
+    return MVD_DEVICE_ERROR;
+  if (bad) return MVD_BAD_ARGUMENT;
+  return dev_ok(hipStreamSynchronize(ctx->stream)) ? MVD_SUCCESS : MVD_DEVICE_ERROR;
 }
 
 int mvd_last_state_flags(mvd_context *ctx) {

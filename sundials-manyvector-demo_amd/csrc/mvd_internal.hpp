@@ -77,6 +77,11 @@ int fe_vjp_launch(const Grid &g, const Fields &w, const Fields &v, const FieldsO
 // N_Vector operations (nvec.hip)
 constexpr int LC_MAX = 16;        // vectors per linear-combination launch
 constexpr int WRMS_MAXSUB = 8;    // subvectors per WRMS norm
+// ghost exchange buffers (W, E, S, N, B, F; null entries skipped) and stencil packs
+int exchange_launch(const Grid &g, const Fields &f, double *const *recv, hipStream_t st);
+int pack1d_launch(const Grid &g, const Fields &f, int d, long ncell, const int *ijk,
+                  double *out, int *scratch, hipStream_t st, int *host_bad);
+
 int linear_combination_launch(int nv, const double *c, const double *const *X, double *z,
                               long len, hipStream_t st);
 int wrms_norm_launch(int nsub, const double *const *x, const double *const *w, const long *len,

@@ -100,6 +100,8 @@ def lib():
         "mvd_chem_lsolve": ([vp, _P, _P], C.c_int),
         "mvd_chem_vjp": ([vp, C.c_double, _P, _P, _P, _P], C.c_int),
         "mvd_chem_temperature_check": ([vp, _P, _P, C.POINTER(C.c_long)], C.c_int),
+        "mvd_exchange": ([vp, _PP, _PP], C.c_int),
+        "mvd_pack1d": ([vp, _PP, C.c_int, C.c_long, C.c_void_p, _P], C.c_int),
         "mvd_dirk_sdirk3": ([C.POINTER(DirkOpts)], C.c_int),
         "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
                                 C.POINTER(DirkStats)], C.c_int),
@@ -193,6 +195,17 @@ class Context:
         fn = lib().mvd_fe_plain if plain else lib().mvd_fe
         allow = (MVD_SUCCESS, MVD_ILLEGAL_STATE) if allow_illegal else (MVD_SUCCESS,)
         return _check("mvd_fe", fn(self.h, t, dptrs(w), dptrs(wdot)), allow)
+
+    def exchange(self, w, recv):
+        """Fill the six ghost receive buffers (W, E, S, N, B, F; None skips one)."""
+        _check("mvd_exchange", lib().mvd_exchange(self.h, dptrs(w), dptrs(recv)))
+
+    def pack1d(self, w, d, ijk, out):
+        """Stencils w(c-3..c+2) along d of the cells ijk (int32 device tensor [n][3])."""
+        assert ijk.dtype.itemsize == 4 and ijk.is_contiguous()
+        n = ijk.numel() // 3
+        _check("mvd_pack1d", lib().mvd_pack1d(self.h, dptrs(w), d, n, C.c_void_p(ijk.data_ptr()),
+                                              dptr(out)))
 
     def erk_evolve(self, w, h, nsteps, table=None, t0=0.0, allow_illegal=False):
         """Fixed-step explicit RK of fe() in place on w (default table: Bogacki-Shampine 3)."""

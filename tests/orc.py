@@ -135,6 +135,7 @@ def lib():
         L.orc_fe.restype = C.c_int
         L.orc_face_flux.argtypes = [P, C.c_int, C.c_int, C.c_double, P]
         L.orc_gather.argtypes = [C.POINTER(Grid), PP, C.c_int, C.c_long, C.c_long, C.c_long, P]
+        L.orc_exchange.argtypes = [C.POINTER(Grid), PP, PP]
         L.orc_max_wave_speed.argtypes = [C.POINTER(Grid), PP]
         L.orc_max_wave_speed.restype = C.c_double
         L.orc_stability_dt.argtypes = [C.POINTER(Grid), PP, C.c_double]
@@ -224,6 +225,15 @@ def gather(g, w, direction, i, j, k):
     out = np.zeros(6 * nv)
     lib().orc_gather(C.byref(g), _pp(w), direction, i, j, k, _p(out))
     return out.reshape(6, nv)
+
+
+def exchange(g, w):
+    """Receive buffers [W, E, S, N, B, F], each flat in the reference's BUFINDX layout."""
+    nv = 5 + g.nchem
+    sizes = [3 * g.ny * g.nz] * 2 + [3 * g.nx * g.nz] * 2 + [3 * g.nx * g.ny] * 2
+    bufs = [np.full(m * nv, np.nan) for m in sizes]
+    lib().orc_exchange(C.byref(g), _pp(w), _pp(bufs))
+    return bufs
 
 
 def face_flux(stencil, direction, gamma=1.4):
