@@ -246,6 +246,34 @@ int mvd_chem_dirk_step_forced(mvd_context *ctx, double t, double h, const mvd_di
 #define MVD_MRI_CONSISTENT_FORCING 1
 int mvd_mri_evolve(mvd_context *ctx, double t0, double h, int nslow, double hmax_fast,
                    const mvd_dirk_opts *fast, int flags, double *const *w);
+
+/* IMEX additive Runge-Kutta tables: explicit Ae (strictly lower triangular) and diagonally
+ * implicit Ai (lower triangular, diagonal >= 0; a zero diagonal is an explicit stage),
+ * 8x8 row-major, common b and c. */
+typedef struct {
+  int stages;              /* 1..8 */
+  double Ae[64];
+  double Ai[64];
+  double b[8];
+  double c[8];
+} mvd_ark_table;
+/* ARK3(2)4L[2]SA (Kennedy & Carpenter 2003), ARKStep's default third-order IMEX pair. */
+int mvd_ark_ark324(mvd_ark_table *t);
+/* IMEX evolution of the reacting flow, device-resident: the IMEX driver
+ * (imex_chem_hydro_main.cpp:337-476, ARKStepCreate(fexpl, fimpl), the block-diagonal
+ * chemistry solver BDMPIMV :1075-1194, PostprocessStep :1002-1050) with a fixed step h:
+ * nsteps steps on w (six subvectors, chemistry scaled by mvd_chem_prepare, nchem = 10).
+ * fexpl = fslow (its total-energy overwrite of the stage included), fimpl = rhs_scale
+ * (TimeUnits) times calculate_rhs_cvklu on the chemistry. Each implicit stage solves
+ * z - h a_ii fimpl(z) = R per cell with the batched Newton/LU kernel and the controls of
+ * newton (its table is not used); after each step the total energy is rebuilt from the
+ * gas energy as PostprocessStep does. flags: MVD_IMEX_CONSISTENT_FORCING converts fexpl's
+ * chemistry rates to the scaled variables (as for mvd_mri_evolve); 0 keeps the
+ * reference's units. Returns 0, the number (> 0) of failed cell solves, MVD_ILLEGAL_STATE
+ * if fexpl met an illegal state, or an error code. */
+#define MVD_IMEX_CONSISTENT_FORCING 1
+int mvd_imex_evolve(mvd_context *ctx, double t0, double h, int nsteps, const mvd_ark_table *tab,
+                    const mvd_dirk_opts *newton, int flags, double *const *w);
 /* Checkpointed discrete adjoint of nsteps chemistry DIRK steps (no counterpart in the
  * reference, which has no adjoint; for sensitivity drivers, e.g. of the domain-integrated
  * energy to the initial species). y (scaled, [cell][10]) is advanced nsteps steps of size h

@@ -44,6 +44,18 @@ typedef struct {
   double c[6];
 } orc_erk;
 int orc_erk_evolve(const orc_grid *g, const orc_erk *tab, double h, int nsteps, double *const *w);
+/* fixed-step IMEX additive Runge-Kutta evolution (orc_imex.cpp) */
+typedef struct {
+  int stages;
+  double Ae[64];         /* 8x8 row-major explicit table, strictly lower triangular */
+  double Ai[64];         /* 8x8 row-major diagonally implicit table */
+  double b[8];
+  double c[8];
+} orc_ark;
+void orc_ark324_table(orc_ark *t);
+int orc_imex_evolve(const orc_grid *g, void *chem, double energy_units, const orc_ark *tab,
+                    const orc_dirk *newton, double h, int nsteps, int consistent,
+                    double *const *w);
 int orc_lu(const double *M, double *LU, int *piv, const double *b, double *x);
 /* single-rank ghost exchange buffers W, E, S, N, B, F (orc_fe.cpp) */
 void orc_exchange(const orc_grid *g, const double *const *w, double *const *recv);

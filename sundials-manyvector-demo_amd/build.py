@@ -16,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmvd_rhs.so")
 SOURCES = ["api.hip", "fe.hip", "fe_vjp.hip", "nvec.hip", "chem.hip", "chem_dirk.hip",
-           "erk.hip", "mri.hip", "exchange.hip"]
+           "erk.hip", "mri.hip", "exchange.hip",
+           "imex.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops: never form packed FP32 instructions (all arithmetic here is FP64).

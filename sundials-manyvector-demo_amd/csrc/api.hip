@@ -5,6 +5,7 @@
 #include <cmath>
 #include "mvd_internal.hpp"
 #include "mvd_chem.hpp"
+#include "mvd_imex.hpp"
 #include "mvd_dirk.hpp"
 #include "mvd_mri.hpp"
 #include "../../include/mvd_rhs.h"
@@ -30,6 +31,7 @@ struct mvd_context {
   size_t adj_len = 0;
   mvd::MriState mri;                  // multirate evolution (first use)
   int *pack_scratch = nullptr;        // bounds flag of mvd_pack1d (first use)
+  mvd::ImexState imex;                // IMEX evolution (first use)
 };
 
 namespace {
@@ -148,6 +150,7 @@ void mvd_destroy(mvd_context *c) {
   if (c->host_flag) (void)hipHostFree(c->host_flag);
   if (c->vjp_buf) (void)hipFree(c->vjp_buf);
   if (c->pack_scratch) (void)hipFree(c->pack_scratch);
+  mvd::imex_free(c->imex);
   if (c->ls_lu) (void)hipFree(c->ls_lu);
   if (c->ls_piv) (void)hipFree(c->ls_piv);
   if (c->ls_nsing) (void)hipFree(c->ls_nsing);
@@ -583,6 +586,73 @@ int mvd_mri_evolve(mvd_context *ctx, double, double h, int nslow, double hmax_fa
                        nslow, hmax_fast, (flags & MVD_MRI_CONSISTENT_FORCING) != 0,
                        default_waves(ctx), ctx->dirk_block, w, ctx->stream,
                        ctx->host_flag, &nfail);
+  if (rc == -2) return MVD_UNSUPPORTED;
+  if (rc) return MVD_DEVICE_ERROR;
+  if (*ctx->host_flag) return MVD_ILLEGAL_STATE;
+  return nfail;
+}
+
+int mvd_ark_ark324(mvd_ark_table *t) {
+  if (!t) return MVD_BAD_ARGUMENT;
+  std::memset(t, 0, sizeof(*t));
+  const double g = 1767732205903.0 / 4055673282236.0;
+  t->stages = 4;
+  double *E = t->Ae, *I = t->Ai;
+  E[1 * 8 + 0] = 1767732205903.0 / 2027836641118.0;
+  E[2 * 8 + 0] = 5535828885825.0 / 10492691773637.0;
+  E[2 * 8 + 1] = 788022342437.0 / 10882634858940.0;
+  E[3 * 8 + 0] = 6485989280629.0 / 16251701735622.0;
+  E[3 * 8 + 1] = -4246266847089.0 / 9704473918619.0;
+  E[3 * 8 + 2] = 10755448449292.0 / 10357097424841.0;
+  I[1 * 8 + 0] = g;
+  I[1 * 8 + 1] = g;
+  I[2 * 8 + 0] = 2746238789719.0 / 10658868560708.0;
+  I[2 * 8 + 1] = -640167445237.0 / 6845629431997.0;
+  I[2 * 8 + 2] = g;
+  I[3 * 8 + 0] = 1471266399579.0 / 7840856788654.0;
+  I[3 * 8 + 1] = -4482444167858.0 / 7529755066697.0;
+  I[3 * 8 + 2] = 11266239266428.0 / 11593286722821.0;
+  I[3 * 8 + 3] = g;
+  for (int j = 0; j < 4; j++) t->b[j] = I[3 * 8 + j];
+  t->c[1] = 1767732205903.0 / 2027836641118.0;
+  t->c[2] = 3.0 / 5.0;
+  t->c[3] = 1.0;
+  return MVD_SUCCESS;
+}
+
+int mvd_imex_evolve(mvd_context *ctx, double, double h, int nsteps, const mvd_ark_table *tab,
+                    const mvd_dirk_opts *newton, int flags, double *const *w) {
+  if (flags & ~MVD_IMEX_CONSISTENT_FORCING) return MVD_BAD_ARGUMENT;
+  if (!ctx || !tab || !(h > 0) || nsteps < 0) return MVD_BAD_ARGUMENT;
+  if (ctx->g.nchem != mvd::NSP) return MVD_UNSUPPORTED;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  if (ctx->chem.n != (long)ctx->g.nx * ctx->g.ny * ctx->g.nz) return MVD_BAD_ARGUMENT;
+  const int S = tab->stages;
+  if (S < 1 || S > mvd::ARK_MAXSTAGE) return MVD_BAD_ARGUMENT;
+  for (int i = 0; i < S; i++) {
+    if (!(tab->Ai[i * 8 + i] >= 0.0) || tab->Ae[i * 8 + i] != 0.0) return MVD_BAD_ARGUMENT;
+    for (int j = i + 1; j < 8; j++)
+      if (tab->Ae[i * 8 + j] != 0.0 || tab->Ai[i * 8 + j] != 0.0) return MVD_BAD_ARGUMENT;
+    for (int j = 0; j < 8; j++)
+      if (!std::isfinite(tab->Ae[i * 8 + j]) || !std::isfinite(tab->Ai[i * 8 + j]))
+        return MVD_BAD_ARGUMENT;
+  }
+  mvd::DirkDev d;
+  int rc = dirk_check(newton, d);
+  if (rc) return rc;
+  rc = check_ptrs(ctx, (const double *const *)w);
+  if (rc) return rc;
+  if (nsteps == 0) return MVD_SUCCESS;
+  mvd::ArkTable t;
+  t.stages = S;
+  std::memcpy(t.Ae, tab->Ae, sizeof(t.Ae));
+  std::memcpy(t.Ai, tab->Ai, sizeof(t.Ai));
+  std::memcpy(t.b, tab->b, sizeof(t.b));
+  std::memcpy(t.c, tab->c, sizeof(t.c));
+  int nfail = 0;
+  rc = mvd::imex_evolve(ctx->g, ctx->fe, ctx->chem, 1.0 / ctx->energy_units, ctx->imex, t, d, h,
+                        nsteps, (flags & MVD_IMEX_CONSISTENT_FORCING) != 0, default_waves(ctx),
+                        ctx->dirk_block, w, ctx->stream, ctx->host_flag, &nfail);
   if (rc == -2) return MVD_UNSUPPORTED;
   if (rc) return MVD_DEVICE_ERROR;
   if (*ctx->host_flag) return MVD_ILLEGAL_STATE;

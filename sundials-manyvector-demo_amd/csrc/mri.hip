@@ -41,16 +41,6 @@ constexpr double KW_A[4][4] = {{0.0, 0.0, 0.0, 0.0},
 
 __global__ void mri_or_flag(const unsigned *src, unsigned *dst) { *dst |= *src; }
 __global__ void mri_add_int(const int *src, int *dst) { *dst += *src; }
-// chemistry rates of one slow RHS into the fast stepper's scaled variables
-__global__ void __launch_bounds__(256)
-mri_scale_forcing(long n, double *Fc, const double *scale, double eunits) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-#pragma unroll
-  for (int s = 0; s < NSP - 1; s++) Fc[NSP * i + s] /= scale[NSP * i + s];
-  Fc[NSP * i + NSP - 1] = Fc[NSP * i + NSP - 1] * eunits / scale[NSP * i + NSP - 1];
-}
-
 __global__ void mri_zero(unsigned *flag, int *nfail) {
   *flag = 0u;
   *nfail = 0;
@@ -99,9 +89,8 @@ int enqueue_slow_step(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_
     hipLaunchKernelGGL(mri_or_flag, dim3(1), dim3(1), 0, st, (const unsigned *)fe.flag, m.flag);
     if (fslow_finish_launch(g, Fi, st)) return -1;
     if (chem_scale_launch(n * g.nchem, Y[5], ch.scale, 0, st)) return -1;
-    if (consistent)
-      hipLaunchKernelGGL(mri_scale_forcing, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                         n, Fi[5], (const double *)ch.scale, 1.0 / eunit_inv);
+    if (consistent && chem_forcing_to_scaled_launch(n, Fi[5], ch.scale, 1.0 / eunit_inv, st))
+      return -1;
     // forcing r_i over the whole ManyVector
     const double dc = KW_C[i] - KW_C[i - 1];
     double coef[3];

@@ -103,6 +103,25 @@ chem_scale(long n, double *chem, const double *scale, int apply) {
   else chem[i] /= scale[i];
 }
 
+// chemistry rates of one slow / explicit RHS into the scaled variables the chemistry is
+// integrated in: species divided by their scale, the gas-energy slot (a code-unit energy
+// rate) times EnergyUnits divided by the ge scale
+__global__ void __launch_bounds__(256)
+chem_forcing_to_scaled(long n, double *Fc, const double *scale, double eunits) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int s = 0; s < 9; s++) Fc[10 * i + s] /= scale[10 * i + s];
+  Fc[10 * i + 9] = Fc[10 * i + 9] * eunits / scale[10 * i + 9];
+}
+
+int chem_forcing_to_scaled_launch(long n, double *Fc, const double *scale, double eunits,
+                                  hipStream_t st) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(chem_forcing_to_scaled, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     st, n, Fc, scale, eunits);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
 int fslow_prepare_launch(const Grid &g, double *const *w, const double *scale,
                          double eunit_inv, hipStream_t st) {
   const long n = (long)g.nx * g.ny * g.nz;

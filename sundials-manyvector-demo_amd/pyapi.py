@@ -37,6 +37,12 @@ class DirkOpts(C.Structure):
                 ("max_halvings", C.c_int)]
 
 
+class ArkTable(C.Structure):
+    """mvd_ark_table: explicit / diagonally implicit IMEX pair, up to 8 stages (8x8)."""
+    _fields_ = [("stages", C.c_int), ("Ae", C.c_double * 64), ("Ai", C.c_double * 64),
+                ("b", C.c_double * 8), ("c", C.c_double * 8)]
+
+
 class ErkTable(C.Structure):
     """mvd_erk_table: explicit Butcher table, up to 6 stages (row-major 6x6 A)."""
     _fields_ = [("stages", C.c_int), ("A", C.c_double * 36), ("b", C.c_double * 6),
@@ -110,6 +116,9 @@ def lib():
                                        C.POINTER(DirkStats)], C.c_int),
         "mvd_mri_evolve": ([vp, C.c_double, C.c_double, C.c_int, C.c_double,
                             C.POINTER(DirkOpts), C.c_int, C.POINTER(_P)], C.c_int),
+        "mvd_ark_ark324": ([C.POINTER(ArkTable)], C.c_int),
+        "mvd_imex_evolve": ([vp, C.c_double, C.c_double, C.c_int, C.POINTER(ArkTable),
+                             C.POINTER(DirkOpts), C.c_int, C.POINTER(_P)], C.c_int),
         "mvd_chem_dirk_adjoint": ([vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                    C.POINTER(DirkOpts), _P, _P, _P], C.c_int),
     }
@@ -154,6 +163,12 @@ def _check(fn, rc, allow=(MVD_SUCCESS,)):
 def bs3():
     t = ErkTable()
     _check("mvd_erk_bs3", lib().mvd_erk_bs3(C.byref(t)))
+    return t
+
+
+def ark324():
+    t = ArkTable()
+    _check("mvd_ark_ark324", lib().mvd_ark_ark324(C.byref(t)))
     return t
 
 
@@ -298,6 +313,16 @@ class Context:
                                   1 if consistent else 0, dptrs(w))
         if rc < 0:
             _check("mvd_mri_evolve", rc)
+        return rc
+
+    def imex_evolve(self, w, h, nsteps, newton, table=None, consistent=False, t0=0.0):
+        """IMEX ARK evolution (explicit fexpl, implicit chemistry; default table
+        ARK3(2)4L[2]SA) in place on w; returns the failed cell solves (0: all converged)."""
+        tab = table if table is not None else ark324()
+        rc = lib().mvd_imex_evolve(self.h, t0, h, int(nsteps), C.byref(tab), C.byref(newton),
+                                   1 if consistent else 0, dptrs(w))
+        if rc < 0:
+            _check("mvd_imex_evolve", rc)
         return rc
 
     def dirk_step_forced(self, y, h, force, opts=None, t=0.0):

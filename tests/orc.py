@@ -61,6 +61,30 @@ class Erk(C.Structure):
                 ("c", C.c_double * 6)]
 
 
+class Ark(C.Structure):
+    """orc_ark (same layout as the library's mvd_ark_table): explicit / implicit 8x8 tables."""
+    _fields_ = [("stages", C.c_int), ("Ae", C.c_double * 64), ("Ai", C.c_double * 64),
+                ("b", C.c_double * 8), ("c", C.c_double * 8)]
+
+    def tables(self):
+        s = self.stages
+        Ae = np.array(self.Ae[:]).reshape(8, 8)[:s, :s]
+        Ai = np.array(self.Ai[:]).reshape(8, 8)[:s, :s]
+        return Ae, Ai, np.array(self.b[:s]), np.array(self.c[:s])
+
+
+def ark324_table():
+    t = Ark()
+    lib().orc_ark324_table(C.byref(t))
+    return t
+
+
+def imex_evolve(g, chem, energy_units, tab, newton, h, nsteps, w, consistent=False):
+    """Oracle IMEX ARK evolution in place on w (6 subvectors, chemistry scaled)."""
+    return lib().orc_imex_evolve(C.byref(g), chem.h, energy_units, C.byref(tab), C.byref(newton),
+                                 h, nsteps, 1 if consistent else 0, _pp(w))
+
+
 def bs3_table():
     """Bogacki-Shampine 3(2) (Bogacki & Shampine, Appl. Math. Lett. 2 (1989) 321-325),
     from its published rationals: c = (0, 1/2, 3/4, 1), a21 = 1/2, a32 = 3/4,
@@ -155,6 +179,10 @@ def lib():
         L.orc_mri_fast_steps.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_mri_fast_steps.restype = C.c_int
         L.orc_erk_evolve.argtypes = [C.POINTER(Grid), C.POINTER(Erk), C.c_double, C.c_int, PP]
+        L.orc_ark324_table.argtypes = [C.POINTER(Ark)]
+        L.orc_imex_evolve.argtypes = [C.POINTER(Grid), C.c_void_p, C.c_double, C.POINTER(Ark),
+                                      C.POINTER(Dirk), C.c_double, C.c_int, C.c_int, PP]
+        L.orc_imex_evolve.restype = C.c_int
         L.orc_erk_evolve.restype = C.c_int
         L.orc_chem_create.argtypes = [C.c_long, P, C.c_double]
         L.orc_chem_create.restype = C.c_void_p
