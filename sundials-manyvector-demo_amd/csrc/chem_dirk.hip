@@ -762,7 +762,7 @@ tord_scatter(long n, const unsigned char *key, int *cnt, int *order) {
 }
 
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
-                        int grid_waves, int block, const double *force) {
+                        int grid_waves, int block, const double *force, bool reuse_order) {
   const long n = s.n;
   if (n <= 0 || n > (1L << 30)) return -1;
   if (block != 64 && block != 128 && block != 256) return -1;
@@ -782,11 +782,13 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   const size_t b_ord = sorted ? align256((size_t)n * sizeof(int)) : 0;
   const size_t b_cnt = sorted ? align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int)) : 0;
   const size_t b_key = sorted ? align256((size_t)n) : 0;
+  const void *work0 = s.work;
   if (ensure_work(s, b_rows + b_ord + b_cnt + b_key)) return -1;
+  if (s.work != work0) reuse_order = false;                // a new buffer holds no order
   char *w = (char *)s.work;
   int *rows = (int *)(w);
   int *order = sorted ? (int *)(w + b_rows) : nullptr;
-  if (sorted) {
+  if (sorted && !reuse_order) {
     int *cnt = (int *)(w + b_rows + b_ord);
     unsigned char *key = (unsigned char *)(w + b_rows + b_ord + b_cnt);
     hipLaunchKernelGGL(fill_int, dim3(2 * TORD_NBK * TORD_PAD / 256), dim3(256), 0, st,

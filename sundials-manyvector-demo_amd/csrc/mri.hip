@@ -107,7 +107,10 @@ int enqueue_slow_step(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_
     const double *X2[2] = {m.Y, m.R};
     for (int k = 0; k < nfast; k++) {
       if (linear_combination_launch(2, c2, X2, m.Y, 5 * n, st)) return -1;   // fluid
-      if (dirk_batched_launch(ch, o, dt, Y[5], st, waves, block, m.R + 5 * n)) return -1;
+      // the cells' temperature order is taken at the stage's first fast step and kept for
+      // the others (temperatures move little over one slow stage)
+      if (dirk_batched_launch(ch, o, dt, Y[5], st, waves, block, m.R + 5 * n, k > 0))
+        return -1;
       hipLaunchKernelGGL(mri_add_int, dim3(1), dim3(1), 0, st,
                          (const int *)(ch.counters + 1 + STAT_FAILED), m.nfail);
     }
@@ -158,9 +161,11 @@ int mri_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, Mr
   hipLaunchKernelGGL(mri_zero, dim3(1), dim3(1), 0, st, m.flag, m.nfail);
   const bool timer_on = fe.timer.on;
   fe.timer.on = false;                    // no event records inside the graph
+  // the captured step holds the addresses of the chemistry work buffer: another solver call
+  // may have reallocated it since
   bool rebuild = !m.exec || m.h != h || m.hmax != hmax || m.consistent != consistent ||
-                 !same_opts(m.o, o) ||
-                 m.waves != waves || m.block != block;
+                 !same_opts(m.o, o) || m.waves != waves || m.block != block ||
+                 m.work != (const void *)ch.work;
   int rc = 0;
   for (int step = 0; step < nslow && rc == 0; step++) {
     if (!rebuild) {
@@ -194,6 +199,7 @@ int mri_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, Mr
     m.o = o;
     m.waves = waves;
     m.block = block;
+    m.work = ch.work;
     rebuild = false;
   }
   fe.timer.on = timer_on;

@@ -26,9 +26,12 @@ struct DirkDev {
   int max_halvings;     // per-cell step rejection (batched solver), 0: off
 };
 
-// force (nullable): constant forcing added to the scaled RHS, [cell][10]
+// force (nullable): constant forcing added to the scaled RHS, [cell][10]. reuse_order:
+// keep the cell order of the previous launch (same cells, work buffer not reallocated
+// since) instead of sorting again.
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
-                        int grid_waves, int block, const double *force = nullptr);
+                        int grid_waves, int block, const double *force = nullptr,
+                        bool reuse_order = false);
 int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st);
 // checkpointed discrete adjoint of nsteps DIRK steps (checkpoint every K steps)
 size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K);

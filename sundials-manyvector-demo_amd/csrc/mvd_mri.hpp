@@ -15,6 +15,7 @@ struct MriState {
   int waves = 0, block = 0;
   bool consistent = false;
   DirkDev o{};
+  const void *work = nullptr;                        // chemistry work buffer when captured
 };
 
 // nslow slow steps of size h (fast steps no longer than hmax) in place on w (6 device

@@ -89,7 +89,8 @@ def test_mri_matches_oracle(mvd, torch_gpu, consistent):
 
 def test_mri_graph_replay_is_exact(mvd, torch_gpu):
     """3 slow steps in one call equal 1 + 2 calls bit for bit (eager step, captured step
-    replayed), and a changed fast bound recaptures."""
+    replayed), also with a plain-path chemistry step between the calls, and a changed fast
+    bound recaptures."""
     torch = torch_gpu
     dims, bc = (10, 8, 8), (3,) * 6
     fast = mvd.sdirk3(rhs_scale=UNITS[2])
@@ -101,6 +102,14 @@ def test_mri_graph_replay_is_exact(mvd, torch_gpu):
     b = [t.clone() for t in a0]
     ctx.chem_state_set("Ts", Ts0)
     ctx.mri_evolve(b, 0.05, 1, 0.01, fast)
+    # a plain-path step in between takes a larger chemistry work buffer, which the captured
+    # slow step must not keep using (it is captured again)
+    Ts1 = ctx.chem_state("Ts", torch).clone()
+    dT1 = ctx.chem_state("dTs_ge", torch).clone()
+    scratch = b[5].clone()
+    ctx.dirk_step(scratch, 1e3, plain=True)
+    ctx.chem_state_set("Ts", Ts1)
+    ctx.chem_state_set("dTs_ge", dT1)
     ctx.mri_evolve(b, 0.05, 2, 0.01, fast)
     for q in range(6):
         assert torch.equal(a[q], b[q]), q
