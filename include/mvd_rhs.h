@@ -101,7 +101,7 @@ int mvd_fslow(mvd_context *ctx, double t, double *const *w, double *const *wdot)
  * and face flux with its Profile class, profiler.hpp:23-206). mvd_profile(ctx,1) resets
  * and enables; mvd_profile_read returns the number of slots and fills per-slot total
  * milliseconds and launch counts. Slots: 0 fe_xface, 1 fe_yface, 2 fe_zface_div,
- * 3 fe_plain_faces, 4 fe_plain_div. */
+ * 3 fe_plain_faces, 4 fe_plain_div, 5 fe_xyface_div. */
 int mvd_profile(mvd_context *ctx, int on);
 int mvd_profile_read(mvd_context *ctx, double *ms, long *count, int nslot);
 

@@ -5,14 +5,20 @@
 //                    (plus the domain's upper boundary faces) with the reference-order
 //                    face flux, into SoA face-flux buffers;
 //   fe_plain_div   : one thread per cell forms the divergence in the reference order.
-// Fast path (three launches):
+// Fast path without tracers (two launches):
+//   fe_xyface_div  : x- and y-faces of 16 x 15-cell plane tiles from one LDS staging of the
+//                    tile and its x/y halos, and their part of the divergence into wdot
+//                    (the x and y face fluxes stay in LDS); the legal-state check;
+//   fe_zface_div   : z-faces (tiles of 16 x-cells by 16 faces, their primitives staged in
+//                    LDS), and the divergence of the 15 cells between a tile's 16 faces
+//                    (tiles overlap by one face), adding the x/y part read back from wdot.
+// Fast path with tracers (three launches):
 //   fe_xface       : x-faces, one thread per face; a block stages the primitives of the
 //                    points its faces need in LDS once, then every thread reads its
 //                    6-point stencil from LDS; also performs the legal-state check;
 //   fe_yface       : y-faces the same way, tiles of 16 x-cells by 16 faces;
-//   fe_zface_div   : z-faces the same way, plus the divergence of the 15 cells between a
-//                    tile's 16 faces (tiles overlap by one face), reading the x- and
-//                    y-face fluxes, and writes wdot.
+//   fe_zface_div   : z-faces as above, reading the x- and y-face fluxes for the
+//                    divergence, and writes wdot.
 // The divergence is accumulated as -((dFx/dx + dFy/dy) + dFz/dz), the reference's
 // association (utilities.cpp:202-207).
 #include "mvd_device.hpp"
@@ -332,7 +338,8 @@ fe_yface(Grid g, Fields f, double *Fy) {
 // z-faces with the divergence fused: a block computes TF z-faces per column and forms
 // wdot for the TF-1 cells between them (tiles overlap by one face):
 //   wdot = 0 - (((Fx[i+1]-Fx[i])/dx + (Fy[j+1]-Fy[j])/dy) + (Fz[k+1]-Fz[k])/dz)
-template <int NC, int TX, int TF>
+// XY: the x and y part is already in out (fe_xyface_div) and is read from there.
+template <int NC, int TX, int TF, bool XY>
 __global__ void __launch_bounds__(256)
 fe_zface_div(Grid g, Fields f, const double *Fx, const double *Fy, FieldsOut out, double idx,
              double idy, double idz) {
@@ -386,12 +393,128 @@ fe_zface_div(Grid g, Fields f, const double *Fx, const double *Fy, FieldsOut out
   const long yf = i + (long)g.nx * (j + (long)nyp * k);
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    const double dx = (Fx[v * nfx + xf + 1] - Fx[v * nfx + xf]) * idx;
-    const double dy = (Fy[v * nfy + yf + g.nx] - Fy[v * nfy + yf]) * idy;
+    double dxy;
+    if (XY) {
+      dxy = out.w[v][cell];
+    } else {
+      const double dx = (Fx[v * nfx + xf + 1] - Fx[v * nfx + xf]) * idx;
+      const double dy = (Fy[v * nfy + yf + g.nx] - Fy[v * nfy + yf]) * idy;
+      dxy = dx + dy;
+    }
     const double dz = (s_buf[(v * TF + ty + 1) * TX + tx] - s_buf[(v * TF + ty) * TX + tx]) * idz;
-    const double o = 0.0 - ((dx + dy) + dz);
+    const double o = 0.0 - (dxy + dz);
     if (v < 5) out.w[v][cell] = o;
     else out.chem[(long)NC * cell + (v - 5)] = o;
+  }
+}
+
+// x- and y-faces of one z-plane tile together, without tracers (nchem = 0): a block covers
+// XY_TX x XY_TY cells of plane k and forms (Fx[i+1]-Fx[i])/dx + (Fy[j+1]-Fy[j])/dy for them
+// into out (the z kernel then adds the z part), so the x and y face fluxes never go to
+// HBM. The points of the tile and of its x and y halos (a cross, 3 deep) are converted
+// once into LDS, keeping mx and my in place and both wave speeds |u|+c and |v|+c, so that
+// every point serves both directions; x-faces (XY_TX+1 per row) and y-faces (XY_TY+1 per
+// column) are 255 and 256 with the 16 x 15 tile: one of each per thread.
+constexpr int XY_TX = 16, XY_TY = 15;
+constexpr int XY_PX = XY_TX + 6, XY_PY = XY_TY + 6;      // point window 22 x 21
+constexpr int XY_NP = XY_PX * XY_PY;
+constexpr int XY_NF = 10;     // r, mx, my, mz, e, p, 1/r, |u|+c, sqrt(r), |v|+c
+
+__global__ void __launch_bounds__(256, 3)
+fe_xyface_div(Grid g, Fields f, FieldsOut out, double idx, double idy, unsigned *flag) {
+  __shared__ double sh[XY_NF * XY_NP];
+  __shared__ double sfy[5 * (XY_TY + 1) * XY_TX];    // y-face fluxes [5][XY_TY+1][XY_TX]
+  const int t = threadIdx.x;
+  const int nxb = (g.nx + XY_TX - 1) / XY_TX, nyb = (g.ny + XY_TY - 1) / XY_TY;
+  int b = blockIdx.x;
+  const int xb = b % nxb;
+  b /= nxb;
+  const int yb = b % nyb;
+  const int k = b / nyb;
+  const int x0 = xb * XY_TX, y0 = yb * XY_TY;
+  // stage the cross: points of the tile's rows (real y) over the x window, with x ghosts,
+  // then the other points of the tile's columns (real x) over the y window, with y ghosts
+  for (int q = t; q < XY_NP; q += blockDim.x) {
+    const int yy = q / XY_PX, xx = q - yy * XY_PX;
+    const int ex = x0 - 3 + xx, ey = y0 - 3 + yy;
+    const bool row = yy >= 3 && yy < 3 + XY_TY && ey < g.ny;          // x-line point
+    const bool col = xx >= 3 && xx < 3 + XY_TX && ex < g.nx;          // y-column point
+    int i = ex, j = ey;
+    double s_all = 1.0, s_mx = 1.0, s_my = 1.0;
+    if (row) {
+      if (ex > g.nx + 2) continue;
+      if (ex < 0 || ex >= g.nx) {
+        const GhostMap m = ghost_map(g, 0, ex);
+        i = m.src;
+        s_all = s_my = m.sgn_all;
+        s_mx = m.sgn_norm;
+      }
+    } else if (col) {
+      if (ey > g.ny + 2) continue;
+      if (ey < 0 || ey >= g.ny) {
+        const GhostMap m = ghost_map(g, 1, ey);
+        j = m.src;
+        s_all = s_mx = m.sgn_all;
+        s_my = m.sgn_norm;
+      }
+    } else {
+      continue;
+    }
+    const long c = i + (long)g.nx * (j + (long)g.ny * k);
+    const double r = f.w[0][c] * s_all, mx = f.w[1][c] * s_mx, my = f.w[2][c] * s_my;
+    const double mz = f.w[3][c] * s_all, e = f.w[4][c] * s_all;
+    const double ir = fast_rcp(r);
+    const double p = (g.gamma - 1.0) * (e - 0.5 * (mx * mx + my * my + mz * mz) * ir);
+    const double cs = sun_sqrt(g.gamma * p * ir);
+    double *pt = sh + q;
+    pt[0 * XY_NP] = r;  pt[1 * XY_NP] = mx; pt[2 * XY_NP] = my;
+    pt[3 * XY_NP] = mz; pt[4 * XY_NP] = e;  pt[5 * XY_NP] = p;
+    pt[6 * XY_NP] = ir; pt[7 * XY_NP] = fabs(mx * ir) + cs;
+    pt[8 * XY_NP] = sun_sqrt(r);
+    pt[9 * XY_NP] = fabs(my * ir) + cs;
+    if (row && col && ex >= 0) {      // a cell of the tile: legal_state once
+      unsigned bits = 0;
+      if (!(r > 0.0)) bits |= 1u;
+      if (!(e > 0.0)) bits |= 2u;
+      if (!(p > 0.0)) bits |= 4u;
+      if (bits) atomicOr(flag, bits);
+    }
+  }
+  __syncthreads();
+  // y-face t: column cx, faces between rows y0+fy-1 and y0+fy (points yy = fy..fy+5)
+  const int cx = t % XY_TX, fy = t / XY_TX;
+  const bool yok = x0 + cx < g.nx && y0 + fy <= g.ny && fy <= XY_TY;
+  // x-face t: row ry, faces between columns x0+fx-1 and x0+fx (points xx = fx..fx+5)
+  const int ry = t / (XY_TX + 1), fx = t % (XY_TX + 1);
+  const bool xok = t < (XY_TX + 1) * XY_TY && y0 + ry < g.ny && x0 + fx <= g.nx;
+  if (yok) {
+    double Fl[5], Fy[5], alpha;
+    face_flux_lds_f<2, 1, 9>(sh + fy * XY_PX + cx + 3, XY_NP, XY_PX, g.gamma, Fl, alpha);
+    to_global<1>(Fl, Fy);
+#pragma unroll
+    for (int a = 0; a < 5; a++) sfy[(a * (XY_TY + 1) + fy) * XY_TX + cx] = Fy[a];
+  }
+  double Fx[5];
+  if (xok) {
+    double alpha;
+    face_flux_lds_f<1, 2, 7>(sh + (ry + 3) * XY_PX + fx, XY_NP, 1, g.gamma, Fx, alpha);
+  }
+  __syncthreads();                    // points dead: their LDS takes the x-face fluxes
+  double *sfx = sh;                   // [5][XY_TY][XY_TX+1]
+  if (xok)
+#pragma unroll
+    for (int a = 0; a < 5; a++) sfx[(a * XY_TY + ry) * (XY_TX + 1) + fx] = Fx[a];
+  __syncthreads();
+  const int ci = t % XY_TX, cj = t / XY_TX;
+  if (cj >= XY_TY || x0 + ci >= g.nx || y0 + cj >= g.ny) return;
+  const long cell = (x0 + ci) + (long)g.nx * ((y0 + cj) + (long)g.ny * k);
+#pragma unroll
+  for (int a = 0; a < 5; a++) {
+    const double dx = (sfx[(a * XY_TY + cj) * (XY_TX + 1) + ci + 1] -
+                       sfx[(a * XY_TY + cj) * (XY_TX + 1) + ci]) * idx;
+    const double dy = (sfy[(a * (XY_TY + 1) + cj + 1) * XY_TX + ci] -
+                       sfy[(a * (XY_TY + 1) + cj) * XY_TX + ci]) * idy;
+    out.w[a][cell] = dx + dy;
   }
 }
 
@@ -422,20 +545,35 @@ static int launch_fe(const Grid &g, const Fields &f, const FieldsOut &o, FeScrat
     tm.end(PROF_PLAIN_DIV, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
   }
+  constexpr int TX = TILE_X, TF = TILE_F;
+  const long nxc = (g.nx + TX - 1) / TX;
+  if (NC == 0) {
+    // x and y faces of each plane tile in one kernel, then the z faces adding their part
+    const long nxyb = ((long)(g.nx + XY_TX - 1) / XY_TX) * ((g.ny + XY_TY - 1) / XY_TY) * g.nz;
+    tm.begin(PROF_XYFACE_DIV, st);
+    hipLaunchKernelGGL(fe_xyface_div, dim3((unsigned)nxyb), dim3(256), 0, st, g, f, o,
+                       1.0 / g.dx, 1.0 / g.dy, s.flag);
+    tm.end(PROF_XYFACE_DIV, st);
+    tm.begin(PROF_ZFACE_DIV, st);
+    hipLaunchKernelGGL((fe_zface_div<NC, TX, TF, true>),
+                       dim3((unsigned)(nxc * ((g.nz + TF - 2) / (TF - 1)) * g.ny)), dim3(TX * TF),
+                       0, st, g, f, (const double *)nullptr, (const double *)nullptr, o,
+                       1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz);
+    tm.end(PROF_ZFACE_DIV, st);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  }
   const long nchunk = (nxp + 63) / 64;
   const long nrowblk = ((long)g.ny * g.nz + XROWS - 1) / XROWS;
   tm.begin(PROF_XFACE, st);
   hipLaunchKernelGGL(fe_xface<NC>, dim3((unsigned)(nchunk * nrowblk)), dim3(64 * XROWS), 0, st,
                      g, f, s.Fx, s.flag);
   tm.end(PROF_XFACE, st);
-  constexpr int TX = TILE_X, TF = TILE_F;
-  const long nxc = (g.nx + TX - 1) / TX;
   tm.begin(PROF_YFACE, st);
   hipLaunchKernelGGL((fe_yface<NC, TX, TF>), dim3((unsigned)(nxc * ((nyp + TF - 1) / TF) * g.nz)),
                      dim3(TX * TF), 0, st, g, f, s.Fy);
   tm.end(PROF_YFACE, st);
   tm.begin(PROF_ZFACE_DIV, st);
-  hipLaunchKernelGGL((fe_zface_div<NC, TX, TF>),
+  hipLaunchKernelGGL((fe_zface_div<NC, TX, TF, false>),
                      dim3((unsigned)(nxc * ((g.nz + TF - 2) / (TF - 1)) * g.ny)), dim3(TX * TF),
                      0, st, g, f, s.Fx, s.Fy, o, 1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz);
   tm.end(PROF_ZFACE_DIV, st);

@@ -261,22 +261,25 @@ __device__ inline double weno5_pair(const double (&fp)[5], const double (&fm)[5]
 }
 
 // Face flux with the 6-point stencil read from LDS as it is needed (point l, field q at
-// base[q*FS + l*LS], fields in Pt order r, mn, m1, m2, e, p, ir, a, sr), one split side at a
-// time, so only one side's 25 projected values are live.
-__device__ inline void face_flux_lds(const double *base, int FS, int LS, double gamma,
-                                     double (&F)[5], double &alpha_out) {
+// base[q*FS + l*LS]), one split side at a time, so only one side's 25 projected values are
+// live. Fields: 0 r, FMN the normal momentum, FM1 the first transverse one, 3 the second,
+// 4 e, 5 p, 6 1/r, FA |u_n| + c, 8 sqrt(r) (Pt order is FMN = 1, FM1 = 2, FA = 7; a point
+// shared by x- and y-faces keeps mx, my in 1, 2 and both speeds, in 7 and 9).
+template <int FMN, int FM1, int FA>
+__device__ inline void face_flux_lds_f(const double *base, int FS, int LS, double gamma,
+                                       double (&F)[5], double &alpha_out) {
   const double gm = gamma - 1.0;
   double alpha = 0.0;
 #pragma unroll
-  for (int l = 0; l < 6; l++) alpha = fmax(alpha, base[7 * FS + l * LS]);
+  for (int l = 0; l < 6; l++) alpha = fmax(alpha, base[FA * FS + l * LS]);
   double u, v, w, H;
   {
     const double *L = base + 2 * LS, *R = base + 3 * LS;
     const double Lsr = L[8 * FS], Rsr = R[8 * FS];
     const double iden = fast_rcp(Lsr + Rsr);
     const double wl = Lsr * L[6 * FS], wr = Rsr * R[6 * FS];
-    u = (L[1 * FS] * wl + R[1 * FS] * wr) * iden;
-    v = (L[2 * FS] * wl + R[2 * FS] * wr) * iden;
+    u = (L[FMN * FS] * wl + R[FMN * FS] * wr) * iden;
+    v = (L[FM1 * FS] * wl + R[FM1 * FS] * wr) * iden;
     w = (L[3 * FS] * wl + R[3 * FS] * wr) * iden;
     H = ((L[5 * FS] + L[4 * FS]) * wl + (R[5 * FS] + R[4 * FS]) * wr) * iden;
   }
@@ -295,7 +298,8 @@ __device__ inline void face_flux_lds(const double *base, int FS, int LS, double 
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       const double *q = base + (l + side) * LS;
-      const double r = q[0], mn = q[FS], m1 = q[2 * FS], m2 = q[3 * FS], e = q[4 * FS];
+      const double r = q[0], mn = q[FMN * FS], m1 = q[FM1 * FS], m2 = q[3 * FS],
+                   e = q[4 * FS];
       const double p = q[5 * FS];
       const double ul = mn * q[6 * FS];
       const double sa = side == 0 ? alpha : -alpha;
@@ -333,6 +337,11 @@ __device__ inline void face_flux_lds(const double *base, int FS, int LS, double 
   F[3] = w * F[0] + ff[2];
   F[4] = H * s04 + u * d40 + v * ff[1] + w * ff[2] + hq * ff[3];
   alpha_out = alpha;
+}
+
+__device__ inline void face_flux_lds(const double *base, int FS, int LS, double gamma,
+                                     double (&F)[5], double &alpha_out) {
+  face_flux_lds_f<1, 2, 7>(base, FS, LS, gamma, F, alpha_out);
 }
 
 // Normal velocities of the 6 LDS stencil points (for the species part).

@@ -267,7 +267,8 @@ class Context:
     def fslow(self, w, wdot, t=0.0):
         return _check("mvd_fslow", lib().mvd_fslow(self.h, t, dptrs(w), dptrs(wdot)))
 
-    PROFILE_SLOTS = ("fe_xface", "fe_yface", "fe_zface_div", "fe_plain_faces", "fe_plain_div")
+    PROFILE_SLOTS = ("fe_xface", "fe_yface", "fe_zface_div", "fe_plain_faces", "fe_plain_div",
+                     "fe_xyface_div")
 
     def profile(self, on=True):
         _check("mvd_profile", lib().mvd_profile(self.h, 1 if on else 0))
