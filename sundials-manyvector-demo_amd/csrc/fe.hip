@@ -197,8 +197,13 @@ fe_xface(Grid g, Fields f, double *Fx, unsigned *flag) {
   const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
   const int nxp = g.nx + 1;
   const int nchunk = (nxp + 63) >> 6;
-  const int chunk = blockIdx.x % nchunk;
-  const int line = (blockIdx.x / nchunk) * XROWS + row;      // j + ny*k
+  // XCD-aware numbering (as in fe_xyface_div): neighbouring chunks of a line share halo
+  // lines and run on the same XCD one after another
+  const unsigned m8 = gridDim.x / 8;
+  const int bid =
+      blockIdx.x < 8 * m8 ? (int)((blockIdx.x % 8) * m8 + blockIdx.x / 8) : (int)blockIdx.x;
+  const int chunk = bid % nchunk;
+  const int line = (bid / nchunk) * XROWS + row;             // j + ny*k
   const int nline = g.ny * g.nz;
   const bool row_ok = line < nline;
   const int i0 = chunk * 64;
@@ -206,7 +211,7 @@ fe_xface(Grid g, Fields f, double *Fx, unsigned *flag) {
   // the block's XROWS x XPTS points, spread over all threads of the block
   for (int q2 = threadIdx.x; q2 < XROWS * XPTS; q2 += blockDim.x) {
     const int rw = q2 / XPTS, s = q2 - rw * XPTS;
-    const int ln = (blockIdx.x / nchunk) * XROWS + rw;
+    const int ln = (bid / nchunk) * XROWS + rw;
     const int e = i0 - 3 + s;
     if (ln < nline && e <= g.nx + 2) {
       double sa, sn;
@@ -426,7 +431,12 @@ fe_xyface_div(Grid g, Fields f, FieldsOut out, double idx, double idy, unsigned 
   __shared__ double sfy[5 * (XY_TY + 1) * XY_TX];    // y-face fluxes [5][XY_TY+1][XY_TX]
   const int t = threadIdx.x;
   const int nxb = (g.nx + XY_TX - 1) / XY_TX, nyb = (g.ny + XY_TY - 1) / XY_TY;
-  int b = blockIdx.x;
+  // The hardware deals blocks to the 8 XCDs in turn, and each XCD has its own L2. Tiles are
+  // renumbered so that consecutive tiles (x neighbours, which read the same halo lines)
+  // run on the same XCD one after another: block b takes tile (b % 8) * m + b / 8 (m = the
+  // blocks per XCD; the remainder past 8 m keeps its own number).
+  const unsigned m8 = gridDim.x / 8;
+  int b = blockIdx.x < 8 * m8 ? (int)((blockIdx.x % 8) * m8 + blockIdx.x / 8) : (int)blockIdx.x;
   const int xb = b % nxb;
   b /= nxb;
   const int yb = b % nyb;
