@@ -42,14 +42,15 @@ __device__ inline CellIn load_cell(const ChemPtrsD &p, long i) {
 
 // calculate_rhs_cvklu for one cell in scaled variables, times srhs; updates Ts/dTs. The
 // temperature solve reads the gamma tables from gt (global memory or an LDS copy).
+// With solve_t false, Ts/dTs already hold the solution at y and the solve is skipped.
 __device__ inline void cell_rhs(const ChemConst &cc, const GammaTabs &gt, const CellIn &c,
                                 double &Ts, double &dTs, const double (&y)[NSP], double srhs,
-                                double (&f)[NSP]) {
+                                double (&f)[NSP], bool solve_t = true) {
 #pragma clang fp contract(off)
   double ya[NSP];
 #pragma unroll
   for (int s = 0; s < NSP; s++) ya[s] = y[s] * c.s(s);
-  chem_temperature(cc, gt, ya, Ts, dTs);
+  if (solve_t) chem_temperature(cc, gt, ya, Ts, dTs);
   // species rows from the reaction tables, then the energy row from the cooling tables;
   // the fence keeps the two groups of table loads from being scheduled together
   double k[NRATE];
@@ -354,7 +355,8 @@ inline size_t dirk_lds_bytes(int wpb, int nstage) {
 template <bool FORCED, bool SUBSTEP>
 __global__ void __launch_bounds__(256)
 chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double *y,
-                  const double *force, const int *order, int *queue, int *rows) {
+                  const double *force, const int *order, const double2 *tdt, int *queue,
+                  int *rows) {
 #pragma clang fp contract(off)
   // block statistics in LDS (the per-cell updates are LDS atomics; one row per block).
   // Dynamic LDS: the gamma tables of the temperature solve (32 KB; the ten-step Newton loop
@@ -373,23 +375,31 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
   double *const yst = Fst + (o.nstage - 1) * NSP * 64;     // y_n[s] at s*64
   // The first chunk of each wave is its global wave index; later ones come from the queue
   // (which starts after them), each fetched while the previous chunk is being solved: lane
-  // 0 issues the atomic at the start of a chunk and the wave reads its result at the end.
+  // 0 issues the atomic after the chunk's prologue loads and the wave reads its result at
+  // the end.
   const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
   int next = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
   for (;;) {
     const int base = next;
     if ((long)base >= n) break;
     int q = 0;
-    if (lane == 0) q = atomicAdd(queue, 64) + nwaves * 64;
     if ((long)base + lane < n) {   // lanes past the end idle to the end of the chunk
     // the lane's cell: queue positions run over the cells in temperature order
     const int cell = order ? order[base + lane] : base + lane;
     // per-lane state for the whole step: iterate, stage right-hand side, LU factors
-    double Ts, dTs, z[NSP], r[NSP], lu[NSP][NSP];
+    double Ts, dTs, Ts0, dTs0, z[NSP], r[NSP], lu[NSP][NSP];
     {
       const long i = cell;
-      Ts = p.Ts[i];
-      dTs = p.dTs[i];
+      Ts0 = p.Ts[i];                                       // warm start at the substep start
+      dTs0 = p.dTs[i];
+      if (tdt) {                                           // solved at y_n by the ordering pass
+        const double2 t = tdt[i];
+        Ts = t.x;
+        dTs = t.y;
+      } else {
+        Ts = Ts0;
+        dTs = dTs0;
+      }
 #pragma unroll
       for (int s = 0; s < NSP; s++) { z[s] = y[NSP * i + s]; r[s] = z[s] + h * 0.0; }
 #pragma unroll
@@ -407,11 +417,13 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     // temperature warm starts) is that of the stage/Newton/re-setup loops:
     //   RHS(y_n), setup; per stage: RHS(z), Newton update, ... ; after max_iters without
     //   convergence: RHS(z), setup, and the stage's Newton loop restarts.
-    bool setup = true, ok = true;
+    // the next chunk's queue position, requested once the cell's loads are in flight (lane
+    // 0 is active whenever the chunk is)
+    if (lane == 0) q = atomicAdd(queue, 64) + nwaves * 64;
+    bool setup = true, ok = true, solved = tdt != nullptr;
     int st = 0, it = 0, setups = 0, nit = 0, nsetup = 0;
     double hs = h;                                         // current (sub)step
     double hg = h * o.A[0];                                // SDIRK: one diagonal value
-    double Ts0 = Ts, dTs0 = dTs;                           // at the substep start
     int halv = 0, done = 0;                                // substep level, units done
     unsigned long long pv = 0;
     for (;;) {
@@ -427,7 +439,8 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
                      CellAux{cl[(2 * NSP) * 64], cl[(2 * NSP + 1) * 64], cl[(2 * NSP + 2) * 64],
                              cl[(2 * NSP + 3) * 64]}};
       double f[NSP];
-      cell_rhs(cc, gt, c, Ts, dTs, z, o.rhs_scale, f);
+      cell_rhs(cc, gt, c, Ts, dTs, z, o.rhs_scale, f, !solved);
+      solved = false;
       if (setup) {
         cell_matrix_regs(cc, c, Ts, dTs, z, hg, o.rhs_scale, lu);
         pv = lu_factor_regs(lu);
@@ -696,8 +709,8 @@ static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // their temperature, in groups of 8 bins, so that the 64 lanes of a wave gather from the
 // same few lines of every rate table (and meet similar stiffness). The queue then hands
 // out positions of this order. Per-cell results do not depend on the order. The key
-// temperature is one step of the temperature iteration from the warm start (the warm
-// start itself may be a common guess such as the 1000 K of a fresh preparation).
+// temperature is the solved temperature at y_n (the warm start alone may be a common
+// guess such as the 1000 K of a fresh preparation).
 // ------------------------------------------------------------------------------------
 constexpr int TORD_NBK = 128;          // keys: table bin (0..1022) / 8
 constexpr long TORD_MIN = 4096;        // smaller batches keep the natural order
@@ -707,22 +720,26 @@ constexpr long TORD_MIN = 4096;        // smaller batches keep the natural order
 constexpr int TORD_PAD = 32;
 
 // Keys (one byte per cell) and the bucket totals, accumulated per block in LDS first. One
-// cell per thread, blocks of TORD_BLOCK threads.
+// cell per thread, blocks of TORD_BLOCK threads. The key is the temperature at y_n, solved
+// from the cell's warm start exactly as the step's first RHS evaluation solves it (same
+// function, same inputs, so the same bits); the solution goes to tdt and that evaluation
+// skips its solve.
 constexpr int TORD_BLOCK = 1024;
 __global__ void __launch_bounds__(TORD_BLOCK)
-tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, unsigned char *key, int *cnt) {
+tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, unsigned char *key, int *cnt,
+           double2 *tdt) {
 #pragma clang fp contract(off)
   __shared__ int hc[TORD_NBK];
   if (threadIdx.x < TORD_NBK) hc[threadIdx.x] = 0;
   __syncthreads();
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    double ya[NSP], d;
+    double ya[NSP], T = p.Ts[i], dT = p.dTs[i];
 #pragma unroll
     for (int s = 0; s < NSP; s++) ya[s] = y[NSP * i + s] * p.scale[NSP * i + s];
-    const double T1 =
-        temperature_step(cc, gamma_tabs(cc), ya, temperature_density(ya) * 1.67e-24, p.Ts[i], d);
-    const int k = chem_bin(cc, log(T1 > cc.tmin ? (T1 < cc.tmax ? T1 : cc.tmax) : cc.tmin)) >> 3;
+    chem_temperature(cc, gamma_tabs(cc), ya, T, dT);
+    tdt[i] = make_double2(T, dT);
+    const int k = chem_bin(cc, log(T > cc.tmin ? (T < cc.tmax ? T : cc.tmax) : cc.tmin)) >> 3;
     key[i] = (unsigned char)k;
     atomicAdd(&hc[k], 1);
   }
@@ -782,38 +799,33 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   const size_t b_ord = sorted ? align256((size_t)n * sizeof(int)) : 0;
   const size_t b_cnt = sorted ? align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int)) : 0;
   const size_t b_key = sorted ? align256((size_t)n) : 0;
+  const size_t b_tdt = sorted ? align256((size_t)n * sizeof(double2)) : 0;
   const void *work0 = s.work;
-  if (ensure_work(s, b_rows + b_ord + b_cnt + b_key)) return -1;
+  if (ensure_work(s, b_rows + b_ord + b_cnt + b_key + b_tdt)) return -1;
   if (s.work != work0) reuse_order = false;                // a new buffer holds no order
   char *w = (char *)s.work;
   int *rows = (int *)(w);
   int *order = sorted ? (int *)(w + b_rows) : nullptr;
+  double2 *tdt = nullptr;                                  // temperatures at y_n, if solved
   if (sorted && !reuse_order) {
     int *cnt = (int *)(w + b_rows + b_ord);
     unsigned char *key = (unsigned char *)(w + b_rows + b_ord + b_cnt);
+    tdt = (double2 *)(w + b_rows + b_ord + b_cnt + b_key);
     hipLaunchKernelGGL(fill_int, dim3(2 * TORD_NBK * TORD_PAD / 256), dim3(256), 0, st,
                        (long)(2 * TORD_NBK * TORD_PAD), cnt, 0);
     const unsigned tb = (unsigned)((n + TORD_BLOCK - 1) / TORD_BLOCK);
     hipLaunchKernelGGL(tord_count, dim3(tb), dim3(TORD_BLOCK), 0, st, n, s.cc, ptrs_d(s),
-                       (const double *)y, key, cnt);
+                       (const double *)y, key, cnt, tdt);
     hipLaunchKernelGGL(tord_scatter, dim3(tb), dim3(TORD_BLOCK), 0, st, n,
                        (const unsigned char *)key, cnt, order);
   }
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
   const bool sub = o.max_halvings > 0;
   const unsigned lds = (unsigned)dirk_lds_bytes(wpb, o.nstage);
-  if (force && sub)
-    hipLaunchKernelGGL((chem_dirk_batched<true, true>), dim3((unsigned)grid), dim3(block), lds, st,
-                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, s.counters, rows);
-  else if (force)
-    hipLaunchKernelGGL((chem_dirk_batched<true, false>), dim3((unsigned)grid), dim3(block), lds, st,
-                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, s.counters, rows);
-  else if (sub)
-    hipLaunchKernelGGL((chem_dirk_batched<false, true>), dim3((unsigned)grid), dim3(block), lds, st,
-                       n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, s.counters, rows);
-  else
-    hipLaunchKernelGGL((chem_dirk_batched<false, false>), dim3((unsigned)grid), dim3(block), lds,
-                       st, n, s.cc, ptrs_d(s), o, h, y, force, (const int *)order, s.counters, rows);
+  auto kern = force ? (sub ? chem_dirk_batched<true, true> : chem_dirk_batched<true, false>)
+                    : (sub ? chem_dirk_batched<false, true> : chem_dirk_batched<false, false>);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, st, n, s.cc, ptrs_d(s), o, h,
+                     y, force, (const int *)order, (const double2 *)tdt, s.counters, rows);
   hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(nrow)), dim3(256), 0, st,
                      (const int *)rows, nrow, s.counters + 1);
   return hipGetLastError() == hipSuccess ? 0 : -1;
