@@ -9,16 +9,17 @@
 //   fe_xyface_div  : x- and y-faces of 16 x 15-cell plane tiles from one LDS staging of the
 //                    tile and its x/y halos, and their part of the divergence into wdot
 //                    (the x and y face fluxes stay in LDS); the legal-state check;
-//   fe_zface_div   : z-faces (tiles of 16 x-cells by 16 faces, their primitives staged in
-//                    LDS), and the divergence of the 15 cells between a tile's 16 faces
-//                    (tiles overlap by one face), adding the x/y part read back from wdot.
+//   fe_zface_div_xy: z-faces (tiles of 16 x-cells by 32 faces, two per thread, their
+//                    primitives staged in LDS), and the divergence of the 31 cells between
+//                    a tile's 32 faces (tiles overlap by one face), adding the x/y part
+//                    read from wdot at the kernel's start.
 // Fast path with tracers (three launches):
 //   fe_xface       : x-faces, one thread per face; a block stages the primitives of the
 //                    points its faces need in LDS once, then every thread reads its
 //                    6-point stencil from LDS; also performs the legal-state check;
 //   fe_yface       : y-faces the same way, tiles of 16 x-cells by 16 faces;
-//   fe_zface_div   : z-faces as above, reading the x- and y-face fluxes for the
-//                    divergence, and writes wdot.
+//   fe_zface_div   : z-faces (tiles of 16 x-cells by 16 faces) with the species, reading
+//                    the x- and y-face fluxes for the divergence, and writes wdot.
 // The divergence is accumulated as -((dFx/dx + dFy/dy) + dFz/dz), the reference's
 // association (utilities.cpp:202-207).
 #include "mvd_device.hpp"
@@ -413,6 +414,90 @@ fe_zface_div(Grid g, Fields f, const double *Fx, const double *Fy, FieldsOut out
   }
 }
 
+// z-faces without tracers (nchem = 0), adding their part to the x/y part that
+// fe_xyface_div left in out. A block of 16 x 16 threads covers 16 x-cells of row j and
+// ZF = 16 FPT consecutive z-faces (FPT per thread, 16 apart), i.e. the ZF - 1 cells between
+// them; the ZF + 5 points of each column are converted once into LDS. The x/y part of the
+// thread's cells is loaded at the start, so that its latency is spent under the staging and
+// the face fluxes instead of after them.
+template <int FPT>
+__global__ void __launch_bounds__(256)
+fe_zface_div_xy(Grid g, Fields f, FieldsOut out, double idz) {
+  constexpr int TX = 16, ZF = 16 * FPT, TPTS = ZF + 5, TC = ZF - 1;
+  constexpr int FS = TPTS * TX;
+  __shared__ double s_pt[9 * FS];
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int nxc = (g.nx + TX - 1) / TX, ncc = (g.nz + TC - 1) / TC;
+  int b = blockIdx.x;
+  const int xc = b % nxc;
+  b /= nxc;
+  const int cc = b % ncc;
+  const int j = b / ncc;
+  const int k0 = cc * TC;
+  const int i = xc * TX + tx;
+  const long plane = (long)g.nx * g.ny;
+  // x/y part of the thread's cells (k0 + ty + 16 q)
+  double dxy[FPT][5];
+#pragma unroll
+  for (int q = 0; q < FPT; q++) {
+    const int c = ty + 16 * q, k = k0 + c;
+    const long cell = i + (long)g.nx * j + plane * k;
+    const bool own = c < TC && i < g.nx && k < g.nz;
+#pragma unroll
+    for (int a = 0; a < 5; a++) dxy[q][a] = own ? out.w[a][cell] : 0.0;
+  }
+  // points k0 - 3 .. k0 + ZF + 1 of the 16 columns
+  {
+    const int stride = g.nx * g.ny;
+    for (int s = threadIdx.x; s < TPTS * TX; s += blockDim.x) {
+      const int r = s / TX, x = s - r * TX;
+      const int ii = xc * TX + x;
+      const int e = k0 - 3 + r;
+      if (ii < g.nx && e <= g.nz + 2) {
+        double sa, sn;
+        const int c = line_cell<2>(g, ii + g.nx * j, stride, g.nz, e, sa, sn);
+        const Pt P = to_point(load_raw<2>(f, c, sa, sn), g.gamma);
+        double *pt = s_pt + r * TX + x;
+        pt[0 * FS] = P.r;  pt[1 * FS] = P.mn; pt[2 * FS] = P.m1;
+        pt[3 * FS] = P.m2; pt[4 * FS] = P.e;  pt[5 * FS] = P.p;
+        pt[6 * FS] = P.ir; pt[7 * FS] = P.a;  pt[8 * FS] = P.sr;
+      }
+    }
+  }
+  __syncthreads();
+  double Fz[FPT][5];
+#pragma unroll
+  for (int q = 0; q < FPT; q++) {
+    const int fz = ty + 16 * q;
+    if (i < g.nx && k0 + fz <= g.nz) {
+      double Fl[5], alpha;
+      face_flux_lds(s_pt + fz * TX + tx, FS, TX, g.gamma, Fl, alpha);
+      to_global<2>(Fl, Fz[q]);
+    }
+  }
+  __syncthreads();   // the point tile is dead; its LDS takes the face fluxes [comp][face][x]
+#pragma unroll
+  for (int q = 0; q < FPT; q++) {
+    const int fz = ty + 16 * q;
+    if (i < g.nx && k0 + fz <= g.nz) {
+#pragma unroll
+      for (int a = 0; a < 5; a++) s_pt[(a * ZF + fz) * TX + tx] = Fz[q][a];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < FPT; q++) {
+    const int c = ty + 16 * q, k = k0 + c;
+    if (c >= TC || i >= g.nx || k >= g.nz) continue;
+    const long cell = i + (long)g.nx * j + plane * k;
+#pragma unroll
+    for (int a = 0; a < 5; a++) {
+      const double dz = (s_pt[(a * ZF + c + 1) * TX + tx] - s_pt[(a * ZF + c) * TX + tx]) * idz;
+      out.w[a][cell] = 0.0 - (dxy[q][a] + dz);
+    }
+  }
+}
+
 // x- and y-faces of one z-plane tile together, without tracers (nchem = 0): a block covers
 // XY_TX x XY_TY cells of plane k and forms (Fx[i+1]-Fx[i])/dx + (Fy[j+1]-Fy[j])/dy for them
 // into out (the z kernel then adds the z part), so the x and y face fluxes never go to
@@ -537,6 +622,7 @@ __global__ void reset_flag(unsigned *flag) { *flag = 0u; }
 static inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 constexpr int TILE_X = 16, TILE_F = 16;
+constexpr int ZFPT = 2;     // z-faces per thread in fe_zface_div_xy
 
 template <int NC>
 static int launch_fe(const Grid &g, const Fields &f, const FieldsOut &o, FeScratch &s,
@@ -564,11 +650,11 @@ static int launch_fe(const Grid &g, const Fields &f, const FieldsOut &o, FeScrat
     hipLaunchKernelGGL(fe_xyface_div, dim3((unsigned)nxyb), dim3(256), 0, st, g, f, o,
                        1.0 / g.dx, 1.0 / g.dy, s.flag);
     tm.end(PROF_XYFACE_DIV, st);
+    constexpr int ZC = 16 * ZFPT - 1;                      // cells per z tile
     tm.begin(PROF_ZFACE_DIV, st);
-    hipLaunchKernelGGL((fe_zface_div<NC, TX, TF, true>),
-                       dim3((unsigned)(nxc * ((g.nz + TF - 2) / (TF - 1)) * g.ny)), dim3(TX * TF),
-                       0, st, g, f, (const double *)nullptr, (const double *)nullptr, o,
-                       1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz);
+    hipLaunchKernelGGL(fe_zface_div_xy<ZFPT>,
+                       dim3((unsigned)(nxc * ((g.nz + ZC - 1) / ZC) * g.ny)), dim3(256), 0, st,
+                       g, f, o, 1.0 / g.dz);
     tm.end(PROF_ZFACE_DIV, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
   }

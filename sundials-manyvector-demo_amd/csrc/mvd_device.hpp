@@ -213,51 +213,55 @@ __device__ inline Pt make_point(double r, double mn, double m1, double m2, doubl
   return q;
 }
 
-// WENO5-JS numerator/denominator pair for one component, with two algebraic savings:
+// WENO5-JS numerator/denominator pair for one component, with three algebraic savings:
 //  * beta_k = (13/12) x_k^2 + (1/4) y_k^2 = (13/12) (x_k^2 + (3/13) y_k^2); the common factor
 //    (13/12)^2 of every (eps + beta_k)^2 cancels in the weight ratio, so the weights use
-//    (eps*12/13 + x_k^2 + (3/13) y_k^2)^2;
-//  * the 1/6 of the candidate stencils is applied once to the quotient by the caller.
-// Returns num/den = 6 * (reconstructed value) split as (num, den).
+//    (eps*12/13 + x_k^2 + (3/13) y_k^2)^2, formed as two FMAs around one product;
+//  * over the common denominator s1 s2 s3 (s_k = (eps + beta_k)^2) the weights are
+//    d_k prod_{m != k} s_m, so the linear weights d_k multiply the candidate stencils
+//    (constant coefficients) instead of the products;
+//  * the candidate stencils' 1/6 is folded into the same coefficients.
+// Returns the reconstructed value as num/den. epsb is eps*12/13 for the values as given;
+// WENO5 is homogeneous in the sense WENO(l f; eps) = l WENO(f; eps / l^2), so a caller
+// that reconstructs f / l passes WENO_EPSB / l^2 and multiplies the result by l.
+constexpr double WENO_EPSB = 1e-6 * 12.0 / 13.0;
 template <bool PLUS>
 __device__ inline void weno5_nd(double f0, double f1, double f2, double f3, double f4,
-                                double &num, double &den) {
-  const double epsb = 1e-6 * 12.0 / 13.0;
+                                double epsb, double &num, double &den) {
   const double c = 3.0 / 13.0;
   const double d01 = f1 - f0, d12 = f2 - f1, d23 = f3 - f2, d34 = f4 - f3;
   const double x1 = d34 - d23, x2 = d23 - d12, x3 = d12 - d01;
   const double y1 = x1 - 2.0 * d23;         // 3 f2 - 4 f3 + f4
   const double y2 = d12 + d23;              // -(f1 - f3)
   const double y3 = x3 + 2.0 * d12;         // f0 - 4 f1 + 3 f2
-  const double b1 = epsb + fma(x1, x1, (c * y1) * y1);
-  const double b2 = epsb + fma(x2, x2, (c * y2) * y2);
-  const double b3 = epsb + fma(x3, x3, (c * y3) * y3);
+  const double b1 = fma(x1, x1, fma(c * y1, y1, epsb));
+  const double b2 = fma(x2, x2, fma(c * y2, y2, epsb));
+  const double b3 = fma(x3, x3, fma(c * y3, y3, epsb));
   const double s1 = b1 * b1, s2 = b2 * b2, s3 = b3 * b3;
-  double q1, q2, q3, d1, d3;
-  if (PLUS) {
-    q1 = 2.0 * f2 + 5.0 * f3 - f4;
-    q2 = -f1 + 5.0 * f2 + 2.0 * f3;
-    q3 = 2.0 * f0 - 7.0 * f1 + 11.0 * f2;
-    d1 = 0.3; d3 = 0.1;
-  } else {
-    q1 = 11.0 * f2 - 7.0 * f3 + 2.0 * f4;
-    q2 = 2.0 * f1 + 5.0 * f2 - f3;
-    q3 = -f0 + 5.0 * f1 + 2.0 * f2;
-    d1 = 0.1; d3 = 0.3;
+  const double s23 = s2 * s3, s13 = s1 * s3, s12 = s1 * s2;
+  double q1, q2, q3;        // d_k / 6 times the candidate stencils
+  if (PLUS) {               // d = (3, 6, 1) / 10
+    q1 = fma(0.1, f2, fma(0.25, f3, -0.05 * f4));                      // (2, 5, -1) / 20
+    q2 = fma(-0.1, f1, fma(0.5, f2, 0.2 * f3));                        // (-1, 5, 2) / 10
+    q3 = fma(1.0 / 30.0, f0, fma(-7.0 / 60.0, f1, (11.0 / 60.0) * f2)); // (2, -7, 11) / 60
+    den = fma(0.3, s23, fma(0.6, s13, 0.1 * s12));
+  } else {                  // d = (1, 6, 3) / 10
+    q1 = fma(11.0 / 60.0, f2, fma(-7.0 / 60.0, f3, (1.0 / 30.0) * f4)); // (11, -7, 2) / 60
+    q2 = fma(0.2, f1, fma(0.5, f2, -0.1 * f3));                        // (2, 5, -1) / 10
+    q3 = fma(-0.05, f0, fma(0.25, f1, 0.1 * f2));                      // (-1, 5, 2) / 20
+    den = fma(0.1, s23, fma(0.6, s13, 0.3 * s12));
   }
-  const double t1 = d1 * (s2 * s3), t2 = 0.6 * (s1 * s3), t3 = d3 * (s1 * s2);
-  num = q1 * t1 + q2 * t2 + q3 * t3;
-  den = t1 + t2 + t3;
+  num = fma(q1, s23, fma(q2, s13, q3 * s12));
 }
 
 // Sum of the f+ and f- reconstructions of one component with a single divide:
-//   n+/(6 d+) + n-/(6 d-) = (n+ d- + n- d+) / (6 d+ d-)
+//   n+/d+ + n-/d- = (n+ d- + n- d+) / (d+ d-)
 // (d+ d- is a product of four (eps+beta)^2 factors: finite for |beta| < 1e38).
 __device__ inline double weno5_pair(const double (&fp)[5], const double (&fm)[5]) {
   double np, dp, nm, dm;
-  weno5_nd<true>(fp[0], fp[1], fp[2], fp[3], fp[4], np, dp);
-  weno5_nd<false>(fm[0], fm[1], fm[2], fm[3], fm[4], nm, dm);
-  return fast_div(fma(np, dm, nm * dp), 6.0 * (dp * dm));
+  weno5_nd<true>(fp[0], fp[1], fp[2], fp[3], fp[4], WENO_EPSB, np, dp);
+  weno5_nd<false>(fm[0], fm[1], fm[2], fm[3], fm[4], WENO_EPSB, nm, dm);
+  return fast_div(fma(np, dm, nm * dp), dp * dm);
 }
 
 // Face flux with the 6-point stencil read from LDS as it is needed (point l, field q at
@@ -287,9 +291,12 @@ __device__ inline void face_flux_lds_f(const double *base, int FS, int LS, doubl
   const double hq = 0.5 * qsq;
   const double cs = gm * (H - hq);
   const double ci = fast_rcp(cs);
-  const double hci = 0.25 * ci, hci2 = 0.5 * ci;
+  const double hci = 0.25 * ci;
   // K = (gamma-1)(q^2/2 x0 - u x1 - v x2 - w x3 + x4) with the factor folded in
   const double gh = gm * hq, gu = gm * u, gv = gm * v, gw = gm * w;
+  // The characteristic values are reconstructed without their constant factors (hci for
+  // components 0 and 4, 1/2 for 1..3), with eps scaled to match (1 / hci^2 = 16 cs^2).
+  const double eps04 = (16.0 * WENO_EPSB) * (cs * cs), eps13 = 4.0 * WENO_EPSB;
   double nps[5], dps[5], ff[5];
 #pragma unroll
   for (int side = 0; side < 2; side++) {
@@ -311,31 +318,34 @@ __device__ inline void face_flux_lds_f(const double *base, int FS, int LS, doubl
       const double x4 = fma(us, e, ul * p);
       const double K = fma(gm, x4, fma(-gw, x3, fma(-gv, x2, fma(-gu, x1, gh * x0))));
       const double A = fma(u, x0, -x1);
-      pr[l][0] = hci * (A + K);
-      pr[l][1] = 0.5 * fma(-v, x0, x2);
-      pr[l][2] = 0.5 * fma(-w, x0, x3);
-      pr[l][3] = fma(-hci2, K, 0.5 * x0);
-      pr[l][4] = hci * (K - A);
+      pr[l][0] = A + K;                  // times hci
+      pr[l][1] = fma(-v, x0, x2);        // times 1/2
+      pr[l][2] = fma(-w, x0, x3);        // times 1/2
+      pr[l][3] = fma(-ci, K, x0);        // times 1/2
+      pr[l][4] = K - A;                  // times hci
     }
 #pragma unroll
     for (int c = 0; c < 5; c++) {
+      const double eps = (c == 0 || c == 4) ? eps04 : eps13;
       if (side == 0) {
-        weno5_nd<true>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c], nps[c], dps[c]);
+        weno5_nd<true>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c], eps, nps[c], dps[c]);
       } else {
         double nm, dm;
-        weno5_nd<false>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c], nm, dm);
-        ff[c] = fast_div(fma(nps[c], dm, nm * dps[c]), 6.0 * (dps[c] * dm));
+        weno5_nd<false>(pr[0][c], pr[1][c], pr[2][c], pr[3][c], pr[4][c], eps, nm, dm);
+        ff[c] = fast_div(fma(nps[c], dm, nm * dps[c]), dps[c] * dm);
       }
     }
   }
   __builtin_amdgcn_sched_barrier(0);
-  const double s04 = ff[0] + ff[4];
-  const double d40 = cs * (ff[4] - ff[0]);
-  F[0] = s04 + ff[3];
+  // the factors back: hci on 0 and 4 (cs hci = 1/4), 1/2 on 1..3
+  const double s04 = hci * (ff[0] + ff[4]);
+  const double d40 = 0.25 * (ff[4] - ff[0]);
+  const double f1 = 0.5 * ff[1], f2 = 0.5 * ff[2], f3 = 0.5 * ff[3];
+  F[0] = s04 + f3;
   F[1] = u * F[0] + d40;
-  F[2] = v * F[0] + ff[1];
-  F[3] = w * F[0] + ff[2];
-  F[4] = H * s04 + u * d40 + v * ff[1] + w * ff[2] + hq * ff[3];
+  F[2] = v * F[0] + f1;
+  F[3] = w * F[0] + f2;
+  F[4] = H * s04 + u * d40 + v * f1 + w * f2 + hq * f3;
   alpha_out = alpha;
 }
 
