@@ -446,22 +446,34 @@ fe_zface_div_xy(Grid g, Fields f, FieldsOut out, double idz) {
 #pragma unroll
     for (int a = 0; a < 5; a++) dxy[q][a] = own ? out.w[a][cell] : 0.0;
   }
-  // points k0 - 3 .. k0 + ZF + 1 of the 16 columns
+  // points k0 - 3 .. k0 + ZF + 1 of the 16 columns; a thread's points (s = t + 256 n) are
+  // located first and all their loads issued before any is converted
   {
+    constexpr int NR = (TPTS * TX + 255) / 256;
     const int stride = g.nx * g.ny;
-    for (int s = threadIdx.x; s < TPTS * TX; s += blockDim.x) {
+    int cs[NR];
+    double sa[NR], sn[NR];
+    bool use[NR];
+#pragma unroll
+    for (int n = 0; n < NR; n++) {
+      const int s = threadIdx.x + 256 * n;
       const int r = s / TX, x = s - r * TX;
-      const int ii = xc * TX + x;
       const int e = k0 - 3 + r;
-      if (ii < g.nx && e <= g.nz + 2) {
-        double sa, sn;
-        const int c = line_cell<2>(g, ii + g.nx * j, stride, g.nz, e, sa, sn);
-        const Pt P = to_point(load_raw<2>(f, c, sa, sn), g.gamma);
-        double *pt = s_pt + r * TX + x;
-        pt[0 * FS] = P.r;  pt[1 * FS] = P.mn; pt[2 * FS] = P.m1;
-        pt[3 * FS] = P.m2; pt[4 * FS] = P.e;  pt[5 * FS] = P.p;
-        pt[6 * FS] = P.ir; pt[7 * FS] = P.a;  pt[8 * FS] = P.sr;
-      }
+      use[n] = s < TPTS * TX && xc * TX + x < g.nx && e <= g.nz + 2;
+      cs[n] = use[n] ? line_cell<2>(g, xc * TX + x + g.nx * j, stride, g.nz, e, sa[n], sn[n]) : 0;
+    }
+    RawPt raw[NR];
+#pragma unroll
+    for (int n = 0; n < NR; n++)
+      if (use[n]) raw[n] = load_raw<2>(f, cs[n], sa[n], sn[n]);
+#pragma unroll
+    for (int n = 0; n < NR; n++) {
+      if (!use[n]) continue;
+      const Pt P = to_point(raw[n], g.gamma);
+      double *pt = s_pt + threadIdx.x + 256 * n;
+      pt[0 * FS] = P.r;  pt[1 * FS] = P.mn; pt[2 * FS] = P.m1;
+      pt[3 * FS] = P.m2; pt[4 * FS] = P.e;  pt[5 * FS] = P.p;
+      pt[6 * FS] = P.ir; pt[7 * FS] = P.a;  pt[8 * FS] = P.sr;
     }
   }
   __syncthreads();
@@ -528,48 +540,64 @@ fe_xyface_div(Grid g, Fields f, FieldsOut out, double idx, double idy, unsigned 
   const int k = b / nyb;
   const int x0 = xb * XY_TX, y0 = yb * XY_TY;
   // stage the cross: points of the tile's rows (real y) over the x window, with x ghosts,
-  // then the other points of the tile's columns (real x) over the y window, with y ghosts
-  for (int q = t; q < XY_NP; q += blockDim.x) {
+  // then the other points of the tile's columns (real x) over the y window, with y ghosts.
+  // A thread's points (q = t + 256 r) are located first and all their loads issued before
+  // any is converted, so that their latencies overlap.
+  constexpr int NR = (XY_NP + 255) / 256;
+  int cq[NR];
+  double s_all[NR], s_mx[NR], s_my[NR];
+  bool use[NR], own[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int q = t + 256 * r;
     const int yy = q / XY_PX, xx = q - yy * XY_PX;
     const int ex = x0 - 3 + xx, ey = y0 - 3 + yy;
     const bool row = yy >= 3 && yy < 3 + XY_TY && ey < g.ny;          // x-line point
     const bool col = xx >= 3 && xx < 3 + XY_TX && ex < g.nx;          // y-column point
     int i = ex, j = ey;
-    double s_all = 1.0, s_mx = 1.0, s_my = 1.0;
+    s_all[r] = s_mx[r] = s_my[r] = 1.0;
+    use[r] = q < XY_NP && (row ? ex <= g.nx + 2 : (col && ey <= g.ny + 2));
+    own[r] = row && col && ex >= 0;   // a cell of the tile: legal_state once
     if (row) {
-      if (ex > g.nx + 2) continue;
       if (ex < 0 || ex >= g.nx) {
         const GhostMap m = ghost_map(g, 0, ex);
         i = m.src;
-        s_all = s_my = m.sgn_all;
-        s_mx = m.sgn_norm;
+        s_all[r] = s_my[r] = m.sgn_all;
+        s_mx[r] = m.sgn_norm;
       }
     } else if (col) {
-      if (ey > g.ny + 2) continue;
       if (ey < 0 || ey >= g.ny) {
         const GhostMap m = ghost_map(g, 1, ey);
         j = m.src;
-        s_all = s_mx = m.sgn_all;
-        s_my = m.sgn_norm;
+        s_all[r] = s_mx[r] = m.sgn_all;
+        s_my[r] = m.sgn_norm;
       }
-    } else {
-      continue;
     }
-    const long c = i + (long)g.nx * (j + (long)g.ny * k);
-    const double r = f.w[0][c] * s_all, mx = f.w[1][c] * s_mx, my = f.w[2][c] * s_my;
-    const double mz = f.w[3][c] * s_all, e = f.w[4][c] * s_all;
-    const double ir = fast_rcp(r);
+    cq[r] = use[r] ? (int)(i + (long)g.nx * (j + (long)g.ny * k)) : 0;
+  }
+  double raw[NR][5];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+#pragma unroll
+    for (int a = 0; a < 5; a++) raw[r][a] = use[r] ? f.w[a][cq[r]] : 1.0;
+  }
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    if (!use[r]) continue;
+    const double rr = raw[r][0] * s_all[r], mx = raw[r][1] * s_mx[r], my = raw[r][2] * s_my[r];
+    const double mz = raw[r][3] * s_all[r], e = raw[r][4] * s_all[r];
+    const double ir = fast_rcp(rr);
     const double p = (g.gamma - 1.0) * (e - 0.5 * (mx * mx + my * my + mz * mz) * ir);
     const double cs = sun_sqrt(g.gamma * p * ir);
-    double *pt = sh + q;
-    pt[0 * XY_NP] = r;  pt[1 * XY_NP] = mx; pt[2 * XY_NP] = my;
+    double *pt = sh + t + 256 * r;
+    pt[0 * XY_NP] = rr; pt[1 * XY_NP] = mx; pt[2 * XY_NP] = my;
     pt[3 * XY_NP] = mz; pt[4 * XY_NP] = e;  pt[5 * XY_NP] = p;
     pt[6 * XY_NP] = ir; pt[7 * XY_NP] = fabs(mx * ir) + cs;
-    pt[8 * XY_NP] = sun_sqrt(r);
+    pt[8 * XY_NP] = sun_sqrt(rr);
     pt[9 * XY_NP] = fabs(my * ir) + cs;
-    if (row && col && ex >= 0) {      // a cell of the tile: legal_state once
+    if (own[r]) {
       unsigned bits = 0;
-      if (!(r > 0.0)) bits |= 1u;
+      if (!(rr > 0.0)) bits |= 1u;
       if (!(e > 0.0)) bits |= 2u;
       if (!(p > 0.0)) bits |= 4u;
       if (bits) atomicOr(flag, bits);
