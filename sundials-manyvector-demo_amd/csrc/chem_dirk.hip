@@ -730,14 +730,16 @@ tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, unsigned char *ke
            double2 *tdt) {
 #pragma clang fp contract(off)
   __shared__ int hc[TORD_NBK];
+  __shared__ double gl[4 * TLEN];        // the gamma tables of the temperature solve
   if (threadIdx.x < TORD_NBK) hc[threadIdx.x] = 0;
+  for (int e = threadIdx.x; e < 4 * TLEN; e += blockDim.x) gl[e] = cc.tab[G_GH2_1 * TLEN + e];
   __syncthreads();
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     double ya[NSP], T = p.Ts[i], dT = p.dTs[i];
 #pragma unroll
     for (int s = 0; s < NSP; s++) ya[s] = y[NSP * i + s] * p.scale[NSP * i + s];
-    chem_temperature(cc, gamma_tabs(cc), ya, T, dT);
+    chem_temperature(cc, gamma_tabs(gl), ya, T, dT);
     tdt[i] = make_double2(T, dT);
     const int k = chem_bin(cc, log(T > cc.tmin ? (T < cc.tmax ? T : cc.tmax) : cc.tmin)) >> 3;
     key[i] = (unsigned char)k;
