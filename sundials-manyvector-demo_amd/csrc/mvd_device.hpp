@@ -220,7 +220,8 @@ __device__ inline Pt make_point(double r, double mn, double m1, double m2, doubl
 //  * over the common denominator s1 s2 s3 (s_k = (eps + beta_k)^2) the weights are
 //    d_k prod_{m != k} s_m, so the linear weights d_k multiply the candidate stencils
 //    (constant coefficients) instead of the products;
-//  * the candidate stencils' 1/6 is folded into the same coefficients.
+//  * the candidate stencils' 1/6 is folded into the same coefficients, and each stencil is
+//    written as its f2 part plus differences already formed for the smoothness indicators.
 // Returns the reconstructed value as num/den. epsb is eps*12/13 for the values as given;
 // WENO5 is homogeneous in the sense WENO(l f; eps) = l WENO(f; eps / l^2), so a caller
 // that reconstructs f / l passes WENO_EPSB / l^2 and multiplies the result by l.
@@ -239,19 +240,21 @@ __device__ inline void weno5_nd(double f0, double f1, double f2, double f3, doub
   const double b3 = fma(x3, x3, fma(c * y3, y3, epsb));
   const double s1 = b1 * b1, s2 = b2 * b2, s3 = b3 * b3;
   const double s23 = s2 * s3, s13 = s1 * s3, s12 = s1 * s2;
-  double q1, q2, q3;        // d_k / 6 times the candidate stencils
+  // d_k / 6 times the candidate stencils, each written as (6 d_k / 6) f2 plus differences:
+  // the f2 terms add up to f2 den, so num = f2 den + sum_k q'_k prod_{m != k} s_m
+  double q1, q2, q3;
   if (PLUS) {               // d = (3, 6, 1) / 10
-    q1 = fma(0.1, f2, fma(0.25, f3, -0.05 * f4));                      // (2, 5, -1) / 20
-    q2 = fma(-0.1, f1, fma(0.5, f2, 0.2 * f3));                        // (-1, 5, 2) / 10
-    q3 = fma(1.0 / 30.0, f0, fma(-7.0 / 60.0, f1, (11.0 / 60.0) * f2)); // (2, -7, 11) / 60
+    q1 = fma(0.2, d23, -0.05 * d34);                       // (2 f2 + 5 f3 - f4) / 20
+    q2 = fma(0.1, d12, 0.2 * d23);                         // (-f1 + 5 f2 + 2 f3) / 10
+    q3 = fma(1.0 / 12.0, d12, (-1.0 / 30.0) * d01);        // (2 f0 - 7 f1 + 11 f2) / 60
     den = fma(0.3, s23, fma(0.6, s13, 0.1 * s12));
   } else {                  // d = (1, 6, 3) / 10
-    q1 = fma(11.0 / 60.0, f2, fma(-7.0 / 60.0, f3, (1.0 / 30.0) * f4)); // (11, -7, 2) / 60
-    q2 = fma(0.2, f1, fma(0.5, f2, -0.1 * f3));                        // (2, 5, -1) / 10
-    q3 = fma(-0.05, f0, fma(0.25, f1, 0.1 * f2));                      // (-1, 5, 2) / 20
+    q1 = fma(-1.0 / 12.0, d23, (1.0 / 30.0) * d34);        // (11 f2 - 7 f3 + 2 f4) / 60
+    q2 = fma(-0.2, d12, -0.1 * d23);                       // (2 f1 + 5 f2 - f3) / 10
+    q3 = fma(-0.2, d12, 0.05 * d01);                       // (-f0 + 5 f1 + 2 f2) / 20
     den = fma(0.1, s23, fma(0.6, s13, 0.3 * s12));
   }
-  num = fma(q1, s23, fma(q2, s13, q3 * s12));
+  num = fma(f2, den, fma(q1, s23, fma(q2, s13, q3 * s12)));
 }
 
 // Sum of the f+ and f- reconstructions of one component with a single divide:
