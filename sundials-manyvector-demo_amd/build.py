@@ -1,9 +1,12 @@
-"""Build libmvd_rhs.so (HIP kernels + C ABI) for gfx950, in-tree.
+"""Build libmvd_rhs.so (HIP kernels + C ABI) for gfx950, and libmvd_io.so (HDF5 solution
+output / restart input, host code), in-tree.
 
     python sundials-manyvector-demo_amd/build.py [--jobs N]
 
 Each .hip translation unit is compiled with hipcc for gfx950 only, then linked into
 sundials-manyvector-demo_amd/libmvd_rhs.so. Objects go to sundials-manyvector-demo_amd/_build/.
+csrc/io.cpp is compiled with g++ against the serial HDF5 of the image (HDF5_DIR, default
+/opt/conda) and the HIP runtime into libmvd_io.so; without HDF5 it is skipped.
 """
 import argparse
 import os
@@ -19,6 +22,9 @@ SOURCES = ["api.hip", "fe.hip", "fe_vjp.hip", "nvec.hip", "chem.hip", "chem_dirk
            "erk.hip", "mri.hip", "exchange.hip",
            "imex.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+IO_LIB = os.path.join(HERE, "libmvd_io.so")
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 # -packed-fp32-ops: never form packed FP32 instructions (all arithmetic here is FP64).
 FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17", "-Wall",
@@ -51,7 +57,31 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_io()
     return LIB
+
+
+def build_io():
+    """libmvd_io.so, or None when the HDF5 headers are absent."""
+    if not os.path.exists(os.path.join(HDF5_DIR, "include", "hdf5.h")):
+        print(f"libmvd_io.so skipped: no hdf5.h under {HDF5_DIR}/include", file=sys.stderr)
+        return None
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    tmp = IO_LIB + ".tmp"
+    # libstdc++ linked in statically: the HDF5 directory on the run path may hold an older
+    # libstdc++ than the HIP runtime needs, and this library must not pull that one in
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-static-libstdc++", "-static-libgcc",
+           "-I", inc, "-I", os.path.join(HDF5_DIR, "include"), "-I", os.path.join(ROCM, "include"),
+           os.path.join(CSRC, "io.cpp"), "-o", tmp,
+           "-L", os.path.join(HDF5_DIR, "lib"), "-lhdf5",
+           "-Wl,-rpath," + os.path.join(HDF5_DIR, "lib"),
+           "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"libmvd_io.so build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 if __name__ == "__main__":

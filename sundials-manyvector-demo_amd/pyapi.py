@@ -130,11 +130,107 @@ def lib():
     return L
 
 
-def exported_symbols_declared():
-    """Names declared in include/mvd_rhs.h (for the ABI completeness test)."""
+def exported_symbols_declared(header=HEADER_PATH):
+    """Names declared in a header of include/ (for the ABI completeness tests)."""
     import re
-    txt = open(HEADER_PATH).read()
+    txt = open(header).read()
     return sorted(set(re.findall(r"\b(mvd_[a-z0-9_]+)\s*\(", txt)))
+
+
+# ---------------------------------------------------------------------------------------
+# solution output / restart input (libmvd_io.so, include/mvd_io.h)
+# ---------------------------------------------------------------------------------------
+IO_LIB_PATH = os.path.join(HERE, "libmvd_io.so")
+IO_HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "mvd_io.h")
+MVD_IO_ERROR, MVD_IO_MISMATCH = -6, -7
+
+
+class RunOptions(C.Structure):
+    """mvd_run_options: the driver/ARKODE options restart_parameters.txt records; defaults
+    are ARKODEParameters' (euler3D.hpp:163-168) with the NONE table ids (-1)."""
+    _fields_ = [("tf", C.c_double), ("nout", C.c_int), ("showstats", C.c_int),
+                ("order", C.c_int), ("dense_order", C.c_int), ("etable", C.c_int),
+                ("itable", C.c_int), ("mtable", C.c_int), ("adapt_method", C.c_int),
+                ("maxnef", C.c_int), ("mxhnil", C.c_int), ("mxsteps", C.c_int),
+                ("safety", C.c_double), ("bias", C.c_double), ("growth", C.c_double),
+                ("pq", C.c_int), ("k1", C.c_double), ("k2", C.c_double), ("k3", C.c_double),
+                ("etamx1", C.c_double), ("etamxf", C.c_double), ("hmin", C.c_double),
+                ("hmax", C.c_double), ("fixedstep", C.c_int), ("htrans", C.c_double),
+                ("rtol", C.c_double), ("atol", C.c_double), ("fusedkernels", C.c_int),
+                ("localreduce", C.c_int)]
+
+    def __init__(self, **kw):
+        d = dict(tf=1.0, nout=10, showstats=0, order=4, dense_order=-1, etable=-1, itable=-1,
+                 mtable=-1, mxsteps=5000, rtol=1e-8, atol=1e-12, fusedkernels=1,
+                 localreduce=1)
+        d.update(kw)
+        super().__init__(**d)
+
+
+_io = None
+
+
+def io_lib():
+    """Load libmvd_io.so and declare its ABI; raises if it has not been built."""
+    global _io
+    if _io is not None:
+        return _io
+    if not os.path.exists(IO_LIB_PATH):
+        raise RuntimeError(f"{IO_LIB_PATH} missing: run `python sundials-manyvector-demo_amd/build.py`")
+    L = C.CDLL(IO_LIB_PATH)
+    PR, RO = C.POINTER(Params), C.POINTER(RunOptions)
+    L.mvd_write_parameters.argtypes = [C.c_char_p, PR, RO, C.c_double, C.c_double, C.c_int]
+    L.mvd_output_solution.argtypes = [C.c_char_p, PR, RO, C.c_double, C.c_double, C.c_int,
+                                      _PP, C.c_int]
+    L.mvd_read_restart.argtypes = [C.c_char_p, PR, C.c_int, C.POINTER(C.c_double), _PP,
+                                   C.c_int]
+    for f in (L.mvd_write_parameters, L.mvd_output_solution, L.mvd_read_restart):
+        f.restype = C.c_int
+    _io = L
+    return L
+
+
+def _field_ptrs(w):
+    """(pointer array, on_device) for six float64 arrays (numpy on the host or torch on the
+    device; w[5] may be None without species)."""
+    import numpy as np
+    dev = any(x is not None and not isinstance(x, np.ndarray) for x in w)
+    ps = []
+    for x in w:
+        if x is None:
+            ps.append(_P())
+        elif isinstance(x, np.ndarray):
+            assert not dev and x.dtype == np.float64 and x.flags.c_contiguous
+            ps.append(x.ctypes.data_as(_P))
+        else:
+            ps.append(dptr(x))
+    return C.cast((_P * len(ps))(*ps), _PP), int(dev)
+
+
+def output_solution(directory, params, t, h, iout, w, options=None):
+    """output_solution: restart_parameters.txt (when options is given) and
+    output-<iout>.hdf5 in directory from w (numpy host arrays or torch device tensors)."""
+    pp, dev = _field_ptrs(w)
+    rc = io_lib().mvd_output_solution(str(directory).encode(), C.byref(params),
+                                      C.byref(options) if options is not None else None,
+                                      t, h, iout, pp, dev)
+    return _check("mvd_output_solution", rc)
+
+
+def read_restart(directory, params, restart, w):
+    """read_restart into w (overwritten); returns the stored time."""
+    pp, dev = _field_ptrs(w)
+    t = C.c_double(0.0)
+    rc = io_lib().mvd_read_restart(str(directory).encode(), C.byref(params), restart,
+                                   C.byref(t), pp, dev)
+    _check("mvd_read_restart", rc)
+    return t.value
+
+
+def write_parameters(directory, params, options, t, h, iout):
+    rc = io_lib().mvd_write_parameters(str(directory).encode(), C.byref(params),
+                                       C.byref(options), t, h, iout)
+    return _check("mvd_write_parameters", rc)
 
 
 def dptr(t):
