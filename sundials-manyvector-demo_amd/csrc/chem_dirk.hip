@@ -383,6 +383,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     const int base = next;
     if ((long)base >= n) break;
     int q = 0;
+    if (SUBSTEP && lane == 0) q = atomicAdd(queue, 64) + nwaves * 64;
     if ((long)base + lane < n) {   // lanes past the end idle to the end of the chunk
     // the lane's cell: queue positions run over the cells in temperature order
     const int cell = order ? order[base + lane] : base + lane;
@@ -392,7 +393,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
       const long i = cell;
       Ts0 = p.Ts[i];                                       // warm start at the substep start
       dTs0 = p.dTs[i];
-      if (tdt) {                                           // solved at y_n by the ordering pass
+      if (!SUBSTEP && tdt) {                               // solved at y_n by the ordering pass
         const double2 t = tdt[i];
         Ts = t.x;
         dTs = t.y;
@@ -418,9 +419,12 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     //   RHS(y_n), setup; per stage: RHS(z), Newton update, ... ; after max_iters without
     //   convergence: RHS(z), setup, and the stage's Newton loop restarts.
     // the next chunk's queue position, requested once the cell's loads are in flight (lane
-    // 0 is active whenever the chunk is)
-    if (lane == 0) q = atomicAdd(queue, 64) + nwaves * 64;
-    bool setup = true, ok = true, solved = tdt != nullptr;
+    // 0 is active whenever the chunk is; the substepping variants request it at the chunk's
+    // start, which costs them fewer registers)
+    if (!SUBSTEP && lane == 0) q = atomicAdd(queue, 64) + nwaves * 64;
+    // (the substepping variants, which hold more state, solve it again: skipping there
+    // costs registers that are worth more than the solve)
+    bool setup = true, ok = true, solved = !SUBSTEP && tdt != nullptr;
     int st = 0, it = 0, setups = 0, nit = 0, nsetup = 0;
     double hs = h;                                         // current (sub)step
     double hg = h * o.A[0];                                // SDIRK: one diagonal value
@@ -827,7 +831,8 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   auto kern = force ? (sub ? chem_dirk_batched<true, true> : chem_dirk_batched<true, false>)
                     : (sub ? chem_dirk_batched<false, true> : chem_dirk_batched<false, false>);
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, st, n, s.cc, ptrs_d(s), o, h,
-                     y, force, (const int *)order, (const double2 *)tdt, s.counters, rows);
+                     y, force, (const int *)order, sub ? nullptr : (const double2 *)tdt,
+                     s.counters, rows);
   hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(nrow)), dim3(256), 0, st,
                      (const int *)rows, nrow, s.counters + 1);
   return hipGetLastError() == hipSuccess ? 0 : -1;
