@@ -47,8 +47,9 @@ int mvd_write_parameters(const char *dir, const mvd_params *p, const mvd_run_opt
                          double tcur, double hcur, int iout);
 
 /* output_solution (io.cpp:723-941): the parameter file (when o is not NULL), then
- * dir/output-<iout>.hdf5. on_device: w holds device pointers (copied to the host through
- * the null stream; the call waits for them), else host pointers. w is not modified.
+ * dir/output-<iout>.hdf5. on_device: w holds device pointers, copied to the host with
+ * hipMemcpy (ordered after work on the null stream only: work producing w on another
+ * stream must have completed), else host pointers. w is not modified.
  * Returns 0, MVD_BAD_ARGUMENT, MVD_DEVICE_ERROR or MVD_IO_ERROR. */
 int mvd_output_solution(const char *dir, const mvd_params *p, const mvd_run_options *o,
                         double tcur, double hcur, int iout, const double *const *w,
@@ -57,7 +58,9 @@ int mvd_output_solution(const char *dir, const mvd_params *p, const mvd_run_opti
 /* read_restart (io.cpp:946-1163): t and w from dir/output-<restart>.hdf5, fluid fields
  * converted back to code units (times 1/Units). Rejects (MVD_IO_MISMATCH) a file whose
  * "time" is not double, whose nchem differs, whose domain differs by more than 1e-14 or
- * whose Density is not [nz][ny][nx]. */
+ * whose fields are not [nz][ny][nx]. The metadata are checked before any field is read;
+ * a field that fails later leaves the fields before it overwritten (the reference zeroes
+ * w first). t is set only on success. */
 int mvd_read_restart(const char *dir, const mvd_params *p, int restart, double *t,
                      double *const *w, int on_device);
 
