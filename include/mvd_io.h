@@ -1,5 +1,5 @@
 /* Solution output, restart input and the restart parameter file in the reference's on-disk
- * format (io.cpp:639-1163), for one rank: libmvd_io.so (host code over serial HDF5; kept
+ * format (io.cpp:645-1162), for one rank: libmvd_io.so (host code over serial HDF5; kept
  * apart from libmvd_rhs.so so that the RHS library does not depend on HDF5).
  *
  * output-%07d.hdf5 holds the datasets
@@ -40,9 +40,9 @@ typedef struct {
   int fusedkernels, localreduce;
 } mvd_run_options;
 
-/* write_parameters (io.cpp:644-714): dir/restart_parameters.txt with t0 = tcur, h0 = hcur,
+/* write_parameters (io.cpp:645-714): dir/restart_parameters.txt with t0 = tcur, h0 = hcur,
  * nout = o->nout - iout and restart = iout, doubles as %.16e. dir NULL: the working
- * directory. Returns 0 or MVD_IO_ERROR. */
+ * directory. Returns 0, MVD_BAD_ARGUMENT or MVD_IO_ERROR. */
 int mvd_write_parameters(const char *dir, const mvd_params *p, const mvd_run_options *o,
                          double tcur, double hcur, int iout);
 
@@ -55,7 +55,7 @@ int mvd_output_solution(const char *dir, const mvd_params *p, const mvd_run_opti
                         double tcur, double hcur, int iout, const double *const *w,
                         int on_device);
 
-/* read_restart (io.cpp:946-1163): t and w from dir/output-<restart>.hdf5, fluid fields
+/* read_restart (io.cpp:946-1162): t and w from dir/output-<restart>.hdf5, fluid fields
  * converted back to code units (times 1/Units). Rejects (MVD_IO_MISMATCH) a file whose
  * "time" is not double, whose nchem differs, whose domain differs by more than 1e-14 or
  * whose fields are not [nz][ny][nx]. The metadata are checked before any field is read;

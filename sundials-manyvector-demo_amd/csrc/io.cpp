@@ -1,5 +1,5 @@
 // Solution output / restart input (serial HDF5) and the restart parameter file, in the
-// reference's format (io.cpp:644-1163), for one rank. Host code: device fields are staged
+// reference's format (io.cpp:645-1162), for one rank. Host code: device fields are staged
 // through host buffers with synchronous copies.
 #include <hdf5.h>
 #include <hip/hip_runtime_api.h>
