@@ -1,0 +1,50 @@
+"""bench.py's host-side contract, without a GPU: the one-GPU guard, the algorithmic FP64
+operation counts behind the roofline line, and the committed HBM-traffic summary it reads."""
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def load_bench():
+    spec = importlib.util.spec_from_file_location("mvd_bench", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "--gpus 1" in (r.stdout + r.stderr)
+
+
+def test_fe_operation_counts():
+    b = load_bench()
+    n = 256
+    faces, cells = (n + 1) * n * n, n ** 3
+    # per face: the reference face_flux's 1543 operations (nchem = 0); per cell and field the
+    # divergence's 9, split 5 (x/y kernel) + 4 (z kernel)
+    assert b.FACE_FLOPS[0] == 1543 and b.DIV_FLOPS_XY + b.DIV_FLOPS_Z == b.DIV_FLOPS_PER_VAR == 9
+    xy = b.kernel_flops("fe_xyface_div", n)
+    z = b.kernel_flops("fe_zface_div", n)
+    assert xy == 2 * faces * 1543 + cells * 5 * 5 == 52396163072
+    assert z == faces * 1543 + cells * 5 * 4
+    # the two kernels together do exactly what the plain faces + divergence kernels count
+    assert xy + z == b.kernel_flops("fe_plain_faces", n) + b.kernel_flops("fe_plain_div", n)
+
+
+def test_committed_traffic_summary():
+    b = load_bench()
+    path = os.path.join(ROOT, "profiles", "fe_256_pmc.json")
+    d = json.load(open(path))
+    t = b.read_traffic("fe_xyface_div")
+    k = d["kernels"]["fe_xyface_div"]
+    assert t == k["hbm_bytes_per_launch"]
+    alg = k["algorithmic_read_bytes"] + k["algorithmic_write_bytes"]
+    assert alg == 2 * 5 * 8 * 256 ** 3
+    assert alg <= t <= 1.25 * alg          # halo re-reads only, no spill traffic
