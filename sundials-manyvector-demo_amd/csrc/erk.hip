@@ -28,7 +28,6 @@ mv_lincomb(long len, const double *y, int nt, Coefs c, const double *K, long kst
   z[i] = acc;
 }
 
-__global__ void or_flag(const unsigned *src, unsigned *dst) { *dst |= *src; }
 __global__ void zero_flag(unsigned *dst) { *dst = 0u; }
 
 inline Fields fields_at(const double *base, long ncell, int nchem) {
@@ -69,10 +68,11 @@ int enqueue_step(const Grid &g, FeScratch &fe, ErkState &e, const ErkTable &tab,
                          (const double *)e.K, len, e.Z);
       zin = e.Z;
     }
+    // the stage's legal-state bits go straight into the run's flag
     const int rc = fe_launch(g, fields_at(zin, ncell, g.nchem),
-                             fields_out_at(e.K + (long)i * len, ncell, g.nchem), fe, st, false);
+                             fields_out_at(e.K + (long)i * len, ncell, g.nchem), fe, st, false,
+                             e.flag);
     if (rc) return rc;
-    hipLaunchKernelGGL(or_flag, dim3(1), dim3(1), 0, st, (const unsigned *)fe.flag, e.flag);
   }
   Coefs c{};
   for (int j = 0; j < tab.stages; j++) c.c[j] = h * tab.b[j];

@@ -654,14 +654,15 @@ constexpr int ZFPT = 2;     // z-faces per thread in fe_zface_div_xy
 
 template <int NC>
 static int launch_fe(const Grid &g, const Fields &f, const FieldsOut &o, FeScratch &s,
-                     hipStream_t st, bool plain) {
+                     hipStream_t st, bool plain, unsigned *accum) {
   const long nxp = g.nx + 1, nyp = g.ny + 1, nzp = g.nz + 1;
-  hipLaunchKernelGGL(reset_flag, dim3(1), dim3(1), 0, st, s.flag);
+  unsigned *const flag = accum ? accum : s.flag;
+  if (!accum) hipLaunchKernelGGL(reset_flag, dim3(1), dim3(1), 0, st, s.flag);
   KernelTimer &tm = s.timer;
   if (plain) {
     tm.begin(PROF_PLAIN_FACES, st);
     hipLaunchKernelGGL(fe_plain_faces<NC>, dim3(nblocks(nxp * nyp * nzp, 256)), dim3(256), 0, st,
-                       g, f, s.Fx, s.Fy, s.Fz, s.flag);
+                       g, f, s.Fx, s.Fy, s.Fz, flag);
     tm.end(PROF_PLAIN_FACES, st);
     tm.begin(PROF_PLAIN_DIV, st);
     hipLaunchKernelGGL(fe_plain_div<NC>, dim3(nblocks((long)g.nx * g.ny * g.nz, 256)), dim3(256),
@@ -676,7 +677,7 @@ static int launch_fe(const Grid &g, const Fields &f, const FieldsOut &o, FeScrat
     const long nxyb = ((long)(g.nx + XY_TX - 1) / XY_TX) * ((g.ny + XY_TY - 1) / XY_TY) * g.nz;
     tm.begin(PROF_XYFACE_DIV, st);
     hipLaunchKernelGGL(fe_xyface_div, dim3((unsigned)nxyb), dim3(256), 0, st, g, f, o,
-                       1.0 / g.dx, 1.0 / g.dy, s.flag);
+                       1.0 / g.dx, 1.0 / g.dy, flag);
     tm.end(PROF_XYFACE_DIV, st);
     constexpr int ZC = 16 * ZFPT - 1;                      // cells per z tile
     tm.begin(PROF_ZFACE_DIV, st);
@@ -690,7 +691,7 @@ static int launch_fe(const Grid &g, const Fields &f, const FieldsOut &o, FeScrat
   const long nrowblk = ((long)g.ny * g.nz + XROWS - 1) / XROWS;
   tm.begin(PROF_XFACE, st);
   hipLaunchKernelGGL(fe_xface<NC>, dim3((unsigned)(nchunk * nrowblk)), dim3(64 * XROWS), 0, st,
-                     g, f, s.Fx, s.flag);
+                     g, f, s.Fx, flag);
   tm.end(PROF_XFACE, st);
   tm.begin(PROF_YFACE, st);
   hipLaunchKernelGGL((fe_yface<NC, TX, TF>), dim3((unsigned)(nxc * ((nyp + TF - 1) / TF) * g.nz)),
@@ -705,16 +706,16 @@ static int launch_fe(const Grid &g, const Fields &f, const FieldsOut &o, FeScrat
 }
 
 int fe_launch(const Grid &g, const Fields &f, const FieldsOut &o, FeScratch &s,
-              hipStream_t st, bool plain) {
+              hipStream_t st, bool plain, unsigned *accum) {
   // 32-bit cell indices in the fast kernels
   if ((long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) >= (1L << 31) / 16) return -2;
   switch (g.nchem) {
-    case 0: return launch_fe<0>(g, f, o, s, st, plain);
-    case 1: return launch_fe<1>(g, f, o, s, st, plain);
-    case 2: return launch_fe<2>(g, f, o, s, st, plain);
-    case 4: return launch_fe<4>(g, f, o, s, st, plain);
-    case 6: return launch_fe<6>(g, f, o, s, st, plain);
-    case 10: return launch_fe<10>(g, f, o, s, st, plain);
+    case 0: return launch_fe<0>(g, f, o, s, st, plain, accum);
+    case 1: return launch_fe<1>(g, f, o, s, st, plain, accum);
+    case 2: return launch_fe<2>(g, f, o, s, st, plain, accum);
+    case 4: return launch_fe<4>(g, f, o, s, st, plain, accum);
+    case 6: return launch_fe<6>(g, f, o, s, st, plain, accum);
+    case 10: return launch_fe<10>(g, f, o, s, st, plain, accum);
     default: return -2;
   }
 }

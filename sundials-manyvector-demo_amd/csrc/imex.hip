@@ -23,7 +23,6 @@ namespace mvd {
 
 namespace {
 
-__global__ void imex_or_flag(const unsigned *src, unsigned *dst) { *dst |= *src; }
 __global__ void imex_add_int(const int *src, int *dst) { *dst += *src; }
 __global__ void imex_zero(unsigned *flag, int *nfail) {
   *flag = 0u;
@@ -141,10 +140,8 @@ int imex_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, I
       FieldsOut fo;
       for (int q = 0; q < 5; q++) fo.w[q] = Fp[q];
       fo.chem = Fp[5];
-      const int frc = fe_launch(g, fz, fo, fe, st, false);
+      const int frc = fe_launch(g, fz, fo, fe, st, false, m.flag);   // bits into the run's flag
       if (frc) { rc = frc; break; }
-      hipLaunchKernelGGL(imex_or_flag, dim3(1), dim3(1), 0, st, (const unsigned *)fe.flag,
-                         m.flag);
       if (fslow_finish_launch(g, Fp, st) || chem_scale_launch(nc, Zp[5], ch.scale, 0, st)) {
         rc = -1;
         break;

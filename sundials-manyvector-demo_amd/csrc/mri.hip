@@ -39,7 +39,6 @@ constexpr double KW_A[4][4] = {{0.0, 0.0, 0.0, 0.0},
                                {-3.0 / 16.0, 15.0 / 16.0, 0.0, 0.0},
                                {1.0 / 6.0, 3.0 / 10.0, 8.0 / 15.0, 0.0}};
 
-__global__ void mri_or_flag(const unsigned *src, unsigned *dst) { *dst |= *src; }
 __global__ void mri_add_int(const int *src, int *dst) { *dst += *src; }
 __global__ void mri_zero(unsigned *flag, int *nfail) {
   *flag = 0u;
@@ -84,9 +83,8 @@ int enqueue_slow_step(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_
     FieldsOut fo;
     for (int q = 0; q < 5; q++) fo.w[q] = Fi[q];
     fo.chem = Fi[5];
-    const int rc = fe_launch(g, fy, fo, fe, st, false);
+    const int rc = fe_launch(g, fy, fo, fe, st, false, m.flag);  // bits into the run's flag
     if (rc) return rc;
-    hipLaunchKernelGGL(mri_or_flag, dim3(1), dim3(1), 0, st, (const unsigned *)fe.flag, m.flag);
     if (fslow_finish_launch(g, Fi, st)) return -1;
     if (chem_scale_launch(n * g.nchem, Y[5], ch.scale, 0, st)) return -1;
     if (consistent && chem_forcing_to_scaled_launch(n, Fi[5], ch.scale, 1.0 / eunit_inv, st))

@@ -56,8 +56,10 @@ struct FeScratch {
   KernelTimer timer;
 };
 
+// accum: when not null, the legal-state bits are OR-ed into *accum (a stage loop's running
+// flag) instead of into the scratch flag, which is then neither reset nor written.
 int fe_launch(const Grid &g, const Fields &f, const FieldsOut &o, FeScratch &s,
-              hipStream_t st, bool plain);
+              hipStream_t st, bool plain, unsigned *accum = nullptr);
 bool fe_nchem_supported(int nchem);
 
 // CFL wave speed: max over cells of |mx/rho| + c (stability, utilities.cpp:483-528).
