@@ -79,14 +79,14 @@ def smooth_state(nx, ny, nz, nchem=0, seed=7, gamma=1.4, amp=0.2, noise=0.0):
 
 
 def chem_cells(n, seed=4321, alpha=(1.0,) * 8, ntot=(1e-2, 1e2), temp=(10.0, 1e4),
-               gamma=5.0 / 3.0):
+               gamma=5.0 / 3.0, with_rho=False):
     """Independent primordial-chemistry cells (AoS, 10 per cell), physical units:
     total number density n_tot log-uniform in [1e-2,1e2] cm^-3 with rho = n_tot*mH;
     mass fractions of the 8 heavy species ~ Dirichlet(alpha); H- capped at half the positive
     charge so the charge-balanced electron density de = (H+ + He+ + 2He++ - H- + H2+)
     is positive; T log-uniform in [10,1e4] K, ge = kB*T*n/(rho*(gamma-1)).
     Species are number densities; de is stored divided by m_amu/mH as in
-    primordial_ode_main.cpp:357-376."""
+    primordial_ode_main.cpp:357-376. with_rho: also return the mass density drawn per cell."""
     rng = np.random.default_rng(seed)
     lo, hi = np.log(ntot[0]), np.log(ntot[1])
     n_tot = np.exp(rng.uniform(lo, hi, size=n))
@@ -106,6 +106,8 @@ def chem_cells(n, seed=4321, alpha=(1.0,) * 8, ntot=(1e-2, 1e2), temp=(10.0, 1e4
     y[:, :8] = nd
     y[:, 8] = de / M_AMU
     y[:, 9] = ge
+    if with_rho:
+        return np.ascontiguousarray(y.reshape(-1)), T, rho
     return np.ascontiguousarray(y.reshape(-1)), T
 
 
