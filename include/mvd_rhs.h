@@ -63,11 +63,18 @@ int mvd_fe_plain(mvd_context *ctx, double t, const double *const *w, double *con
 /* Queue fe() without waiting or checking the legal-state flag (for stage loops and graph
  * capture); the flag is read by the next mvd_last_state_flags call. */
 int mvd_fe_async(mvd_context *ctx, double t, const double *const *w, double *const *wdot);
+/* Legal-state bits (1 rho, 2 et, 4 p <= 0) of the last mvd_fe, mvd_fe_plain, mvd_fe_async or
+ * mvd_fslow call. The evolution drivers (mvd_erk_evolve, mvd_mri_evolve, mvd_imex_evolve)
+ * gather the bits of their stage evaluations in a flag of their own and report them through
+ * their return code (MVD_ILLEGAL_STATE); they do not change what this returns. */
 int mvd_last_state_flags(mvd_context *ctx);
 /* Vector-Jacobian product of fe() (no counterpart in the reference, which has no adjoint;
  * for adjoint/sensitivity callers): wbar = (d wdot / d w)^T v, v and wbar in the layout of
  * wdot. The derivative of the Lax-Friedrichs speed max() follows its first maximum.
- * Allocates a face buffer of 6*(5+nchem)*(nx+1)(ny+1)(nz+1) doubles on first use. */
+ * Keeps no face buffer. On first use it allocates its scratch: a side buffer for the stencil
+ * adjoints that leave an x-line or a y/z column segment (12*(5+nchem) doubles per x-line,
+ * 6*(5+nchem) per column segment) and, with tracers, three tracer-major planes of
+ * nchem*nx*ny*nz doubles each (w, v and the tracer adjoint). */
 int mvd_fe_vjp(mvd_context *ctx, double t, const double *const *w, const double *const *v,
                double *const *wbar);
 
@@ -293,6 +300,14 @@ int mvd_chem_dirk_adjoint(mvd_context *ctx, double t, double h, int nsteps, int 
  * choice is kept in the context; returns the best time in ms through best_ms. */
 int mvd_chem_dirk_tune(mvd_context *ctx, double h, const mvd_dirk_opts *o, const double *y,
                        double *best_ms);
+/* Set the batched kernel's launch shape by hand (what mvd_chem_dirk_tune chooses): block
+ * threads per block (64, 128 or 256; blocks whose LDS would not fit a CU are split at
+ * launch) and waves resident waves of the persistent grid (0: one per SIMD of the device).
+ * Each wave takes 64-cell chunks, the first by its index and the rest from the work queue,
+ * so waves smaller than the number of chunks exercises the queue. The results do not
+ * depend on the shape. get_shape returns the shape the next launch uses (waves resolved). */
+int mvd_chem_dirk_set_shape(mvd_context *ctx, int block, int waves);
+int mvd_chem_dirk_get_shape(mvd_context *ctx, int *block, int *waves);
 
 #ifdef __cplusplus
 }

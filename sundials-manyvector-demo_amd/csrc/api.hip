@@ -687,6 +687,21 @@ int mvd_chem_dirk_adjoint(mvd_context *ctx, double, double h, int nsteps, int ck
   return nfail;
 }
 
+int mvd_chem_dirk_set_shape(mvd_context *ctx, int block, int waves) {
+  if (!ctx || (block != 64 && block != 128 && block != 256) || waves < 0 || waves > (1 << 24))
+    return MVD_BAD_ARGUMENT;
+  ctx->dirk_block = block;
+  ctx->dirk_waves = waves;
+  return MVD_SUCCESS;
+}
+
+int mvd_chem_dirk_get_shape(mvd_context *ctx, int *block, int *waves) {
+  if (!ctx || !block || !waves) return MVD_BAD_ARGUMENT;
+  *block = ctx->dirk_block;
+  *waves = default_waves(ctx);
+  return MVD_SUCCESS;
+}
+
 int mvd_chem_dirk_tune(mvd_context *ctx, double h, const mvd_dirk_opts *o, const double *y,
                        double *best_ms) {
   if (!ctx || !y || !(h > 0)) return MVD_BAD_ARGUMENT;

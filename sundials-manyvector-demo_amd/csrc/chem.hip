@@ -184,7 +184,9 @@ void chem_free(ChemState &s) {
 }
 
 int chem_setup(ChemState &s, const double *tables_host, long n, double z, hipStream_t st) {
+  const unsigned long gen = s.gen;
   chem_free(s);
+  s.gen = gen + 1;
   bool ok = hipMalloc(&s.tab, sizeof(double) * NTAB * TLEN) == hipSuccess &&
             hipMalloc(&s.scale, sizeof(double) * NSP * n) == hipSuccess &&
             hipMalloc(&s.inv_scale, sizeof(double) * NSP * n) == hipSuccess &&
@@ -196,7 +198,7 @@ int chem_setup(ChemState &s, const double *tables_host, long n, double z, hipStr
             hipMalloc(&s.h2_oda, sizeof(double) * n) == hipSuccess &&
             hipMalloc(&s.counters, sizeof(int) * 64) == hipSuccess &&
             hipHostMalloc(&s.host_counters, sizeof(int) * 64, hipHostMallocDefault) == hipSuccess;
-  if (!ok) { chem_free(s); return MVD_DEVICE_ERROR; }
+  if (!ok) { chem_free(s); s.gen = gen + 1; return MVD_DEVICE_ERROR; }
   if (hipMemcpyAsync(s.tab, tables_host, sizeof(double) * NTAB * TLEN, hipMemcpyHostToDevice,
                      st) != hipSuccess) {
     chem_free(s);

@@ -701,6 +701,7 @@ static int ensure_work(ChemState &s, size_t bytes) {
   if (s.work) (void)hipFree(s.work);
   s.work = nullptr;
   s.work_bytes = 0;
+  s.gen++;
   if (hipMalloc(&s.work, bytes) != hipSuccess) return -1;
   s.work_bytes = bytes;
   return 0;

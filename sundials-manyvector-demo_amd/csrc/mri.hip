@@ -121,7 +121,8 @@ bool same_opts(const DirkDev &a, const DirkDev &b) {
     return false;
   for (int k = 0; k < 16; k++)
     if (a.A[k] != b.A[k]) return false;
-  return a.rtol == b.rtol && a.atol == b.atol && a.ntol == b.ntol && a.rhs_scale == b.rhs_scale;
+  return a.rtol == b.rtol && a.atol == b.atol && a.ntol == b.ntol &&
+         a.rhs_scale == b.rhs_scale && a.max_halvings == b.max_halvings;
 }
 
 }  // namespace
@@ -159,11 +160,12 @@ int mri_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, Mr
   hipLaunchKernelGGL(mri_zero, dim3(1), dim3(1), 0, st, m.flag, m.nfail);
   const bool timer_on = fe.timer.on;
   fe.timer.on = false;                    // no event records inside the graph
-  // the captured step holds the addresses of the chemistry work buffer: another solver call
-  // may have reallocated it since
+  // the captured step holds the addresses of the chemistry arrays and work buffer and the
+  // chemistry constants (redshift) by value: a chemistry setup or a reallocation of the
+  // work buffer by another solver call since the capture makes it stale
   bool rebuild = !m.exec || m.h != h || m.hmax != hmax || m.consistent != consistent ||
                  !same_opts(m.o, o) || m.waves != waves || m.block != block ||
-                 m.work != (const void *)ch.work;
+                 m.chem_gen != ch.gen;
   int rc = 0;
   for (int step = 0; step < nslow && rc == 0; step++) {
     if (!rebuild) {
@@ -197,7 +199,7 @@ int mri_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, Mr
     m.o = o;
     m.waves = waves;
     m.block = block;
-    m.work = ch.work;
+    m.chem_gen = ch.gen;
     rebuild = false;
   }
   fe.timer.on = timer_on;

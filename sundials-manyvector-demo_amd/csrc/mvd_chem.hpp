@@ -46,6 +46,9 @@ struct ChemState {
   size_t work_bytes = 0;
   int *counters = nullptr;    // work queue / statistics
   int *host_counters = nullptr;
+  // bumped by every setup and every reallocation of the work buffer: a captured graph that
+  // holds this state's addresses or constants is stale when its generation differs
+  unsigned long gen = 0;
 };
 
 int chem_setup(ChemState &s, const double *tables_host, long n, double z, hipStream_t st);

@@ -15,7 +15,7 @@ struct MriState {
   int waves = 0, block = 0;
   bool consistent = false;
   DirkDev o{};
-  const void *work = nullptr;                        // chemistry work buffer when captured
+  unsigned long chem_gen = 0;                        // ChemState::gen when captured
 };
 
 // nslow slow steps of size h (fast steps no longer than hmax) in place on w (6 device

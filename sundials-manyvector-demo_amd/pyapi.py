@@ -112,6 +112,8 @@ def lib():
         "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
                                 C.POINTER(DirkStats)], C.c_int),
         "mvd_chem_dirk_tune": ([vp, C.c_double, C.POINTER(DirkOpts), _P, _P], C.c_int),
+        "mvd_chem_dirk_set_shape": ([vp, C.c_int, C.c_int], C.c_int),
+        "mvd_chem_dirk_get_shape": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "mvd_chem_dirk_step_forced": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, _P,
                                        C.POINTER(DirkStats)], C.c_int),
         "mvd_mri_evolve": ([vp, C.c_double, C.c_double, C.c_int, C.c_double,
@@ -500,6 +502,15 @@ class Context:
         if rc < 0:
             raise MvdError("mvd_chem_dirk_step", rc)
         return rc, st.as_dict()
+
+    def dirk_set_shape(self, block=256, waves=0):
+        _check("mvd_chem_dirk_set_shape", lib().mvd_chem_dirk_set_shape(self.h, block, waves))
+
+    def dirk_get_shape(self):
+        b, w = C.c_int(), C.c_int()
+        _check("mvd_chem_dirk_get_shape", lib().mvd_chem_dirk_get_shape(self.h, C.byref(b),
+                                                                        C.byref(w)))
+        return b.value, w.value
 
     def dirk_tune(self, y, h, opts=None):
         o = opts if opts is not None else sdirk3()

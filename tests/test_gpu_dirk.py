@@ -149,3 +149,70 @@ def test_dirk_tune_keeps_results(mvd, torch_gpu):
     assert f1 == f2 == 0
     assert torch.equal(yd, yd2)
     assert s1["newton_iters"] == s2["newton_iters"]
+
+
+@pytest.mark.parametrize("variant", ["plain", "forced", "substep"])
+def test_dirk_work_queue_matches_default_shape(mvd, torch_gpu, variant):
+    """8192 cells are 128 chunks of 64: on 16 resident waves each wave takes its first chunk
+    by index and ~7 more from the atomic work queue (at the default one wave per SIMD every
+    wave gets one chunk and the queue hands out none). The queue changes which wave solves
+    a cell, not the cell's arithmetic: states, temperatures and Newton histograms equal the
+    default shape's bit for bit and the oracle's to 1e-10, for the plain, forced and
+    substepping kernels, with 64- and 256-thread blocks."""
+    torch = torch_gpu
+    n, seed, h = 8192, 31, 1e3
+    print(f"[dirk queue] n={n} variant={variant}", flush=True)
+    rng = np.random.default_rng(seed + 1)
+    force = rng.standard_normal(10 * n) * 1e-5 if variant == "forced" else None
+    mh = 4 if variant == "substep" else 0
+    hstep = 1e5 if variant == "substep" else h
+    kw = dict(max_iters=2, max_setups=0) if variant == "substep" else {}
+    y, _ = problems.chem_cells(n, seed=seed)
+    oc = orc.Chem(n)
+    yo = y.copy()
+    oc.prepare(yo)
+    oc.extra()
+    oo = orc.dirk_opts(max_halvings=mh, **kw)
+    if force is not None:
+        fo = orc.dirk_step_forced(oc, oo, hstep, yo, force)
+        iters_o = None
+    else:
+        fo, iters_o = orc.dirk_step(oc, oo, hstep, yo)
+    runs = []
+    for block, waves in ((256, 0), (256, 16), (64, 16), (128, 3)):
+        ctx = mvd.Context(mvd.make_params(4, 4, 4, 10))
+        ctx.chem_setup(orc.load_tables(), n)
+        yd = torch.from_numpy(y.copy()).to("cuda")
+        ctx.chem_prepare(yd)
+        ctx.dirk_set_shape(block, waves)
+        b, w = ctx.dirk_get_shape()
+        assert b == block and (waves == 0 or w == waves)
+        o = mvd.sdirk3(max_halvings=mh, **kw)
+        if force is not None:
+            fd = ctx.dirk_step_forced(yd, hstep, torch.from_numpy(force).to("cuda"), o)
+            st = None
+        else:
+            fd, st = ctx.dirk_step(yd, hstep, o)
+        runs.append((fd, yd.cpu().numpy(), ctx.chem_state("Ts", torch).cpu().numpy(), st))
+        ctx.close()
+    f0, y0, T0, s0 = runs[0]
+    assert f0 == fo
+    for fd, yd_, Td, st in runs[1:]:
+        assert fd == f0
+        assert np.array_equal(yd_, y0) and np.array_equal(Td, T0)
+        if st is not None:
+            assert st["hist"] == s0["hist"] and st["newton_iters"] == s0["newton_iters"]
+    ok = np.ones(n, dtype=bool) if iters_o is None else iters_o >= 0
+    assert compare(y0, yo, ok) <= TOL
+    if iters_o is not None and fo == 0:
+        hist_o = np.bincount(np.minimum(iters_o, 15), minlength=16)
+        assert list(hist_o) == s0["hist"]
+
+
+def test_dirk_set_shape_rejects_bad_shapes(mvd, torch_gpu):
+    ctx = mvd.Context(mvd.make_params(4, 4, 4, 10))
+    for block, waves in ((32, 0), (512, 0), (96, 0), (256, -1)):
+        with pytest.raises(mvd.MvdError) as e:
+            ctx.dirk_set_shape(block, waves)
+        assert e.value.code == mvd.MVD_BAD_ARGUMENT
+    ctx.close()

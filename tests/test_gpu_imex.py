@@ -89,3 +89,26 @@ def test_imex_rejects_bad_tables(mvd, torch_gpu):
     with pytest.raises(mvd.MvdError):
         ctx.imex_evolve(wd, 1e-4, 1, mvd.sdirk3(rhs_scale=UNITS[2]), table=t)
     ctx.close()
+
+
+def test_imex_reports_illegal_state(mvd, torch_gpu):
+    """fexpl's fe() runs the tracer kernels; a cell with negative density makes the run
+    return MVD_ILLEGAL_STATE (the stage evaluations OR their legal-state bits into the run's
+    flag), while mvd_last_state_flags keeps what the last standalone fe() call left."""
+    torch = torch_gpu
+    dims = (8, 6, 6)
+    n = dims[0] * dims[1] * dims[2]
+    ctx = mvd.Context(mvd.make_params(*dims, 10, (0,) * 6, gamma=5 / 3, units=UNITS))
+    ctx.chem_setup(orc.load_tables(), n)
+    fluid, chem = problems.primordial_blast_mri(*dims, bump=3.0, width=0.25)
+    wd = [torch.from_numpy(a.copy()).to("cuda") for a in fluid] + \
+         [torch.from_numpy(chem.copy()).to("cuda")]
+    ctx.chem_prepare(wd[5])
+    assert ctx.last_state_flags() == 0
+    # (fexpl rebuilds the total energy from the gas energy, so the bad value is a density)
+    wd[0][n // 3] = -abs(float(wd[0][n // 3]))
+    with pytest.raises(mvd.MvdError) as e:
+        ctx.imex_evolve(wd, 1e-4, 1, mvd.sdirk3(rhs_scale=UNITS[2]))
+    assert e.value.code == mvd.MVD_ILLEGAL_STATE
+    assert ctx.last_state_flags() == 0
+    ctx.close()

@@ -158,3 +158,57 @@ def test_step_rejection_matches_oracle(mvd, torch_gpu):
     # h = 1e5 s: most cells fail without halving, a few even at the finest size
     assert results[0][0] > 500 and 0 < results[6][0] < results[0][0]
     ctx.close()
+
+
+def test_mri_reports_illegal_state(mvd, torch_gpu):
+    """The slow RHS (fslow, tracer kernels) meeting a cell of negative density makes the
+    multirate run return MVD_ILLEGAL_STATE; mvd_last_state_flags is not changed by it."""
+    torch = torch_gpu
+    dims, bc = (8, 8, 6), (3,) * 6
+    ctx, wd, _, _ = mri_pair(mvd, torch, dims, bc)
+    assert ctx.last_state_flags() == 0
+    c = dims[0] * dims[1] * 2 + 3
+    wd[0][c] = -abs(float(wd[0][c]))
+    fast = mvd.sdirk3(rhs_scale=UNITS[2])
+    with pytest.raises(mvd.MvdError) as e:
+        ctx.mri_evolve(wd, 0.05, 1, 0.01, fast)
+    assert e.value.code == mvd.MVD_ILLEGAL_STATE
+    assert ctx.last_state_flags() == 0
+    ctx.close()
+
+
+def test_mri_graph_recaptured_after_chemistry_setup(mvd, torch_gpu):
+    """A captured slow step holds the chemistry arrays' addresses and the redshift by value.
+    After mvd_chem_setup (new arrays, another redshift) and a plain DIRK step (which may
+    reallocate the work buffer), the next multirate run must capture again: its result
+    equals the same run in a fresh context bit for bit."""
+    torch = torch_gpu
+    dims, bc = (10, 8, 8), (3,) * 6
+    n = dims[0] * dims[1] * dims[2]
+    fast = mvd.sdirk3(rhs_scale=UNITS[2])
+    fluid, chem = problems.primordial_blast_mri(*dims)
+    ctx, a, _, _ = mri_pair(mvd, torch, dims, bc)
+    assert ctx.mri_evolve(a, 0.05, 2, 0.01, fast) == 0        # eager step + captured step
+    # new chemistry setup with another redshift, re-prepared from the initial species
+    ctx.chem_setup(orc.load_tables(), n, current_z=0.5)
+    b = [torch.from_numpy(x.copy()).to("cuda") for x in fluid] + \
+        [torch.from_numpy(chem.copy()).to("cuda")]
+    ctx.chem_prepare(b[5])
+    Ts0 = ctx.chem_state("Ts", torch).clone()
+    dT0 = ctx.chem_state("dTs_ge", torch).clone()
+    scratch = b[5].clone()
+    ctx.dirk_step(scratch, 1e3, plain=True)
+    ctx.chem_state_set("Ts", Ts0)
+    ctx.chem_state_set("dTs_ge", dT0)
+    assert ctx.mri_evolve(b, 0.05, 3, 0.01, fast) == 0
+    fresh = mvd.Context(mvd.make_params(*dims, 10, bc, gamma=5 / 3, units=UNITS))
+    fresh.chem_setup(orc.load_tables(), n, current_z=0.5)
+    c = [torch.from_numpy(x.copy()).to("cuda") for x in fluid] + \
+        [torch.from_numpy(chem.copy()).to("cuda")]
+    fresh.chem_prepare(c[5])
+    assert fresh.mri_evolve(c, 0.05, 3, 0.01, fast) == 0
+    for q in range(6):
+        assert torch.equal(b[q], c[q]), q
+    assert torch.equal(ctx.chem_state("Ts", torch), fresh.chem_state("Ts", torch))
+    ctx.close()
+    fresh.close()
