@@ -307,6 +307,14 @@ int mvd_chem_dirk_tune(mvd_context *ctx, double h, const mvd_dirk_opts *o, const
  * so waves smaller than the number of chunks exercises the queue. The results do not
  * depend on the shape. get_shape returns the shape the next launch uses (waves resolved). */
 int mvd_chem_dirk_set_shape(mvd_context *ctx, int block, int waves);
+/* The block split of a batched launch (no GPU needed; diagnostics and tests): the threads
+ * per block a launch asked for `block` threads runs with for an nstage-stage table, when the
+ * kernel's static LDS is static_lds bytes and the device allows lds_limit bytes per block
+ * (the launch asks the runtime for both). The dynamic LDS is 8 * (4*1024 + waves * (24 +
+ * 10*nstage) * 64) bytes: the gamma tables, then per wave its cells' constants, the stage
+ * derivatives of the earlier stages and the step start. Halves the block until it fits;
+ * 0 when not even one wave fits. */
+int mvd_chem_dirk_fit_block(int block, int nstage, long static_lds, long lds_limit);
 int mvd_chem_dirk_get_shape(mvd_context *ctx, int *block, int *waves);
 
 #ifdef __cplusplus

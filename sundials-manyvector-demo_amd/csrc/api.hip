@@ -687,6 +687,13 @@ int mvd_chem_dirk_adjoint(mvd_context *ctx, double, double h, int nsteps, int ck
   return nfail;
 }
 
+int mvd_chem_dirk_fit_block(int block, int nstage, long static_lds, long lds_limit) {
+  if ((block != 64 && block != 128 && block != 256) || nstage < 1 ||
+      nstage > mvd::MAXSTAGE || static_lds < 0 || lds_limit <= 0)
+    return MVD_BAD_ARGUMENT;
+  return mvd::dirk_fit_block(block, nstage, (size_t)static_lds, (size_t)lds_limit);
+}
+
 int mvd_chem_dirk_set_shape(mvd_context *ctx, int block, int waves) {
   if (!ctx || (block != 64 && block != 128 && block != 256) || waves < 0 || waves > (1 << 24))
     return MVD_BAD_ARGUMENT;

@@ -336,11 +336,19 @@ constexpr int CELL_WORDS = 2 * NSP + 4;
 __host__ __device__ inline int dirk_wave_words(int nstage) {
   return CELL_WORDS + (nstage - 1) * NSP + NSP;
 }
-constexpr size_t LDS_LIMIT = 160 * 1024;   // LDS of a gfx950 CU, all of it one block's
-
 // dynamic LDS of the batched kernel for `wpb` waves per block
 inline size_t dirk_lds_bytes(int wpb, int nstage) {
   return sizeof(double) * ((size_t)4 * TLEN + (size_t)wpb * dirk_wave_words(nstage) * 64);
+}
+
+// Threads per block of a batched launch asked for `block` threads: blocks whose LDS (the
+// dynamic part for their waves plus the kernel's static part) would exceed the per-block
+// limit are halved (fewer waves then share one copy of the gamma tables); 0 when not even
+// one wave fits.
+int dirk_fit_block(int block, int nstage, size_t static_lds, size_t limit) {
+  for (; block >= 64; block /= 2)
+    if (dirk_lds_bytes(block / 64, nstage) + static_lds <= limit) return block;
+  return 0;
 }
 
 // FORCED: y' = s_rhs f(y) + force, force constant over the step ([cell][10], the forcing an
@@ -791,9 +799,19 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
   if (n <= 0 || n > (1L << 30)) return -1;
   if (block != 64 && block != 128 && block != 256) return -1;
   if (o.nstage < 1 || o.nstage > MAXSTAGE) return -1;
-  // blocks whose LDS would not fit one CU are split (fewer waves share the gamma tables)
-  while (block > 64 && dirk_lds_bytes(block / 64, o.nstage) + NSTAT * sizeof(int) > LDS_LIMIT)
-    block /= 2;
+  const bool sub = o.max_halvings > 0;
+  auto kern = force ? (sub ? chem_dirk_batched<true, true> : chem_dirk_batched<true, false>)
+                    : (sub ? chem_dirk_batched<false, true> : chem_dirk_batched<false, false>);
+  // blocks whose LDS would not fit are split: the kernel's static LDS as compiled and the
+  // device's per-block limit, both asked of the runtime
+  hipFuncAttributes fa{};
+  int dev = 0, lim = 0;
+  if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kern)) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return -1;
+  block = dirk_fit_block(block, o.nstage, fa.sharedSizeBytes, (size_t)lim);
+  if (block == 0) return -1;
   const int wpb = block / 64;
   const size_t chunks = (size_t)(n + 63) / 64;
   long waves = grid_waves > 0 ? grid_waves : 1;
@@ -827,10 +845,7 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
                        (const unsigned char *)key, cnt, order);
   }
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
-  const bool sub = o.max_halvings > 0;
   const unsigned lds = (unsigned)dirk_lds_bytes(wpb, o.nstage);
-  auto kern = force ? (sub ? chem_dirk_batched<true, true> : chem_dirk_batched<true, false>)
-                    : (sub ? chem_dirk_batched<false, true> : chem_dirk_batched<false, false>);
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, st, n, s.cc, ptrs_d(s), o, h,
                      y, force, (const int *)order, sub ? nullptr : (const double2 *)tdt,
                      s.counters, rows);

@@ -113,6 +113,7 @@ def lib():
                                 C.POINTER(DirkStats)], C.c_int),
         "mvd_chem_dirk_tune": ([vp, C.c_double, C.POINTER(DirkOpts), _P, _P], C.c_int),
         "mvd_chem_dirk_set_shape": ([vp, C.c_int, C.c_int], C.c_int),
+        "mvd_chem_dirk_fit_block": ([C.c_int, C.c_int, C.c_long, C.c_long], C.c_int),
         "mvd_chem_dirk_get_shape": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "mvd_chem_dirk_step_forced": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, _P,
                                        C.POINTER(DirkStats)], C.c_int),
