@@ -230,6 +230,18 @@ int mvd_dirk_sdirk3(mvd_dirk_opts *o);
  * multi-kernel path. Returns 0, or the number of failed cells (> 0, recoverable). */
 int mvd_chem_dirk_step(mvd_context *ctx, double t, double h, const mvd_dirk_opts *o, double *y,
                        int plain, mvd_dirk_stats *stats);
+/* Allocation-free variant of mvd_chem_dirk_step with plain = 0 (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same kernels and the same results
+ * bit for bit; the work buffer (cell order, sort keys, bucket counters, per-block statistics)
+ * and the work-queue and statistics counters live in a library-owned array of 21.3 MiB per
+ * GPU instead of the context's lazily allocated buffer, so the call allocates nothing.
+ * Limits: at most 1048576 cells (mvd_chem_setup's ncell) and 4096 blocks in the persistent
+ * grid, else MVD_BAD_ARGUMENT before any device work. Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, threads or contexts) race and
+ * must be serialised by the caller. stats->ms is host wall time of the call. */
+int mvd_chem_dirk_step_noalloc(mvd_context *ctx, double t, double h, const mvd_dirk_opts *o,
+                               double *y, mvd_dirk_stats *stats);
 /* The same step of y' = rhs_scale*f(y) + force with a constant forcing force ([cell][10],
  * scaled; the forcing MRIStep hands its fast stepper, applied by ffast at
  * multirate_chem_hydro_main.cpp:922-946), batched kernel only. */

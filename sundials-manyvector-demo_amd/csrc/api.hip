@@ -1,4 +1,5 @@
 // C-ABI layer (include/mvd_rhs.h): context lifetime, argument checks, launches.
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
@@ -132,6 +133,10 @@ int mvd_create(const mvd_params *p, void *stream, mvd_context **out) {
     mvd_destroy(c);
     return MVD_DEVICE_ERROR;
   }
+  // set-up of mvd_chem_dirk_step_noalloc: the device address of its scratch array on this
+  // device (a failure here surfaces as MVD_DEVICE_ERROR from that entry point, which looks
+  // the address up again)
+  (void)mvd::dirk_noalloc_scratch();
   *out = c;
   return MVD_SUCCESS;
 }
@@ -559,6 +564,38 @@ static int dirk_step_common(mvd_context *ctx, double h, const mvd_dirk_opts *o, 
 int mvd_chem_dirk_step(mvd_context *ctx, double, double h, const mvd_dirk_opts *o, double *y,
                        int plain, mvd_dirk_stats *stats) {
   return dirk_step_common(ctx, h, o, y, plain, nullptr, stats);
+}
+
+int mvd_chem_dirk_step_noalloc(mvd_context *ctx, double, double h, const mvd_dirk_opts *o,
+                               double *y, mvd_dirk_stats *stats) {
+  if (!ctx || !y || !(h > 0)) return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  mvd::DirkDev d;
+  int rc = dirk_check(o, d);
+  if (rc) return rc;
+  if (ctx->chem.n > mvd::NOALLOC_MAX_CELLS) return MVD_BAD_ARGUMENT;
+  const auto t0 = std::chrono::steady_clock::now();
+  int *cnt = nullptr;
+  rc = mvd::dirk_batched_launch_noalloc(ctx->chem, d, h, y, ctx->stream, default_waves(ctx),
+                                        ctx->dirk_block, nullptr, &cnt);
+  if (rc == MVD_BAD_ARGUMENT) return rc;
+  if (rc) return MVD_DEVICE_ERROR;
+  int *hc = ctx->chem.host_counters;                       // pinned, from mvd_chem_setup
+  if (!dev_ok(hipMemcpyAsync(hc, cnt + 1, sizeof(int) * mvd::NSTAT, hipMemcpyDeviceToHost,
+                             ctx->stream)) ||
+      !dev_ok(hipStreamSynchronize(ctx->stream)))
+    return MVD_DEVICE_ERROR;
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    stats->newton_iters = hc[mvd::STAT_ITERS];
+    stats->max_cell_iters = hc[mvd::STAT_MAXIT];
+    stats->failed_cells = hc[mvd::STAT_FAILED];
+    stats->setups = hc[mvd::STAT_SETUPS];
+    for (int b = 0; b < 16; b++) stats->hist[b] = hc[mvd::STAT_HIST + b];
+    stats->ms = ms;
+  }
+  return hc[mvd::STAT_FAILED];
 }
 
 int mvd_chem_dirk_step_forced(mvd_context *ctx, double, double h, const mvd_dirk_opts *o,

@@ -19,6 +19,7 @@
 // Both follow the same per-cell algorithm (stage predictor, modified Newton with the
 // Jacobian from y_n, re-setup after max_iters) and the same operation order, so they agree
 // with each other bitwise and with the CPU oracle iteration for iteration.
+#include <mutex>
 #include "mvd_chem.hpp"
 #include "mvd_dirk.hpp"
 #include "../../include/mvd_rhs.h"
@@ -715,7 +716,7 @@ static int ensure_work(ChemState &s, size_t bytes) {
   return 0;
 }
 
-static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+static constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // ------------------------------------------------------------------------------------
 // Temperature order of the batched step: a counting sort of the cells by the table bin of
@@ -793,49 +794,66 @@ tord_scatter(long n, const unsigned char *key, int *cnt, int *order) {
   if (i < n) order[(k ? pre[k - 1] : 0) + res[k] + r] = (int)i;
 }
 
-int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
-                        int grid_waves, int block, const double *force, bool reuse_order) {
-  const long n = s.n;
+// Launch plan of one batched step: kernel variant, block after the LDS split, grid, and the
+// layout of its work buffer (per-block statistics rows | cell order | bucket totals and
+// running offsets | keys | temperatures at y_n).
+struct BatchedPlan {
+  void (*kern)(long, ChemConst, ChemPtrsD, DirkDev, double, double *, const double *,
+               const int *, const double2 *, int *, int *);
+  int block, wpb;
+  long grid, nrow;
+  bool sorted;
+  size_t b_rows, b_ord, b_cnt, b_key, b_tdt;
+  size_t bytes() const { return b_rows + b_ord + b_cnt + b_key + b_tdt; }
+};
+
+static int batched_plan(long n, const DirkDev &o, int grid_waves, int block, bool forced,
+                        BatchedPlan &p) {
   if (n <= 0 || n > (1L << 30)) return -1;
   if (block != 64 && block != 128 && block != 256) return -1;
   if (o.nstage < 1 || o.nstage > MAXSTAGE) return -1;
   const bool sub = o.max_halvings > 0;
-  auto kern = force ? (sub ? chem_dirk_batched<true, true> : chem_dirk_batched<true, false>)
-                    : (sub ? chem_dirk_batched<false, true> : chem_dirk_batched<false, false>);
+  p.kern = forced ? (sub ? chem_dirk_batched<true, true> : chem_dirk_batched<true, false>)
+                  : (sub ? chem_dirk_batched<false, true> : chem_dirk_batched<false, false>);
   // blocks whose LDS would not fit are split: the kernel's static LDS as compiled and the
   // device's per-block limit, both asked of the runtime
   hipFuncAttributes fa{};
   int dev = 0, lim = 0;
-  if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(kern)) != hipSuccess ||
+  if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(p.kern)) != hipSuccess ||
       hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
     return -1;
-  block = dirk_fit_block(block, o.nstage, fa.sharedSizeBytes, (size_t)lim);
-  if (block == 0) return -1;
-  const int wpb = block / 64;
+  p.block = dirk_fit_block(block, o.nstage, fa.sharedSizeBytes, (size_t)lim);
+  if (p.block == 0) return -1;
+  p.wpb = p.block / 64;
   const size_t chunks = (size_t)(n + 63) / 64;
   long waves = grid_waves > 0 ? grid_waves : 1;
   if (waves > (long)chunks) waves = (long)chunks;
-  const long grid = (waves + wpb - 1) / wpb;
-  const long nrow = grid;                                  // one statistics row per block
-  // work: per-block statistics rows | cell order | bucket totals and running offsets | keys
-  const bool sorted = n >= TORD_MIN;
-  const size_t b_rows = align256((size_t)nrow * NSTAT * sizeof(int));
-  const size_t b_ord = sorted ? align256((size_t)n * sizeof(int)) : 0;
-  const size_t b_cnt = sorted ? align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int)) : 0;
-  const size_t b_key = sorted ? align256((size_t)n) : 0;
-  const size_t b_tdt = sorted ? align256((size_t)n * sizeof(double2)) : 0;
-  const void *work0 = s.work;
-  if (ensure_work(s, b_rows + b_ord + b_cnt + b_key + b_tdt)) return -1;
-  if (s.work != work0) reuse_order = false;                // a new buffer holds no order
-  char *w = (char *)s.work;
-  int *rows = (int *)(w);
-  int *order = sorted ? (int *)(w + b_rows) : nullptr;
+  p.grid = (waves + p.wpb - 1) / p.wpb;
+  p.nrow = p.grid;                                         // one statistics row per block
+  p.sorted = n >= TORD_MIN;
+  p.b_rows = align256((size_t)p.nrow * NSTAT * sizeof(int));
+  p.b_ord = p.sorted ? align256((size_t)n * sizeof(int)) : 0;
+  p.b_cnt = p.sorted ? align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int)) : 0;
+  p.b_key = p.sorted ? align256((size_t)n) : 0;
+  p.b_tdt = p.sorted ? align256((size_t)n * sizeof(double2)) : 0;
+  return 0;
+}
+
+// Queue one planned step: the temperature-bin order (unless reused), the counters cleared,
+// the persistent kernel and the statistics reduction into counters[1..NSTAT]. work holds
+// p.bytes(); counters 1 + NSTAT ints (the work-queue position, then the statistics).
+static int batched_enqueue(const BatchedPlan &p, ChemState &s, const DirkDev &o, double h,
+                           double *y, const double *force, bool reuse_order, char *work,
+                           int *counters, hipStream_t st) {
+  const long n = s.n;
+  int *rows = (int *)(work);
+  int *order = p.sorted ? (int *)(work + p.b_rows) : nullptr;
   double2 *tdt = nullptr;                                  // temperatures at y_n, if solved
-  if (sorted && !reuse_order) {
-    int *cnt = (int *)(w + b_rows + b_ord);
-    unsigned char *key = (unsigned char *)(w + b_rows + b_ord + b_cnt);
-    tdt = (double2 *)(w + b_rows + b_ord + b_cnt + b_key);
+  if (p.sorted && !reuse_order) {
+    int *cnt = (int *)(work + p.b_rows + p.b_ord);
+    unsigned char *key = (unsigned char *)(work + p.b_rows + p.b_ord + p.b_cnt);
+    tdt = (double2 *)(work + p.b_rows + p.b_ord + p.b_cnt + p.b_key);
     hipLaunchKernelGGL(fill_int, dim3(2 * TORD_NBK * TORD_PAD / 256), dim3(256), 0, st,
                        (long)(2 * TORD_NBK * TORD_PAD), cnt, 0);
     const unsigned tb = (unsigned)((n + TORD_BLOCK - 1) / TORD_BLOCK);
@@ -844,14 +862,76 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
     hipLaunchKernelGGL(tord_scatter, dim3(tb), dim3(TORD_BLOCK), 0, st, n,
                        (const unsigned char *)key, cnt, order);
   }
-  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), s.counters, 0);
-  const unsigned lds = (unsigned)dirk_lds_bytes(wpb, o.nstage);
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(block), lds, st, n, s.cc, ptrs_d(s), o, h,
-                     y, force, (const int *)order, sub ? nullptr : (const double2 *)tdt,
-                     s.counters, rows);
-  hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(nrow)), dim3(256), 0, st,
-                     (const int *)rows, nrow, s.counters + 1);
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), counters, 0);
+  const unsigned lds = (unsigned)dirk_lds_bytes(p.wpb, o.nstage);
+  const bool sub = o.max_halvings > 0;
+  hipLaunchKernelGGL(p.kern, dim3((unsigned)p.grid), dim3(p.block), lds, st, n, s.cc,
+                     ptrs_d(s), o, h, y, force, (const int *)order,
+                     sub ? nullptr : (const double2 *)tdt, counters, rows);
+  hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(p.nrow)), dim3(256), 0, st,
+                     (const int *)rows, p.nrow, counters + 1);
   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
+                        int grid_waves, int block, const double *force, bool reuse_order) {
+  BatchedPlan p;
+  if (batched_plan(s.n, o, grid_waves, block, force != nullptr, p)) return -1;
+  const void *work0 = s.work;
+  if (ensure_work(s, p.bytes())) return -1;
+  if (s.work != work0) reuse_order = false;                // a new buffer holds no order
+  return batched_enqueue(p, s, o, h, y, force, reuse_order, (char *)s.work, s.counters, st);
+}
+
+// ------------------------------------------------------------------------------------
+// Allocation-free batched step: the same kernels with their work buffer and counters in
+// one file-scope device array per GPU instead of the context's lazily sized buffer, for
+// callers that cannot allocate device memory (graph capture, tight serving loops). Each
+// GPU holds its own copy for the life of the process; every call on a GPU uses that GPU's
+// copy, so calls that overlap on one GPU (other streams, other threads, other contexts)
+// race and must not be issued. Everything a step needs to start from zero (bucket counters,
+// work-queue position, statistics) is cleared by the step itself.
+// ------------------------------------------------------------------------------------
+constexpr size_t NOALLOC_COUNTERS = 256;                   // 1 + NSTAT ints, padded
+constexpr size_t NOALLOC_WORK =
+    align256((size_t)NOALLOC_MAX_ROWS * NSTAT * sizeof(int)) +
+    align256((size_t)NOALLOC_MAX_CELLS * sizeof(int)) +
+    align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int)) +
+    align256((size_t)NOALLOC_MAX_CELLS) +
+    align256((size_t)NOALLOC_MAX_CELLS * sizeof(double2));
+static __device__ __align__(256) unsigned char scratch[NOALLOC_COUNTERS + NOALLOC_WORK];
+
+constexpr int NOALLOC_MAX_DEV = 64;
+static std::mutex noalloc_mu;
+static unsigned char *noalloc_ptr[NOALLOC_MAX_DEV] = {};
+
+// The current device's address of `scratch`, looked up once per device (with that device
+// current); nullptr on failure.
+unsigned char *dirk_noalloc_scratch() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEV) return nullptr;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void *a = nullptr;
+    if (hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)) != hipSuccess || !a) return nullptr;
+    noalloc_ptr[dev] = (unsigned char *)a;
+  }
+  return noalloc_ptr[dev];
+}
+
+int dirk_batched_launch_noalloc(ChemState &s, const DirkDev &o, double h, double *y,
+                                hipStream_t st, int grid_waves, int block, const double *force,
+                                int **counters_out) {
+  if (s.n > NOALLOC_MAX_CELLS) return MVD_BAD_ARGUMENT;
+  BatchedPlan p;
+  if (batched_plan(s.n, o, grid_waves, block, force != nullptr, p)) return -1;
+  if (p.nrow > NOALLOC_MAX_ROWS || p.bytes() > NOALLOC_WORK) return MVD_BAD_ARGUMENT;
+  unsigned char *base = dirk_noalloc_scratch();
+  if (!base) return -1;
+  int *counters = (int *)base;
+  *counters_out = counters;
+  return batched_enqueue(p, s, o, h, y, force, false, (char *)(base + NOALLOC_COUNTERS),
+                         counters, st);
 }
 
 int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st) {

@@ -32,6 +32,17 @@ struct DirkDev {
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
                         int grid_waves, int block, const double *force = nullptr,
                         bool reuse_order = false);
+// The same step with its work buffer and counters in the library's per-GPU scratch array
+// instead of the context's buffer (no device allocation); at most NOALLOC_MAX_CELLS cells
+// and NOALLOC_MAX_ROWS blocks, else MVD_BAD_ARGUMENT. *counters_out: the counters it used
+// (work-queue position, then NSTAT statistics), valid until the next such call on this GPU.
+constexpr long NOALLOC_MAX_CELLS = 1L << 20;
+constexpr long NOALLOC_MAX_ROWS = 4096;
+int dirk_batched_launch_noalloc(ChemState &s, const DirkDev &o, double h, double *y,
+                                hipStream_t st, int grid_waves, int block, const double *force,
+                                int **counters_out);
+// the current device's address of that scratch array (looked up once per device)
+unsigned char *dirk_noalloc_scratch();
 int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st);
 // threads per block of a batched launch (blocks split to fit the LDS limit; 0: none fits)
 int dirk_fit_block(int block, int nstage, size_t static_lds, size_t limit);

@@ -112,6 +112,8 @@ def lib():
         "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
                                 C.POINTER(DirkStats)], C.c_int),
         "mvd_chem_dirk_tune": ([vp, C.c_double, C.POINTER(DirkOpts), _P, _P], C.c_int),
+        "mvd_chem_dirk_step_noalloc": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P,
+                                        C.POINTER(DirkStats)], C.c_int),
         "mvd_chem_dirk_set_shape": ([vp, C.c_int, C.c_int], C.c_int),
         "mvd_chem_dirk_fit_block": ([C.c_int, C.c_int, C.c_long, C.c_long], C.c_int),
         "mvd_chem_dirk_get_shape": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -512,6 +514,16 @@ class Context:
         _check("mvd_chem_dirk_get_shape", lib().mvd_chem_dirk_get_shape(self.h, C.byref(b),
                                                                         C.byref(w)))
         return b.value, w.value
+
+    def dirk_step_noalloc(self, y, h, opts=None, t=0.0):
+        """mvd_chem_dirk_step_noalloc (batched step, no device allocation). Returns (failed,
+        stats)."""
+        o = opts if opts is not None else sdirk3()
+        st = DirkStats()
+        rc = lib().mvd_chem_dirk_step_noalloc(self.h, t, h, C.byref(o), dptr(y), C.byref(st))
+        if rc < 0:
+            raise MvdError("mvd_chem_dirk_step_noalloc", rc)
+        return rc, st.as_dict()
 
     def dirk_tune(self, y, h, opts=None):
         o = opts if opts is not None else sdirk3()
