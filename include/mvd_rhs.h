@@ -103,6 +103,11 @@ int mvd_stability(mvd_context *ctx, const double *const *w, double t, double *dt
  * energy, calls fe(), moves etdot into the gas-energy slot of wdot[5], zeroes wdot[4] and
  * unscales chem. Requires nchem > 0 and mvd_chem_setup. */
 int mvd_fslow(mvd_context *ctx, double t, double *const *w, double *const *wdot);
+/* PostprocessStep of the IMEX driver (imex_chem_hydro_main.cpp:1002-1050): apply the Dengo
+ * scaling to w[5] in place, rebuild et = ge / EnergyUnits + |m|^2 / (2 rho) from the gas
+ * energy and the momenta, unapply the scaling (x * s / s: the chemistry's bits may change,
+ * as in the reference). Requires nchem = 10 and mvd_chem_setup. */
+int mvd_postprocess_step(mvd_context *ctx, double t, double *const *w);
 
 /* Kernel timing with HIP events on the context's stream (the reference times every RHS
  * and face flux with its Profile class, profiler.hpp:23-206). mvd_profile(ctx,1) resets
@@ -131,6 +136,15 @@ int mvd_chem_rhs(mvd_context *ctx, double t, const double *y, double *ydot);
  * 10x10 blocks J[100*cell + 10*col + row] (DENSEIDX, line 40), evaluated from the
  * temperature of the last mvd_chem_rhs call. */
 int mvd_chem_jac(mvd_context *ctx, double t, const double *y, double *J);
+/* The same RHS and Jacobian times a factor applied to every entry after the reference's
+ * expressions: ffast / fimpl (multirate_chem_hydro_main.cpp:898-920, imex_chem_hydro_main.cpp:
+ * 811-840) scale the RHS by TimeUnits (N_VScale), Jfast / Jimpl (multirate...:954-994,
+ * imex...:850-908) the Jacobian data (J *= TimeUnits, :980-982). A factor of 1 gives
+ * mvd_chem_rhs / mvd_chem_jac bit for bit. */
+int mvd_chem_rhs_scaled(mvd_context *ctx, double t, double rhs_scale, const double *y,
+                        double *ydot);
+int mvd_chem_jac_scaled(mvd_context *ctx, double t, double jac_scale, const double *y,
+                        double *J);
 /* N_Vector operations on device arrays, as the integrators' stage algebra uses them (the
  * ManyVector ops of SUNDIALS 6.2.0 that ARKStep/MRIStep call; no counterpart in the
  * reference tree, which links them from the library).
@@ -139,6 +153,8 @@ int mvd_chem_jac(mvd_context *ctx, double t, const double *y, double *J);
  * pointers. */
 int mvd_nvec_linear_combination(mvd_context *ctx, int nv, const double *c,
                                 const double *const *X, double *z, long len);
+/* N_VConst: z[0..len) = c (device array). */
+int mvd_nvec_const(mvd_context *ctx, double c, double *z, long len);
 /* N_VWrmsNorm of a ManyVector of nsub (<= 8) subvectors: sqrt(sum_q sum_i (x_q[i] w_q[i])^2 /
  * sum_q len[q]), deterministic; x, w host arrays of device pointers, len host array. */
 int mvd_nvec_wrms_norm(mvd_context *ctx, int nsub, const double *const *x,
@@ -172,6 +188,11 @@ int mvd_erk_evolve(mvd_context *ctx, const mvd_erk_table *tab, double t0, double
  * chemistry subvector ([cell][10]; x may equal b). */
 int mvd_chem_lsetup(mvd_context *ctx, double gamma, const double *J);
 int mvd_chem_lsolve(mvd_context *ctx, const double *b, double *x);
+/* Setup of the block linear solver from the matrix itself, as SUNLinSolSetup(S, A) receives
+ * it (ARKLS has formed A = I - gamma J in the SUNMatrix_MagmaDenseBlock before the call;
+ * Setup_RankLocalLS passes A to the block solver, multirate_chem_hydro_main.cpp:1360-1368):
+ * A in the mvd_chem_jac layout, factored as given. Returns as mvd_chem_lsetup. */
+int mvd_chem_lsetup_matrix(mvd_context *ctx, const double *A);
 /* Vector-Jacobian product of calculate_rhs_cvklu (no counterpart in the reference, which
  * has only the forward Jacobian callback; the adjoint callers are sensitivity and
  * checkpointed-adjoint drivers): ybar = (d ydot / d y)^T v in the scaled variables, for the

@@ -6,7 +6,9 @@ output / restart input, host code), in-tree.
 Each .hip translation unit is compiled with hipcc for gfx950 only, then linked into
 sundials-manyvector-demo_amd/libmvd_rhs.so. Objects go to sundials-manyvector-demo_amd/_build/.
 csrc/io.cpp is compiled with g++ against the serial HDF5 of the image (HDF5_DIR, default
-/opt/conda) and the HIP runtime into libmvd_io.so; without HDF5 it is skipped.
+/opt/conda) and the HIP runtime into libmvd_io.so; without HDF5 it is skipped. arkode/
+mvd_arkode.cpp (the ARKODE callbacks over the C ABI) is compiled with g++ into
+libmvd_arkode.so.
 """
 import argparse
 import os
@@ -23,6 +25,7 @@ SOURCES = ["api.hip", "fe.hip", "fe_vjp.hip", "nvec.hip", "chem.hip", "chem_dirk
            "imex.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 IO_LIB = os.path.join(HERE, "libmvd_io.so")
+ARKODE_LIB = os.path.join(HERE, "libmvd_arkode.so")
 HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
@@ -58,7 +61,25 @@ def build(jobs=4, verbose=False):
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_io()
+    build_arkode()
     return LIB
+
+
+def build_arkode():
+    """libmvd_arkode.so: the ARKODE callbacks (include/mvd_arkode.h) over libmvd_rhs.so, host
+    code. Built against the restated SUNDIALS declarations (arkode/mvd_sundials_min.h); its
+    SUNDIALS symbols stay undefined and are resolved by the SUNDIALS libraries of the process
+    that loads it (a driver linking SUNDIALS, or the tests' stand-in)."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    tmp = ARKODE_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-Wextra",
+           "-Wno-unused-parameter", "-I", inc, os.path.join(HERE, "arkode", "mvd_arkode.cpp"),
+           "-o", tmp, "-L", HERE, "-l:libmvd_rhs.so", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"libmvd_arkode.so build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, ARKODE_LIB)
+    return ARKODE_LIB
 
 
 def build_io():

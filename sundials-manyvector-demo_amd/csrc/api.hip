@@ -342,6 +342,36 @@ int mvd_chem_rhs(mvd_context *ctx, double, const double *y, double *ydot) {
   return mvd::chem_rhs(ctx->chem, y, ydot, ctx->stream, true);
 }
 
+int mvd_chem_rhs_scaled(mvd_context *ctx, double, double rhs_scale, const double *y,
+                        double *ydot) {
+  if (!ctx || !y || !ydot || !std::isfinite(rhs_scale)) return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  return mvd::chem_rhs(ctx->chem, y, ydot, ctx->stream, true, rhs_scale);
+}
+
+int mvd_chem_jac_scaled(mvd_context *ctx, double, double jac_scale, const double *y,
+                        double *J) {
+  if (!ctx || !y || !J || !std::isfinite(jac_scale)) return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  return mvd::chem_jac(ctx->chem, y, J, ctx->stream, true, jac_scale);
+}
+
+int mvd_postprocess_step(mvd_context *ctx, double, double *const *w) {
+  if (!ctx) return MVD_BAD_ARGUMENT;
+  if (ctx->g.nchem != mvd::NSP) return MVD_UNSUPPORTED;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  if (ctx->chem.n != (long)ctx->g.nx * ctx->g.ny * ctx->g.nz) return MVD_BAD_ARGUMENT;
+  const int rc = check_ptrs(ctx, (const double *const *)w);
+  if (rc) return rc;
+  // apply_Dengo_scaling, et from the gas energy and the kinetic energy, unapply
+  if (mvd::fslow_prepare_launch(ctx->g, w, ctx->chem.scale, 1.0 / ctx->energy_units,
+                                ctx->stream) ||
+      mvd::chem_scale_launch(ctx->chem.n * mvd::NSP, w[5], ctx->chem.scale, 0, ctx->stream) ||
+      !dev_ok(hipStreamSynchronize(ctx->stream)))
+    return MVD_DEVICE_ERROR;
+  return MVD_SUCCESS;
+}
+
 int mvd_chem_vjp(mvd_context *ctx, double, const double *y, const double *v, double *ybar,
                  double *ydot) {
   if (!ctx || !y || !v || !ybar) return MVD_BAD_ARGUMENT;
@@ -385,6 +415,28 @@ int mvd_chem_lsetup(mvd_context *ctx, double gamma, const double *J) {
   return nsing;
 }
 
+int mvd_chem_lsetup_matrix(mvd_context *ctx, const double *A) {
+  if (!ctx || !A) return MVD_BAD_ARGUMENT;
+  if (!ctx->chem.ready) return MVD_NOT_READY;
+  const long n = ctx->chem.n;
+  if (!ctx->ls_lu) {
+    if (!dev_ok(hipMalloc(&ctx->ls_lu, sizeof(double) * 100 * n)) ||
+        !dev_ok(hipMalloc(&ctx->ls_piv, sizeof(unsigned long long) * n)) ||
+        !dev_ok(hipMalloc(&ctx->ls_nsing, sizeof(int))))
+      return MVD_DEVICE_ERROR;
+  }
+  ctx->ls_ready = false;
+  if (mvd::chem_ls_setup(n, 0.0, A, ctx->ls_lu, ctx->ls_piv, ctx->ls_nsing, ctx->stream, true))
+    return MVD_DEVICE_ERROR;
+  int nsing = 0;
+  if (!dev_ok(hipMemcpyAsync(&nsing, ctx->ls_nsing, sizeof(int), hipMemcpyDeviceToHost,
+                             ctx->stream)) ||
+      !dev_ok(hipStreamSynchronize(ctx->stream)))
+    return MVD_DEVICE_ERROR;
+  ctx->ls_ready = nsing == 0;
+  return nsing;
+}
+
 int mvd_chem_lsolve(mvd_context *ctx, const double *b, double *x) {
   if (!ctx || !b || !x) return MVD_BAD_ARGUMENT;
   if (!ctx->ls_ready) return MVD_NOT_READY;
@@ -405,6 +457,13 @@ int mvd_nvec_linear_combination(mvd_context *ctx, int nv, const double *c,
     if (X[k] == z) return MVD_BAD_ARGUMENT;     // only X[0] may be the output
   if (mvd::linear_combination_launch(nv, c, X, z, len, ctx->stream) ||
       !dev_ok(hipStreamSynchronize(ctx->stream)))
+    return MVD_DEVICE_ERROR;
+  return MVD_SUCCESS;
+}
+
+int mvd_nvec_const(mvd_context *ctx, double c, double *z, long len) {
+  if (!ctx || len < 0 || (len > 0 && !z)) return MVD_BAD_ARGUMENT;
+  if (mvd::const_launch(c, z, len, ctx->stream) || !dev_ok(hipStreamSynchronize(ctx->stream)))
     return MVD_DEVICE_ERROR;
   return MVD_SUCCESS;
 }

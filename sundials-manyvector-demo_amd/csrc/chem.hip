@@ -59,7 +59,7 @@ struct ChemPtrs {
 
 // calculate_rhs_cvklu (raja_primordial_network.cpp:1015-1389)
 __global__ void __launch_bounds__(256)
-chem_rhs_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *ydot) {
+chem_rhs_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *ydot, double srhs) {
 #pragma clang fp contract(off)
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -78,8 +78,9 @@ chem_rhs_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *ydot
 #pragma unroll
   for (int v = 0; v < NSP; v++) f[v] *= p.iscale[NSP * i + v];
   f[9] *= ax.inv_mdensity;
+  // times the caller's factor (TimeUnits for ffast/fimpl, N_VScale after the RHS; 1 exact)
 #pragma unroll
-  for (int v = 0; v < NSP; v++) ydot[NSP * i + v] = f[v];
+  for (int v = 0; v < NSP; v++) ydot[NSP * i + v] = srhs * f[v];
 }
 
 // Vector-Jacobian product of calculate_rhs_cvklu in the scaled variables:
@@ -142,7 +143,7 @@ chem_temperature_check_kernel(long n, ChemConst cc, const double *scale, const d
 // calculate_jacobian_cvklu (raja_primordial_network.cpp:1393-1965): dense column-major
 // blocks, each entry times inv_scale[row]*scale[col].
 __global__ void __launch_bounds__(256)
-chem_jac_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *J) {
+chem_jac_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *J, double sjac) {
 #pragma clang fp contract(off)
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -163,7 +164,7 @@ chem_jac_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *J) {
 #pragma unroll
   for (int c = 0; c < NSP; c++)
 #pragma unroll
-    for (int r = 0; r < NSP; r++) blk[NSP * c + r] = Jp[r][c] * (isc[r] * sc[c]);
+    for (int r = 0; r < NSP; r++) blk[NSP * c + r] = Jp[r][c] * (isc[r] * sc[c]) * sjac;
 }
 
 static ChemPtrs ptrs_of(ChemState &s) {
@@ -231,17 +232,19 @@ int chem_prepare(ChemState &s, double *chem, hipStream_t st) {
   return hipStreamSynchronize(st) == hipSuccess ? MVD_SUCCESS : MVD_DEVICE_ERROR;
 }
 
-int chem_rhs(ChemState &s, const double *y, double *ydot, hipStream_t st, bool sync) {
+int chem_rhs(ChemState &s, const double *y, double *ydot, hipStream_t st, bool sync,
+             double srhs) {
   hipLaunchKernelGGL(chem_rhs_kernel, dim3(nb(s.n)), dim3(256), 0, st, s.n, s.cc, ptrs_of(s), y,
-                     ydot);
+                     ydot, srhs);
   if (hipGetLastError() != hipSuccess) return MVD_DEVICE_ERROR;
   if (!sync) return MVD_SUCCESS;
   return hipStreamSynchronize(st) == hipSuccess ? MVD_SUCCESS : MVD_DEVICE_ERROR;
 }
 
-int chem_jac(ChemState &s, const double *y, double *J, hipStream_t st, bool sync) {
+int chem_jac(ChemState &s, const double *y, double *J, hipStream_t st, bool sync,
+             double sjac) {
   hipLaunchKernelGGL(chem_jac_kernel, dim3(nb(s.n)), dim3(256), 0, st, s.n, s.cc, ptrs_of(s), y,
-                     J);
+                     J, sjac);
   if (hipGetLastError() != hipSuccess) return MVD_DEVICE_ERROR;
   if (!sync) return MVD_SUCCESS;
   return hipStreamSynchronize(st) == hipSuccess ? MVD_SUCCESS : MVD_DEVICE_ERROR;

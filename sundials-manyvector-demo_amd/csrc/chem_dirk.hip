@@ -1022,7 +1022,7 @@ int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStrea
 // ---------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 ls_setup_kernel(long n, double gamma, const double *J, double *LU, unsigned long long *piv,
-                int *nsing) {
+                int *nsing, bool formed) {
 #pragma clang fp contract(off)
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1031,7 +1031,8 @@ ls_setup_kernel(long n, double gamma, const double *J, double *LU, unsigned long
 #pragma unroll
   for (int q = 0; q < NSP; q++)
 #pragma unroll
-    for (int r = 0; r < NSP; r++) a[r][q] = (r == q ? 1.0 : 0.0) - gamma * jb[NSP * q + r];
+    for (int r = 0; r < NSP; r++)
+      a[r][q] = formed ? jb[NSP * q + r] : (r == q ? 1.0 : 0.0) - gamma * jb[NSP * q + r];
   piv[i] = lu_factor_regs(a);
   bool sing = false;
   double *blk = LU + (long)NSP * NSP * i;
@@ -1058,10 +1059,11 @@ ls_solve_kernel(long n, const double *LU, const unsigned long long *piv, const d
 }
 
 int chem_ls_setup(long n, double gamma, const double *J, double *LU, unsigned long long *piv,
-                  int *nsing, hipStream_t st) {
+                  int *nsing, hipStream_t st, bool formed) {
   const unsigned nb = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, 1L, nsing, 0);
-  hipLaunchKernelGGL(ls_setup_kernel, dim3(nb), dim3(256), 0, st, n, gamma, J, LU, piv, nsing);
+  hipLaunchKernelGGL(ls_setup_kernel, dim3(nb), dim3(256), 0, st, n, gamma, J, LU, piv, nsing,
+                     formed);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -53,8 +53,11 @@ struct ChemState {
 
 int chem_setup(ChemState &s, const double *tables_host, long n, double z, hipStream_t st);
 int chem_prepare(ChemState &s, double *chem, hipStream_t st);
-int chem_rhs(ChemState &s, const double *y, double *ydot, hipStream_t st, bool sync);
-int chem_jac(ChemState &s, const double *y, double *J, hipStream_t st, bool sync);
+// srhs / sjac: factor applied to every entry after the reference's expressions (1: none)
+int chem_rhs(ChemState &s, const double *y, double *ydot, hipStream_t st, bool sync,
+             double srhs = 1.0);
+int chem_jac(ChemState &s, const double *y, double *J, hipStream_t st, bool sync,
+             double sjac = 1.0);
 int chem_vjp(ChemState &s, const double *y, const double *v, double *ybar, double *ydot,
              hipStream_t st, bool sync);
 int chem_temperature_check(ChemState &s, const double *y, const double *Ts0, long *bad,

@@ -51,9 +51,10 @@ size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K);
 int chem_adjoint_run(ChemState &s, const DirkDev &o, double h, int nsteps, int K, double *y,
                      const double *lamT, double *lam0, double *scratch, hipStream_t st,
                      int *host_fail);
-// block linear solver of the chemistry subvector (A = I - gamma J, batched LU)
+// block linear solver of the chemistry subvector (A = I - gamma J, batched LU); formed:
+// J already holds A (gamma unused)
 int chem_ls_setup(long n, double gamma, const double *J, double *LU, unsigned long long *piv,
-                  int *nsing, hipStream_t st);
+                  int *nsing, hipStream_t st, bool formed = false);
 int chem_ls_solve(long n, const double *LU, const unsigned long long *piv, const double *b,
                   double *x, hipStream_t st);
 

@@ -71,6 +71,8 @@ int fslow_prepare_launch(const Grid &g, double *const *w, const double *scale,
                          double eunit_inv, hipStream_t st);
 int fslow_finish_launch(const Grid &g, double *const *wdot, hipStream_t st);
 int chem_scale_launch(long n, double *chem, const double *scale, int apply, hipStream_t st);
+// N_VConst: z[0..len) = c
+int const_launch(double c, double *z, long len, hipStream_t st);
 // n cells, 10 species each: species rates / scale, energy slot * eunits / scale
 int chem_forcing_to_scaled_launch(long n, double *Fc, const double *scale, double eunits,
                                   hipStream_t st);

@@ -137,6 +137,18 @@ int fslow_finish_launch(const Grid &g, double *const *wdot, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
+__global__ void __launch_bounds__(256)
+nvec_const(long n, double c, double *z) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = c;
+}
+
+int const_launch(double c, double *z, long len, hipStream_t st) {
+  if (len <= 0) return 0;
+  hipLaunchKernelGGL(nvec_const, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, len, c, z);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
 int chem_scale_launch(long n, double *chem, const double *scale, int apply, hipStream_t st) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(chem_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, chem,
