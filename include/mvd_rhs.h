@@ -284,6 +284,9 @@ int mvd_chem_dirk_step_forced(mvd_context *ctx, double t, double h, const mvd_di
  * the number (> 0) of failed fast cell steps, MVD_ILLEGAL_STATE if any slow RHS met an
  * illegal state, or an error code. */
 #define MVD_MRI_CONSISTENT_FORCING 1
+/* Fast steps mvd_mri_evolve takes in slow stage `stage` (2..4, the stages after the first
+ * of Knoth-Wolke 3): ceil((c_stage - c_{stage-1}) h / hmax_fast), at least 1. */
+int mvd_mri_fast_steps(int stage, double h, double hmax_fast);
 int mvd_mri_evolve(mvd_context *ctx, double t0, double h, int nslow, double hmax_fast,
                    const mvd_dirk_opts *fast, int flags, double *const *w);
 

@@ -688,6 +688,11 @@ int mvd_mri_evolve(mvd_context *ctx, double, double h, int nslow, double hmax_fa
   return nfail;
 }
 
+int mvd_mri_fast_steps(int stage, double h, double hmax_fast) {
+  if (stage < 2 || stage > 4 || !(h > 0) || !(hmax_fast > 0)) return MVD_BAD_ARGUMENT;
+  return mvd::mri_fast_steps(stage - 1, h, hmax_fast);
+}
+
 int mvd_ark_ark324(mvd_ark_table *t) {
   if (!t) return MVD_BAD_ARGUMENT;
   std::memset(t, 0, sizeof(*t));

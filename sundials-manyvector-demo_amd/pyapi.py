@@ -100,6 +100,7 @@ def lib():
         "mvd_chem_jac_scaled": ([vp, C.c_double, C.c_double, _P, _P], C.c_int),
         "mvd_chem_lsetup_matrix": ([vp, _P], C.c_int),
         "mvd_nvec_const": ([vp, C.c_double, _P, C.c_long], C.c_int),
+        "mvd_mri_fast_steps": ([C.c_int, C.c_double, C.c_double], C.c_int),
         "mvd_postprocess_step": ([vp, C.c_double, _PP], C.c_int),
         "mvd_chem_state": ([vp, C.c_int], _P),
         "mvd_chem_lsetup": ([vp, C.c_double, _P], C.c_int),
@@ -286,6 +287,18 @@ def write_parameters(directory, params, options, t, h, iout):
     rc = io_lib().mvd_write_parameters(str(directory).encode(), C.byref(params),
                                        C.byref(options), t, h, iout)
     return _check("mvd_write_parameters", rc)
+
+
+def load_tables():
+    """The 50 primordial rate / cooling / gamma tables (50 x 1024, the committed blob in the
+    reference's read order, data/primordial_tables.json) for mvd_chem_setup."""
+    import numpy as np
+    return np.fromfile(TABLES_PATH, dtype="<f8").reshape(50, 1024)
+
+
+def mri_fast_steps(stage, h, hmax):
+    """Fast steps of slow stage `stage` (2..4) of mvd_mri_evolve."""
+    return lib().mvd_mri_fast_steps(stage, h, hmax)
 
 
 def dptr(t):

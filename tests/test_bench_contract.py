@@ -48,3 +48,25 @@ def test_committed_traffic_summary():
     alg = k["algorithmic_read_bytes"] + k["algorithmic_write_bytes"]
     assert alg == 2 * 5 * 8 * 256 ** 3
     assert alg <= t <= 1.25 * alg          # halo re-reads only, no spill traffic
+
+
+def test_chemistry_operation_counts_match_the_oracle_counter():
+    """bench.py's per-cell chemistry operation counts are what the oracle's op-counting
+    scalar counts through the restated reference expressions."""
+    import importlib.util as iu
+    import orc
+    b = load_bench()
+    spec = iu.spec_from_file_location("mvd_problems", os.path.join(
+        ROOT, "sundials-manyvector-demo_amd", "problems.py"))
+    problems = iu.module_from_spec(spec)
+    spec.loader.exec_module(problems)
+    y, _ = problems.chem_cells(1, seed=5)
+    ch = orc.Chem(1)
+    ys = y.copy()
+    ch.prepare(ys)
+    ch.extra()
+    rhs, jac = ch.opcount()
+    fl = lambda d: d["add"] + d["mul"] + d["div"] + d["sqrt"]
+    assert fl(rhs) == b.CHEM_RHS_FLOPS
+    assert fl(jac) == b.CHEM_JAC_FLOPS
+    assert rhs["trans"] == b.CHEM_RHS_TRANSCENDENTALS
