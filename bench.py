@@ -250,6 +250,48 @@ def bench_vjp(api, problems, torch, n, seed, reps):
     }
 
 
+def bench_fe_tracers(api, problems, torch, n, seed, reps):
+    """fe() with 10 advected species (the RHS of configurations 4 and 5) at n^3: the
+    configuration-4 fields (blast fluid, prepared species as tracers), periodic; the mean
+    of `reps` calls and the per-kernel HIP-event times."""
+    ncell = n ** 3
+    fluid = problems.blast3d(n, seed=seed)[:5]
+    y, _ = problems.chem_cells(ncell, seed=seed + 1)
+    ctx = api.Context(api.make_params(n, n, n, 10, (0,) * 6))
+    ctx.chem_setup(api.load_tables(), ncell)
+    chem = torch.from_numpy(y).to("cuda")
+    ctx.chem_prepare(chem)
+    w = [torch.from_numpy(a.reshape(-1)).to("cuda") for a in fluid] + [chem]
+    wd = [torch.empty_like(a) for a in w]
+    for _ in range(3):
+        ctx.fe(w, wd)
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.fe(w, wd)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    prof = ctx.profile_read()
+    ctx.close()
+    kernels, total_fl, total_ms = {}, 0, 0.0
+    for name, (kms, cnt) in prof.items():
+        if cnt == 0:
+            continue
+        avg = kms / cnt
+        fl = kernel_flops(name, n, 10)
+        kernels[name] = {"avg_ms": avg, "launches": cnt, "tflops": fl / (avg * 1e-3) / 1e12,
+                         "fp64_frac": fl / (avg * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+        total_fl += fl
+        total_ms += avg
+    return {"workload": f"fe() {n}^3, 10 tracers (blast seed {seed}, species seed {seed + 1}), "
+                        f"periodic", "ms_per_call": ms, "gcells": ncell / (ms * 1e-3) / 1e9,
+            "kernels": kernels, "kernel_sum_ms": total_ms,
+            "fp64_tflops": total_fl / (total_ms * 1e-3) / 1e12,
+            "fp64_frac": total_fl / (total_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+            "hbm_algorithmic_gbs": 16 * 15 * ncell / (total_ms * 1e-3) / 1e9}
+
+
 def bench_sod(api, problems, torch, cpu=True):
     """Config-1 Sod shock tube: 128x3x3 cells, Neumann, 2000 fixed steps of 1e-4 to t=0.2 with
     Bogacki-Shampine 3 (mvd_erk_evolve, one captured step replayed); the oracle's ERK on one
@@ -418,7 +460,7 @@ def main():
     ap.add_argument("--no-mri", action="store_true")
     ap.add_argument("--mri-n", type=int, default=128)
     ap.add_argument("--mri-steps", type=int, default=20)
-    ap.add_argument("--only", choices=["vjp", "sod", "mri"], default=None,
+    ap.add_argument("--only", choices=["vjp", "sod", "mri", "chem", "fe10"], default=None,
                     help="run one extra workload alone and print its JSON (profiling runs)")
     ap.add_argument("--vjp-n", type=int, default=128)
     ap.add_argument("--vjp-reps", type=int, default=10)
@@ -443,6 +485,12 @@ def main():
         elif args.only == "mri":
             print(json.dumps(bench_mri(api, problems, torch, args.mri_n, args.mri_steps,
                                        cpu=not args.no_cpu)))
+        elif args.only == "chem":
+            print(json.dumps(bench_chem(api, problems, torch, args.chem_n, 4321,
+                                        args.chem_steps)))
+        elif args.only == "fe10":
+            print(json.dumps(bench_fe_tracers(api, problems, torch, args.vjp_n, 777,
+                                              args.vjp_reps)))
         else:
             print(json.dumps(bench_sod(api, problems, torch, cpu=not args.no_cpu)))
         return

@@ -255,7 +255,7 @@ int mvd_chem_dirk_step(mvd_context *ctx, double t, double h, const mvd_dirk_opts
  * reference; for callers that cannot allocate device memory: code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc). Same kernels and the same results
  * bit for bit; the work buffer (cell order, sort keys, bucket counters, per-block statistics)
- * and the work-queue and statistics counters live in a library-owned array of 21.3 MiB per
+ * and the work-queue and statistics counters live in a library-owned array of 22.4 MiB per
  * GPU instead of the context's lazily allocated buffer, so the call allocates nothing.
  * Limits: at most 1048576 cells (mvd_chem_setup's ncell) and 4096 blocks in the persistent
  * grid, else MVD_BAD_ARGUMENT before any device work. Every call on one GPU shares that

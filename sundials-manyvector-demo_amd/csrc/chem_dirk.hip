@@ -117,6 +117,9 @@ __device__ inline void cell_matrix(const ChemConst &cc, const CellIn &c, double 
                     hg * (srhs * (J[NSP - 1][q] * (c.is(NSP - 1) * c.s(q))));
 }
 
+// Packed pivots of a factorisation without row exchanges (pivot k at step k).
+constexpr unsigned long long PIV_IDENTITY = 0x9876543210ULL;
+
 // LU with partial pivoting (first maximum wins, LAPACK getrf row interchanges) with the
 // whole matrix in registers (right-looking, predicated row exchanges so that the runtime
 // pivot never indexes a register array). Returns the pivots packed 4 bits each.
@@ -302,18 +305,24 @@ __device__ inline void cell_matrix_regs(const ChemConst &cc, const CellIn &c, do
                     hg * (srhs * (M[NSP - 1][q] * (c.is(NSP - 1) * c.s(q))));
 }
 
+// Solve with the factors of lu_factor_regs. The row exchanges are applied only when some
+// lane of the wave pivoted: for I - h*gamma*J of the stiff network the diagonal is the
+// column maximum in nearly every cell and step, and applying the packed pivots costs 180
+// selects per solve.
 __device__ inline void lu_solve_regs(const double (&a)[NSP][NSP], unsigned long long pv,
                                      double (&b)[NSP]) {
 #pragma clang fp contract(off)
+  if (__builtin_amdgcn_ballot_w64(pv != PIV_IDENTITY) != 0) {   // wave-uniform
 #pragma unroll
-  for (int k = 0; k < NSP; k++) {
-    const int p = (int)((pv >> (4 * k)) & 15);
+    for (int k = 0; k < NSP; k++) {
+      const int p = (int)((pv >> (4 * k)) & 15);
 #pragma unroll
-    for (int r = k + 1; r < NSP; r++) {
-      const bool sw = (r == p);
-      const double bk = b[k], br = b[r];
-      b[k] = sw ? br : bk;
-      b[r] = sw ? bk : br;
+      for (int r = k + 1; r < NSP; r++) {
+        const bool sw = (r == p);
+        const double bk = b[k], br = b[r];
+        b[k] = sw ? br : bk;
+        b[r] = sw ? bk : br;
+      }
     }
   }
 #pragma unroll
@@ -547,7 +556,33 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     next = __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));
   }
   __syncthreads();
+  // block statistics: one row per block; the last block to finish sums the rows into
+  // queue[1 .. NSTAT] (the ticket counter at queue[1 + NSTAT] starts at zero)
   if (threadIdx.x < NSTAT) rows[(long)blockIdx.x * NSTAT + threadIdx.x] = bs[threadIdx.x];
+  __threadfence();
+  __shared__ unsigned ticket;
+  if (threadIdx.x == 0) ticket = atomicAdd((unsigned *)(queue + 1 + NSTAT), 1u);
+  __syncthreads();
+  if (ticket != gridDim.x - 1) return;
+  __threadfence();
+  // each thread reads whole rows (independent loads, issued together), then LDS atomics
+  __shared__ int tot[NSTAT];
+  if (threadIdx.x < NSTAT) tot[threadIdx.x] = 0;
+  __syncthreads();
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+    int v[NSTAT];
+#pragma unroll
+    for (int st = 0; st < NSTAT; st++)
+      v[st] = __hip_atomic_load(rows + (long)b * NSTAT + st, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int st = 0; st < NSTAT; st++) {
+      if (st == STAT_MAXIT) atomicMax(&tot[st], v[st]);
+      else if (v[st]) atomicAdd(&tot[st], v[st]);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < NSTAT) queue[1 + threadIdx.x] = tot[threadIdx.x];
 }
 
 // ------------------------------------------------------------------------------------
@@ -675,6 +710,14 @@ fill_int(long n, int *a, int v) {
   if (i < n) a[i] = v;
 }
 
+// zero a[0..na) and b[0..nb) (b may be null)
+__global__ void __launch_bounds__(256)
+zero_two(long na, int *a, long nb, int *b) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) a[i] = 0;
+  if (b && i < nb) b[i] = 0;
+}
+
 __global__ void __launch_bounds__(256)
 copy_double(long n, const double *a, double *b) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -719,33 +762,52 @@ static int ensure_work(ChemState &s, size_t bytes) {
 static constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // ------------------------------------------------------------------------------------
-// Temperature order of the batched step: a counting sort of the cells by the table bin of
-// their temperature, in groups of 8 bins, so that the 64 lanes of a wave gather from the
-// same few lines of every rate table (and meet similar stiffness). The queue then hands
-// out positions of this order. Per-cell results do not depend on the order. The key
-// temperature is the solved temperature at y_n (the warm start alone may be a common
-// guess such as the 1000 K of a fresh preparation).
+// Cell order of the batched step: a counting sort of the cells by a two-level key, so that
+// the 64 lanes of a wave gather from the same few lines of every rate table and need
+// similar Newton work (a wave runs to its slowest cell). Major: the table bin of the
+// temperature at y_n in groups of 8 bins. Minor: the size of the first Newton correction,
+// classed by log10(h * max_s |y'_s(y_n)|) over the nine species in half decades over
+// [1e-8, 1) (16 classes; the energy row adds nothing to the grouping):
+// cells whose scaled rate change over the step is tiny converge in one iteration per
+// stage, the others in two or more, and the temperature alone does not tell them apart
+// (for the 64^3 synthetic cells the sum over chunks of the slowest cell's iterations is
+// 1.19 times the mean work in temperature order, 1.07 with the minor key). The queue then
+// hands out positions of this order. Per-cell results do not depend on the order. The
+// temperature is the solved one at y_n (the warm start alone may be a common guess such
+// as the 1000 K of a fresh preparation).
 // ------------------------------------------------------------------------------------
-constexpr int TORD_NBK = 128;          // keys: table bin (0..1022) / 8
-constexpr long TORD_MIN = 4096;        // smaller batches keep the natural order
-// Bucket counters sit 128 bytes apart (one cache line each) so that the blocks' atomics on
-// different buckets do not queue on the same line: total of bucket k at cnt[k * TORD_PAD],
-// its running offset at cnt[(TORD_NBK + k) * TORD_PAD].
-constexpr int TORD_PAD = 32;
+constexpr int TORD_TB = 128;                  // temperature buckets: table bin (0..1022) / 8
+constexpr int TORD_RC = 16;                   // rate classes within a temperature bucket
+constexpr int TORD_NBK = TORD_TB * TORD_RC;   // keys
+constexpr long TORD_MIN = 4096;               // smaller batches keep the natural order
+// Bucket counters sit 32 bytes apart so that the blocks' atomics on different buckets
+// rarely share an address: total of bucket k at cnt[k * TORD_PAD], its running offset at
+// cnt[(TORD_NBK + k) * TORD_PAD].
+constexpr int TORD_PAD = 8;
 
-// Keys (one byte per cell) and the bucket totals, accumulated per block in LDS first. One
-// cell per thread, blocks of TORD_BLOCK threads. The key is the temperature at y_n, solved
-// from the cell's warm start exactly as the step's first RHS evaluation solves it (same
-// function, same inputs, so the same bits); the solution goes to tdt and that evaluation
-// skips its solve.
-constexpr int TORD_BLOCK = 1024;
-__global__ void __launch_bounds__(TORD_BLOCK)
-tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, unsigned char *key, int *cnt,
-           double2 *tdt) {
+// rate class of a cell from m = h * max_s |y'_s|: 0 below 1e-8 (or not finite), 15 at 1 and
+// above, else floor(2 log10 m) + 16
+__device__ inline int tord_rate_class(double m) {
+  if (!(m > 1e-8)) return 0;
+  if (m >= 1.0) return TORD_RC - 1;
+  const int c = (int)floor(2.0 * log10(m)) + 16;
+  return c < 0 ? 0 : (c > TORD_RC - 1 ? TORD_RC - 1 : c);
+}
+
+// Keys (two bytes per cell) and the bucket totals, accumulated per block in LDS first. One
+// cell per thread, blocks of TORD_CBLOCK threads. The temperature is solved from the
+// cell's warm start exactly as the step's first RHS evaluation solves it (same function,
+// same inputs, so the same bits); the solution goes to tdt and that evaluation skips its
+// solve. The rate class uses the species rows of the RHS at that temperature (times the
+// RHS scale, plus the forcing when the step is forced).
+constexpr int TORD_CBLOCK = 256;
+__global__ void __launch_bounds__(TORD_CBLOCK)
+tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, const double *force, double h,
+           double srhs, unsigned short *key, int *cnt, double2 *tdt) {
 #pragma clang fp contract(off)
   __shared__ int hc[TORD_NBK];
   __shared__ double gl[4 * TLEN];        // the gamma tables of the temperature solve
-  if (threadIdx.x < TORD_NBK) hc[threadIdx.x] = 0;
+  for (int e = threadIdx.x; e < TORD_NBK; e += blockDim.x) hc[e] = 0;
   for (int e = threadIdx.x; e < 4 * TLEN; e += blockDim.x) gl[e] = cc.tab[G_GH2_1 * TLEN + e];
   __syncthreads();
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -755,33 +817,64 @@ tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, unsigned char *ke
     for (int s = 0; s < NSP; s++) ya[s] = y[NSP * i + s] * p.scale[NSP * i + s];
     chem_temperature(cc, gamma_tabs(gl), ya, T, dT);
     tdt[i] = make_double2(T, dT);
-    const int k = chem_bin(cc, log(T > cc.tmin ? (T < cc.tmax ? T : cc.tmax) : cc.tmin)) >> 3;
-    key[i] = (unsigned char)k;
+    double kr[NRATE], f[NSP];
+#pragma unroll
+    for (int r = 0; r < NRATE; r++) kr[r] = 0.0;
+    chem_rates_only_range<0, C_BREM>(cc, T, kr);
+    chem_rhs_species(ya, kr, f);
+    double m = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP - 1; s++) {
+      double v = f[s] * p.iscale[NSP * i + s] * srhs;
+      if (force) v += force[NSP * i + s];
+      v = fabs(v);
+      m = v > m ? v : m;
+    }
+    const int kt = chem_bin(cc, log(T > cc.tmin ? (T < cc.tmax ? T : cc.tmax) : cc.tmin)) >> 3;
+    const int k = kt * TORD_RC + tord_rate_class(h * m);
+    key[i] = (unsigned short)k;
     atomicAdd(&hc[k], 1);
   }
   __syncthreads();
-  if (threadIdx.x < TORD_NBK && hc[threadIdx.x])
-    atomicAdd(&cnt[threadIdx.x * TORD_PAD], hc[threadIdx.x]);
+  for (int e = threadIdx.x; e < TORD_NBK; e += blockDim.x)
+    if (hc[e]) atomicAdd(&cnt[e * TORD_PAD], hc[e]);
 }
 
 // order[start of bucket k + reserved range of this block + local rank] = cell. Each block
 // takes the exclusive prefix of the bucket totals, ranks its cells per bucket in LDS and
 // reserves one range per bucket from the running counters (zeroed beforehand).
+constexpr int TORD_BLOCK = 1024;
+static_assert(TORD_NBK == 2 * TORD_BLOCK, "the scan below gives each thread two buckets");
 __global__ void __launch_bounds__(TORD_BLOCK)
-tord_scatter(long n, const unsigned char *key, int *cnt, int *order) {
-  __shared__ int pre[TORD_NBK], hc[TORD_NBK], res[TORD_NBK];
-  const int t = threadIdx.x;
-  if (t < TORD_NBK) {
-    pre[t] = cnt[t * TORD_PAD];
-    hc[t] = 0;
+tord_scatter(long n, const unsigned short *key, int *cnt, int *order) {
+  __shared__ int pre[TORD_NBK], hc[TORD_NBK], res[TORD_NBK], wsum[TORD_BLOCK / 64];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int a = cnt[(2 * t) * TORD_PAD], b = cnt[(2 * t + 1) * TORD_PAD];
+  hc[2 * t] = 0;
+  hc[2 * t + 1] = 0;
+  // inclusive scan of the pair sums: within each wave by shuffles, then over the waves'
+  // totals by the first wave
+  int v = a + b;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  if (lane == 63) wsum[wid] = v;
+  __syncthreads();
+  if (wid == 0) {
+    int w = lane < TORD_BLOCK / 64 ? wsum[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < TORD_BLOCK / 64; o <<= 1) {
+      const int u = __shfl_up(w, o, 64);
+      if (lane >= o) w += u;
+    }
+    if (lane < TORD_BLOCK / 64) wsum[lane] = w;
   }
   __syncthreads();
-  for (int off = 1; off < TORD_NBK; off <<= 1) {           // inclusive scan of the totals
-    const int v = (t < TORD_NBK && t >= off) ? pre[t - off] : 0;
-    __syncthreads();
-    if (t < TORD_NBK) pre[t] += v;
-    __syncthreads();
-  }
+  const int ex = v + (wid > 0 ? wsum[wid - 1] : 0) - (a + b);   // totals of buckets < 2t
+  pre[2 * t] = ex + a;                                     // inclusive prefix per bucket
+  pre[2 * t + 1] = ex + a + b;
   const long i = (long)blockIdx.x * blockDim.x + t;
   int k = 0, r = 0;
   if (i < n) {
@@ -789,7 +882,8 @@ tord_scatter(long n, const unsigned char *key, int *cnt, int *order) {
     r = atomicAdd(&hc[k], 1);
   }
   __syncthreads();
-  if (t < TORD_NBK && hc[t]) res[t] = atomicAdd(&cnt[(TORD_NBK + t) * TORD_PAD], hc[t]);
+  for (int e = t; e < TORD_NBK; e += TORD_BLOCK)
+    if (hc[e]) res[e] = atomicAdd(&cnt[(TORD_NBK + e) * TORD_PAD], hc[e]);
   __syncthreads();
   if (i < n) order[(k ? pre[k - 1] : 0) + res[k] + r] = (int)i;
 }
@@ -835,14 +929,15 @@ static int batched_plan(long n, const DirkDev &o, int grid_waves, int block, boo
   p.b_rows = align256((size_t)p.nrow * NSTAT * sizeof(int));
   p.b_ord = p.sorted ? align256((size_t)n * sizeof(int)) : 0;
   p.b_cnt = p.sorted ? align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int)) : 0;
-  p.b_key = p.sorted ? align256((size_t)n) : 0;
+  p.b_key = p.sorted ? align256((size_t)n * sizeof(unsigned short)) : 0;
   p.b_tdt = p.sorted ? align256((size_t)n * sizeof(double2)) : 0;
   return 0;
 }
 
-// Queue one planned step: the temperature-bin order (unless reused), the counters cleared,
-// the persistent kernel and the statistics reduction into counters[1..NSTAT]. work holds
-// p.bytes(); counters 1 + NSTAT ints (the work-queue position, then the statistics).
+// Queue one planned step: the cell order (unless reused), the counters cleared, the
+// persistent kernel (whose last block sums the statistics into counters[1..NSTAT]). work
+// holds p.bytes(); counters 2 + NSTAT ints (the work-queue position, the statistics, the
+// kernel's finishing ticket).
 static int batched_enqueue(const BatchedPlan &p, ChemState &s, const DirkDev &o, double h,
                            double *y, const double *force, bool reuse_order, char *work,
                            int *counters, hipStream_t st) {
@@ -852,24 +947,24 @@ static int batched_enqueue(const BatchedPlan &p, ChemState &s, const DirkDev &o,
   double2 *tdt = nullptr;                                  // temperatures at y_n, if solved
   if (p.sorted && !reuse_order) {
     int *cnt = (int *)(work + p.b_rows + p.b_ord);
-    unsigned char *key = (unsigned char *)(work + p.b_rows + p.b_ord + p.b_cnt);
+    unsigned short *key = (unsigned short *)(work + p.b_rows + p.b_ord + p.b_cnt);
     tdt = (double2 *)(work + p.b_rows + p.b_ord + p.b_cnt + p.b_key);
-    hipLaunchKernelGGL(fill_int, dim3(2 * TORD_NBK * TORD_PAD / 256), dim3(256), 0, st,
-                       (long)(2 * TORD_NBK * TORD_PAD), cnt, 0);
+    hipLaunchKernelGGL(zero_two, dim3(2 * TORD_NBK * TORD_PAD / 256), dim3(256), 0, st,
+                       (long)(2 * TORD_NBK * TORD_PAD), cnt, (long)(2 + NSTAT), counters);
+    const unsigned cb = (unsigned)((n + TORD_CBLOCK - 1) / TORD_CBLOCK);
+    hipLaunchKernelGGL(tord_count, dim3(cb), dim3(TORD_CBLOCK), 0, st, n, s.cc, ptrs_d(s),
+                       (const double *)y, force, h, o.rhs_scale, key, cnt, tdt);
     const unsigned tb = (unsigned)((n + TORD_BLOCK - 1) / TORD_BLOCK);
-    hipLaunchKernelGGL(tord_count, dim3(tb), dim3(TORD_BLOCK), 0, st, n, s.cc, ptrs_d(s),
-                       (const double *)y, key, cnt, tdt);
     hipLaunchKernelGGL(tord_scatter, dim3(tb), dim3(TORD_BLOCK), 0, st, n,
-                       (const unsigned char *)key, cnt, order);
+                       (const unsigned short *)key, cnt, order);
   }
-  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(1 + NSTAT), counters, 0);
+  if (!(p.sorted && !reuse_order))   // (else cleared with the bucket counters above)
+    hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(2 + NSTAT), counters, 0);
   const unsigned lds = (unsigned)dirk_lds_bytes(p.wpb, o.nstage);
   const bool sub = o.max_halvings > 0;
   hipLaunchKernelGGL(p.kern, dim3((unsigned)p.grid), dim3(p.block), lds, st, n, s.cc,
                      ptrs_d(s), o, h, y, force, (const int *)order,
                      sub ? nullptr : (const double2 *)tdt, counters, rows);
-  hipLaunchKernelGGL(stats_reduce, dim3(stats_blocks(p.nrow)), dim3(256), 0, st,
-                     (const int *)rows, p.nrow, counters + 1);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -892,12 +987,12 @@ int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hip
 // race and must not be issued. Everything a step needs to start from zero (bucket counters,
 // work-queue position, statistics) is cleared by the step itself.
 // ------------------------------------------------------------------------------------
-constexpr size_t NOALLOC_COUNTERS = 256;                   // 1 + NSTAT ints, padded
+constexpr size_t NOALLOC_COUNTERS = 256;                   // 2 + NSTAT ints, padded
 constexpr size_t NOALLOC_WORK =
     align256((size_t)NOALLOC_MAX_ROWS * NSTAT * sizeof(int)) +
     align256((size_t)NOALLOC_MAX_CELLS * sizeof(int)) +
     align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int)) +
-    align256((size_t)NOALLOC_MAX_CELLS) +
+    align256((size_t)NOALLOC_MAX_CELLS * sizeof(unsigned short)) +
     align256((size_t)NOALLOC_MAX_CELLS * sizeof(double2));
 static __device__ __align__(256) unsigned char scratch[NOALLOC_COUNTERS + NOALLOC_WORK];
 
