@@ -323,11 +323,12 @@ int mvd_imex_evolve(mvd_context *ctx, double t0, double h, int nsteps, const mvd
  * by a thread-per-cell run of the plain solver's per-cell algorithm, keeping the state every
  * ckpt_every steps; then lam0 = (d y_N / d y_0)^T lamT (both [cell][10]) for the stage
  * equations solved exactly along that trajectory, with the exact stage Jacobians; each
- * segment is recomputed from its checkpoint. o must be stiffly accurate (the last row of A
- * sums to 1, as mvd_dirk_sdirk3). Ts/dTs end at the forward run's final state. Returns 0,
- * the number (> 0) of cells whose forward Newton budget ran out (lam0 is still computed
- * along the computed trajectory), or an error code. Device scratch held by the context:
- * (ceil(nsteps/ckpt_every) + ckpt_every) * 12 + 40 doubles per cell. */
+ * segment is recomputed once from its checkpoint, keeping its steps' stage values. o must
+ * be stiffly accurate (the last row of A sums to 1, as mvd_dirk_sdirk3). Ts/dTs end at the
+ * forward run's final state. Returns 0, the number (> 0) of cells whose forward Newton
+ * budget ran out (lam0 is still computed along the computed trajectory), or an error code.
+ * Device scratch held by the context: ceil(nsteps/ckpt_every) * 12
+ * + ckpt_every * (10 * nstage + 1) doubles per cell. */
 int mvd_chem_dirk_adjoint(mvd_context *ctx, double t, double h, int nsteps, int ckpt_every,
                           const mvd_dirk_opts *o, double *y, const double *lamT, double *lam0);
 

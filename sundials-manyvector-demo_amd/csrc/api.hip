@@ -773,7 +773,8 @@ int mvd_chem_dirk_adjoint(mvd_context *ctx, double, double h, int nsteps, int ck
   double cs = 0.0;
   for (int j = 0; j < d.nstage; j++) cs += d.A[(d.nstage - 1) * 4 + j];
   if (std::fabs(cs - 1.0) > 1e-12) return MVD_BAD_ARGUMENT;
-  const size_t need = mvd::chem_adjoint_scratch_doubles(ctx->chem.n, nsteps, ckpt_every);
+  const size_t need =
+      mvd::chem_adjoint_scratch_doubles(ctx->chem.n, nsteps, ckpt_every, d.nstage);
   if (need > ctx->adj_len) {
     if (ctx->adj_buf) (void)hipFree(ctx->adj_buf);
     ctx->adj_buf = nullptr;

@@ -1176,16 +1176,16 @@ int chem_ls_solve(long n, const double *LU, const unsigned long long *piv, const
 //   forward  - advances the cell nsteps steps with the per-cell step below, storing
 //              (y, Ts, dTs) every K steps;
 //   backward - segment by segment, last first: recomputes the segment's steps from its
-//              checkpoint (the per-step states kept in scratch), then walks them backwards;
-//              each step is recomputed from its state to get the stage values Y_k, and the
-//              exact stage Jacobians J_k = s_rhs dF/dy(Y_k) (rows from the exact
-//              vector-Jacobian product of the RHS) enter
+//              checkpoint, keeping every step's stage values Y_k and end temperature in
+//              scratch, then walks them backwards; the exact stage Jacobians
+//              J_k = s_rhs dF/dy(Y_k) (rows from the exact vector-Jacobian product of the
+//              RHS, its cotangent-independent part formed once per stage) enter
 //                for k = s..1: Ybar_k = h J_k^T (b_k lam + sum_{m>k} a_mk nu_m),
 //                              nu_k = (I - h a_kk J_k)^{-T} Ybar_k;   lam += sum_k nu_k,
 //              b = the last row of A (the stiffly accurate tables of mvd_chem_dirk_step).
 // The per-cell step is the plain solver's algorithm (modified Newton, J at y_n, refactor
 // after max_iters), thread-local, so that recomputation repeats the forward run bit for bit.
-// Scratch per cell (element-major, coalesced): (nckpt + K) x 12 + MAXSTAGE x 10 doubles.
+// Scratch per cell (element-major, coalesced): nckpt x 12 + K x (nstage x 10 + 1) doubles.
 // ---------------------------------------------------------------------------------
 __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, double &dTs,
                           const DirkDev &o, double h, double (&y)[NSP], double *Ys, long n,
@@ -1288,17 +1288,17 @@ __device__ inline void stage_adjoint_matrix(const ChemConst &cc, const CellIn &c
   double dT;
   chem_temperature(cc, ya, T, dT);
   const bool clamped = !(T > cc.tmin && T < cc.tmax);
+  ChemVjpPre P;                  // rates, gas-energy tangents, temperature path: once
+  chem_vjp_prepare(cc, ya, T, clamped, c.ax, P);
   double acc[NSP];
 #pragma unroll
   for (int q = 0; q < NSP; q++) acc[q] = 0.0;
 #pragma unroll
   for (int r = 0; r < NSP; r++) {
-    double vp[NSP], row[NSP];
-#pragma unroll
-    for (int s = 0; s < NSP; s++) vp[s] = 0.0;
-    vp[r] = c.is(r);
-    if (r == NSP - 1) vp[r] *= c.ax.inv_mdensity;
-    chem_vjp_physical(cc, ya, T, clamped, c.ax, vp, row);
+    // row r of the RHS Jacobian in scaled variables: the VJP of the cotangent is(r) e_r
+    // (times inv_mdensity on the gas-energy row)
+    double row[NSP];
+    chem_vjp_row(P, ya, r, r == NSP - 1 ? c.is(r) * c.ax.inv_mdensity : c.is(r), row);
 #pragma unroll
     for (int q = 0; q < NSP; q++) {
       const double J = row[q] * c.s(q);                 // scaled dF_r/dy_q
@@ -1312,7 +1312,7 @@ __device__ inline void stage_adjoint_matrix(const ChemConst &cc, const CellIn &c
 
 __global__ void __launch_bounds__(256)
 chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, int nsteps, int K,
-                      const double *ck, double *seg, double *Ys, const double *lamT,
+                      const double *ck, double *Te, double *Ys, const double *lamT,
                       double *lam0) {
 #pragma clang fp contract(off)
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1326,25 +1326,19 @@ chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, in
   for (int sg = nseg - 1; sg >= 0; sg--) {
     const int n0 = sg * K, n1 = n0 + K < nsteps ? n0 + K : nsteps;
     const double *rec = ck + (long)sg * ADJ_REC * n;
-    double z[NSP];
+    {
+      double z[NSP];
 #pragma unroll
-    for (int s = 0; s < NSP; s++) z[s] = rec[s * n + i];
-    double Ts = rec[NSP * n + i], dTs = rec[(NSP + 1) * n + i];
-    for (int st = n0; st < n1; st++) {
-      double *sr = seg + (long)(st - n0) * ADJ_REC * n;
-#pragma unroll
-      for (int s = 0; s < NSP; s++) sr[s * n + i] = z[s];
-      sr[NSP * n + i] = Ts;
-      sr[(NSP + 1) * n + i] = dTs;
-      if (st + 1 < n1) (void)cell_step(cc, c, Ts, dTs, o, h, z, nullptr, n, i);
+      for (int s = 0; s < NSP; s++) z[s] = rec[s * n + i];
+      double Ts = rec[NSP * n + i], dTs = rec[(NSP + 1) * n + i];
+      for (int st = n0; st < n1; st++) {
+        (void)cell_step(cc, c, Ts, dTs, o, h, z, Ys + (long)(st - n0) * ns * NSP * n, n, i);
+        Te[(long)(st - n0) * n + i] = Ts;   // the warm start of the step's stage solves
+      }
     }
     for (int st = n1 - 1; st >= n0; st--) {
-      const double *sr = seg + (long)(st - n0) * ADJ_REC * n;
-#pragma unroll
-      for (int s = 0; s < NSP; s++) z[s] = sr[s * n + i];
-      Ts = sr[NSP * n + i];
-      dTs = sr[(NSP + 1) * n + i];
-      (void)cell_step(cc, c, Ts, dTs, o, h, z, Ys, n, i);
+      const double Ts = Te[(long)(st - n0) * n + i];
+      const double *Yst = Ys + (long)(st - n0) * ns * NSP * n;
       double nu[MAXSTAGE][NSP];
 #pragma unroll
       for (int k = 0; k < MAXSTAGE; k++)
@@ -1362,7 +1356,7 @@ chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, in
             for (int m = k + 1; m < MAXSTAGE; m++)
               if (m < ns) acc += dirk_coef(o, m, k) * nu[m][s];
             w[s] = acc;
-            Y[s] = Ys[(long)(k * NSP + s) * n + i];
+            Y[s] = Yst[(long)(k * NSP + s) * n + i];
           }
           double Mt[NSP][NSP], yb[NSP];
           stage_adjoint_matrix(cc, c, Ts, Y, o.rhs_scale, h, h * o.A[k * 4 + k], w, Mt, yb);
@@ -1383,9 +1377,9 @@ chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, in
   for (int s = 0; s < NSP; s++) lam0[NSP * i + s] = lam[s];
 }
 
-size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K) {
+size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K, int nstage) {
   const long nckpt = (nsteps + K - 1) / K;
-  return (size_t)n * ((nckpt + K) * ADJ_REC + MAXSTAGE * NSP);
+  return (size_t)n * (nckpt * ADJ_REC + (long)K * (nstage * NSP + 1));
 }
 
 int chem_adjoint_run(ChemState &s, const DirkDev &o, double h, int nsteps, int K, double *y,
@@ -1394,14 +1388,14 @@ int chem_adjoint_run(ChemState &s, const DirkDev &o, double h, int nsteps, int K
   const long n = s.n;
   const unsigned nbk = (unsigned)((n + 255) / 256);
   const long nckpt = (nsteps + K - 1) / K;
-  double *ck = scratch, *seg = ck + nckpt * ADJ_REC * n, *Ys = seg + (long)K * ADJ_REC * n;
+  double *ck = scratch, *Te = ck + nckpt * ADJ_REC * n, *Ys = Te + (long)K * n;
   const ChemPtrsD p = ptrs_d(s);
   int *nfail = s.counters;
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, 1L, nfail, 0);
   hipLaunchKernelGGL(chem_adjoint_forward, dim3(nbk), dim3(256), 0, st, n, s.cc, p, o, h, nsteps,
                      K, y, ck, nfail);
   hipLaunchKernelGGL(chem_adjoint_backward, dim3(nbk), dim3(256), 0, st, n, s.cc, p, o, h,
-                     nsteps, K, (const double *)ck, seg, Ys, lamT, lam0);
+                     nsteps, K, (const double *)ck, Te, Ys, lamT, lam0);
   if (hipGetLastError() != hipSuccess) return -1;
   if (hipMemcpyAsync(host_fail, nfail, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)

@@ -441,30 +441,32 @@ __device__ constexpr signed char CHEM_STO[NSP - 1][NREACT] = {
 // differentiated in reverse through the reactions, the gas-energy row in forward mode
 // along each species it depends on and along T. `clamped`: T sits on a clamp bound, so it
 // does not depend on y.
-__device__ inline void chem_vjp_physical(const ChemConst &c, const double (&y)[NSP], double T,
-                                         bool clamped, const CellAux &ax,
-                                         const double (&vp)[NSP], double (&yab)[NSP]) {
+//
+// In two parts: chem_vjp_prepare forms what does not depend on vp (reaction rates and
+// their T-derivatives, the gas-energy row's tangents, the temperature path's dg/dT and
+// dg/dy), chem_vjp_apply contracts one cotangent with it; chem_vjp_row gives the row
+// vp = v e_r with the same arithmetic restricted to its nonzero terms (the exact Jacobian
+// one row at a time, for many cotangents at one state).
+struct ChemVjpPre {
+  double k[NREACT], dk[NREACT];   // reaction rates and T-derivatives (reaction order)
+  double e9[NSP];                 // d f_9 / d y_s along H2_1, H_1, H_2, He_1..3, de; [9]: d/dT
+  double dgdT, dgdy[NSP - 1];     // temperature path (not formed when clamped)
+  bool clamped;
+};
+
+__device__ inline void chem_vjp_prepare(const ChemConst &c, const double (&y)[NSP], double T,
+                                        bool clamped, const CellAux &ax, ChemVjpPre &P) {
 #pragma clang fp contract(off)
   double k[NRATE], dk[NRATE];
   chem_rates(c, T, k, dk);
 #pragma unroll
-  for (int s = 0; s < NSP; s++) yab[s] = 0.0;
-  double Tb = 0.0;
-  // species rows: reverse through r_j = k_j * prod(reactants)
-#pragma unroll
   for (int j = 0; j < NREACT; j++) {
-    double rb = 0.0;
-#pragma unroll
-    for (int s = 0; s < NSP - 1; s++)
-      if (CHEM_STO[s][j] != 0) rb += (double)CHEM_STO[s][j] * vp[s];
-    const int a = CHEM_REACTANTS[j][0], b = CHEM_REACTANTS[j][1], cc = CHEM_REACTANTS[j][2];
-    const double ya = y[a], yb = y[b], yc = cc >= 0 ? y[cc] : 1.0;
-    yab[a] += rb * (k[j] * yb * yc);
-    yab[b] += rb * (k[j] * ya * yc);
-    if (cc >= 0) yab[cc] += rb * (k[j] * ya * yb);
-    Tb += rb * (dk[j] * ya * yb * yc);
+    P.k[j] = k[j];
+    P.dk[j] = dk[j];
   }
   // gas-energy row: forward tangents along H2_1, H_1, H_2, He_1, He_2, He_3, de and T
+#pragma unroll
+  for (int s = 0; s < NSP; s++) P.e9[s] = 0.0;
   {
     Dual1 yd[NSP], kd[NRATE];
     constexpr int dirs[8] = {0, 2, 3, 5, 6, 7, 8, NSP};
@@ -477,10 +479,13 @@ __device__ inline void chem_vjp_physical(const ChemConst &c, const double (&y)[N
       for (int r = 0; r < NRATE; r++) kd[r] = Dual1{k[r], s == NSP ? dk[r] : 0.0};
       const Dual1 f9 = chem_rhs_energy_t<Dual1>(yd, kd, Dual1{T, s == NSP ? 1.0 : 0.0},
                                                 c.z, ax);
-      if (s == NSP) Tb += vp[9] * f9.d;
-      else yab[s] += vp[9] * f9.d;
+      P.e9[s == NSP ? NSP - 1 : s] = f9.d;   // the T tangent in slot 9 (ge has none)
     }
   }
+  P.clamped = clamped;
+  P.dgdT = 1.0;
+#pragma unroll
+  for (int s = 0; s < NSP - 1; s++) P.dgdy[s] = 0.0;
   if (clamped) return;
   // temperature path
   const double kb = 1.3806504e-16, mh = 1.67e-24;
@@ -508,14 +513,86 @@ __device__ inline void chem_vjp_physical(const ChemConst &c, const double (&y)[N
     density += ws[s] * y[s];
   }
   const double dm = density * mh;
-  const double dgdT = kb * nsum / dm + T * kb * ((-y[0]) * g1 * g1 * sl1 - y[1] * g2 * g2 * sl2) / dm;
-  const double TbT = Tb / dgdT;
+  P.dgdT = kb * nsum / dm + T * kb * ((-y[0]) * g1 * g1 * sl1 - y[1] * g2 * g2 * sl2) / dm;
 #pragma unroll
-  for (int s = 0; s < NSP - 1; s++) {
-    const double dgdy = T * kb / mh * (cs[s] * density - nsum * ws[s]) / (density * density);
-    yab[s] -= TbT * dgdy;
-  }
+  for (int s = 0; s < NSP - 1; s++)
+    P.dgdy[s] = T * kb / mh * (cs[s] * density - nsum * ws[s]) / (density * density);
+}
+
+// The temperature path's share of yab for the T-cotangent Tb.
+__device__ inline void chem_vjp_temperature(const ChemVjpPre &P, double Tb,
+                                            double (&yab)[NSP]) {
+#pragma clang fp contract(off)
+  if (P.clamped) return;
+  const double TbT = Tb / P.dgdT;
+#pragma unroll
+  for (int s = 0; s < NSP - 1; s++) yab[s] -= TbT * P.dgdy[s];
   yab[NSP - 1] += TbT;                                            // dg/dge = -1
+}
+
+__device__ inline void chem_vjp_apply(const ChemVjpPre &P, const double (&y)[NSP],
+                                      const double (&vp)[NSP], double (&yab)[NSP]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int s = 0; s < NSP; s++) yab[s] = 0.0;
+  double Tb = 0.0;
+  // species rows: reverse through r_j = k_j * prod(reactants)
+#pragma unroll
+  for (int j = 0; j < NREACT; j++) {
+    double rb = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSP - 1; s++)
+      if (CHEM_STO[s][j] != 0) rb += (double)CHEM_STO[s][j] * vp[s];
+    const int a = CHEM_REACTANTS[j][0], b = CHEM_REACTANTS[j][1], cc = CHEM_REACTANTS[j][2];
+    const double ya = y[a], yb = y[b], yc = cc >= 0 ? y[cc] : 1.0;
+    yab[a] += rb * (P.k[j] * yb * yc);
+    yab[b] += rb * (P.k[j] * ya * yc);
+    if (cc >= 0) yab[cc] += rb * (P.k[j] * ya * yb);
+    Tb += rb * (P.dk[j] * ya * yb * yc);
+  }
+  // gas-energy row
+  constexpr int dirs[7] = {0, 2, 3, 5, 6, 7, 8};
+#pragma unroll
+  for (int q = 0; q < 7; q++) yab[dirs[q]] += vp[9] * P.e9[dirs[q]];
+  Tb += vp[9] * P.e9[NSP - 1];
+  chem_vjp_temperature(P, Tb, yab);
+}
+
+// chem_vjp_apply for vp = v e_r (row r of the Jacobian times v): the terms of the other
+// rows, all zero, are left out.
+__device__ inline void chem_vjp_row(const ChemVjpPre &P, const double (&y)[NSP], int r,
+                                    double v, double (&row)[NSP]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int s = 0; s < NSP; s++) row[s] = 0.0;
+  double Tb = 0.0;
+  if (r < NSP - 1) {
+#pragma unroll
+    for (int j = 0; j < NREACT; j++) {
+      if (CHEM_STO[r][j] == 0) continue;
+      const double rb = (double)CHEM_STO[r][j] * v;
+      const int a = CHEM_REACTANTS[j][0], b = CHEM_REACTANTS[j][1], cc = CHEM_REACTANTS[j][2];
+      const double ya = y[a], yb = y[b], yc = cc >= 0 ? y[cc] : 1.0;
+      row[a] += rb * (P.k[j] * yb * yc);
+      row[b] += rb * (P.k[j] * ya * yc);
+      if (cc >= 0) row[cc] += rb * (P.k[j] * ya * yb);
+      Tb += rb * (P.dk[j] * ya * yb * yc);
+    }
+  } else {
+    constexpr int dirs[7] = {0, 2, 3, 5, 6, 7, 8};
+#pragma unroll
+    for (int q = 0; q < 7; q++) row[dirs[q]] += v * P.e9[dirs[q]];
+    Tb += v * P.e9[NSP - 1];
+  }
+  chem_vjp_temperature(P, Tb, row);
+}
+
+__device__ inline void chem_vjp_physical(const ChemConst &c, const double (&y)[NSP], double T,
+                                         bool clamped, const CellAux &ax,
+                                         const double (&vp)[NSP], double (&yab)[NSP]) {
+  ChemVjpPre P;
+  chem_vjp_prepare(c, y, T, clamped, ax, P);
+  chem_vjp_apply(P, y, vp, yab);
 }
 
 __device__ inline void chem_rhs_physical(const double (&y)[NSP], const double (&k)[NRATE],

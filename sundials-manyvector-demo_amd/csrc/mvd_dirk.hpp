@@ -47,7 +47,7 @@ int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStrea
 // threads per block of a batched launch (blocks split to fit the LDS limit; 0: none fits)
 int dirk_fit_block(int block, int nstage, size_t static_lds, size_t limit);
 // checkpointed discrete adjoint of nsteps DIRK steps (checkpoint every K steps)
-size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K);
+size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K, int nstage);
 int chem_adjoint_run(ChemState &s, const DirkDev &o, double h, int nsteps, int K, double *y,
                      const double *lamT, double *lam0, double *scratch, hipStream_t st,
                      int *host_fail);
