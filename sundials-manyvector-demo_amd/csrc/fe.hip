@@ -13,13 +13,12 @@
 //                    primitives staged in LDS), and the divergence of the 31 cells between
 //                    a tile's 32 faces (tiles overlap by one face), adding the x/y part
 //                    read from wdot at the kernel's start.
-// Fast path with tracers (three launches):
-//   fe_xface       : x-faces, one thread per face; a block stages the primitives of the
-//                    points its faces need in LDS once, then every thread reads its
-//                    6-point stencil from LDS; also performs the legal-state check;
-//   fe_yface       : y-faces the same way, tiles of 16 x-cells by 16 faces;
-//   fe_zface_div   : z-faces (tiles of 16 x-cells by 16 faces) with the species, reading
-//                    the x- and y-face fluxes for the divergence, and writes wdot.
+// Fast path with tracers (two launches):
+//   fe_xyface_div<NC>: as without tracers, then the tracers on the same x- and y-faces
+//                    (their values staged in LDS as [tracer][point] after the fluid phase)
+//                    and their part of the divergence into wdot;
+//   fe_zface_div   : z-faces (tiles of 16 x-cells by 16 faces), fluid then tracers from
+//                    one LDS tile each, adding their part of the divergence.
 // The divergence is accumulated as -((dFx/dx + dFy/dy) + dFz/dz), the reference's
 // association (utilities.cpp:202-207).
 #include "mvd_device.hpp"
@@ -183,111 +182,52 @@ __device__ inline Pt to_point(const RawPt &q, double gamma) {
   return make_point(q.r, q.mn, q.m1, q.m2, q.e, gamma);
 }
 
-// x-faces. A block holds XROWS rows x 64 consecutive faces. The 69 points the 64 faces
-// need (faces i0..i0+63 use points i0-3..i0+65) are converted once into LDS (SoA), then
-// each thread computes one face from LDS. Fx layout [component][face], global order.
-constexpr int XROWS = 4;
-constexpr int XPTS = 64 + 5;
-
-template <int NC>
-__global__ void __launch_bounds__(256)
-fe_xface(Grid g, Fields f, double *Fx, unsigned *flag) {
-  __shared__ double sh[9][XROWS][XPTS];
-  __shared__ int scell[XROWS][XPTS];
-  __shared__ double ssgn[XROWS][XPTS];
-  const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
-  const int nxp = g.nx + 1;
-  const int nchunk = (nxp + 63) >> 6;
-  // XCD-aware numbering (as in fe_xyface_div): neighbouring chunks of a line share halo
-  // lines and run on the same XCD one after another
-  const unsigned m8 = gridDim.x / 8;
-  const int bid =
-      blockIdx.x < 8 * m8 ? (int)((blockIdx.x % 8) * m8 + blockIdx.x / 8) : (int)blockIdx.x;
-  const int chunk = bid % nchunk;
-  const int line = (bid / nchunk) * XROWS + row;             // j + ny*k
-  const int nline = g.ny * g.nz;
-  const bool row_ok = line < nline;
-  const int i0 = chunk * 64;
-  // stage points i0-3+s, s = lane and (for lanes < 5) 64+lane
-  // the block's XROWS x XPTS points, spread over all threads of the block
-  for (int q2 = threadIdx.x; q2 < XROWS * XPTS; q2 += blockDim.x) {
-    const int rw = q2 / XPTS, s = q2 - rw * XPTS;
-    const int ln = (bid / nchunk) * XROWS + rw;
-    const int e = i0 - 3 + s;
-    if (ln < nline && e <= g.nx + 2) {
-      double sa, sn;
-      const int c = line_cell<0>(g, ln * g.nx, 1, g.nx, e, sa, sn);
-      const RawPt q = load_raw<0>(f, c, sa, sn);
-      const Pt P = to_point(q, g.gamma);
-      sh[0][rw][s] = P.r;  sh[1][rw][s] = P.mn; sh[2][rw][s] = P.m1;
-      sh[3][rw][s] = P.m2; sh[4][rw][s] = P.e;  sh[5][rw][s] = P.p;
-      sh[6][rw][s] = P.ir; sh[7][rw][s] = P.a;  sh[8][rw][s] = P.sr;
-      scell[rw][s] = c;
-      ssgn[rw][s] = sa;
-      if (e >= 0 && e < g.nx) {   // each cell is checked once (legal_state)
-        unsigned bits = 0;
-        if (!(q.r > 0.0)) bits |= 1u;
-        if (!(q.e > 0.0)) bits |= 2u;
-        if (!(P.p > 0.0)) bits |= 4u;
-        if (bits) atomicOr(flag, bits);
-      }
-    }
-  }
-  __syncthreads();
-  const int i = i0 + lane;
-  if (!row_ok || i > g.nx) return;
-  const double *pbase = &sh[0][row][lane];
-  double F[5], alpha;
-  face_flux_lds(pbase, XROWS * XPTS, 1, g.gamma, F, alpha);
-  const long nf = (long)nxp * nline;
-  const long face = (long)line * nxp + i;
-#pragma unroll
-  for (int a = 0; a < 5; a++) Fx[a * nf + face] = F[a];
-  double un[6];
-  if (NC > 0) lds_velocities(pbase, XROWS * XPTS, 1, un);
-#pragma unroll 1
-  for (int v = 0; v < NC; v++) {
-    double cv[6];
-#pragma unroll
-    for (int l = 0; l < 6; l++)
-      cv[l] = f.chem[(long)NC * scell[row][lane + l] + v] * ssgn[row][lane + l];
-    Fx[(5 + v) * nf + face] = tracer_face_flux(cv, un, alpha);
-  }
-}
-
-// y- and z-faces: one thread per face; a block covers TX consecutive x-cells and TF
-// consecutive faces along the sweep direction. The TF+5 points of each column are
-// converted once into LDS (SoA, conflict-free), then every thread reads its 6-point
-// stencil from LDS.
+// z-faces with tracers: one thread per face; a block covers TX consecutive x-cells and TF
+// consecutive faces. The TF+5 points of each column are converted once into LDS (SoA,
+// conflict-free), with each point's cell and ghost sign for the tracer reads, then every
+// thread reads its 6-point stencil from LDS.
 struct TileLds {
   double *pt;      // [9][TPTS][TX]
   int *cell;       // [TPTS][TX]
   double *sgn;     // [TPTS][TX]
 };
 
+// Blocks of 256 threads. A thread's points (s = t + 256 r) are located first and all their
+// loads issued before any is converted, so that their latencies overlap.
 template <int D, int TX, int TF>
 __device__ inline void stage_tile(const Grid &g, const Fields &f, const TileLds &L, int x0,
                                   int t, int e0) {
-  constexpr int TPTS = TF + 5;
+  constexpr int TPTS = TF + 5, FS = TPTS * TX, NR = (FS + 255) / 256;
   const int n = (D == 1) ? g.ny : g.nz;
   const int stride = (D == 1) ? g.nx : g.nx * g.ny;
-  for (int s = threadIdx.x; s < TPTS * TX; s += blockDim.x) {
-    const int r = s / TX, x = s - r * TX;
+  int cs[NR];
+  double sa[NR], sn[NR];
+  bool use[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int s = threadIdx.x + 256 * r;
+    const int row = s / TX, x = s - row * TX;
     const int i = x0 + x;
-    const int e = e0 - 3 + r;
-    if (i < g.nx && e <= n + 2) {
-      const int base = (D == 1) ? i + g.nx * g.ny * t : i + g.nx * t;
-      double sa, sn;
-      const int c = line_cell<D>(g, base, stride, n, e, sa, sn);
-      const Pt P = to_point(load_raw<D>(f, c, sa, sn), g.gamma);
-      double *pt = L.pt + r * TX + x;
-      constexpr int FS = TPTS * TX;
-      pt[0 * FS] = P.r;  pt[1 * FS] = P.mn; pt[2 * FS] = P.m1;
-      pt[3 * FS] = P.m2; pt[4 * FS] = P.e;  pt[5 * FS] = P.p;
-      pt[6 * FS] = P.ir; pt[7 * FS] = P.a;  pt[8 * FS] = P.sr;
-      L.cell[r * TX + x] = c;
-      L.sgn[r * TX + x] = sa;
-    }
+    const int e = e0 - 3 + row;
+    use[r] = s < FS && i < g.nx && e <= n + 2;
+    const int base = (D == 1) ? i + g.nx * g.ny * t : i + g.nx * t;
+    cs[r] = use[r] ? line_cell<D>(g, base, stride, n, e, sa[r], sn[r]) : 0;
+  }
+  RawPt raw[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+    if (use[r]) raw[r] = load_raw<D>(f, cs[r], sa[r], sn[r]);
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    if (!use[r]) continue;
+    const int s = threadIdx.x + 256 * r;
+    const Pt P = to_point(raw[r], g.gamma);
+    double *pt = L.pt + s;
+    pt[0 * FS] = P.r;  pt[1 * FS] = P.mn; pt[2 * FS] = P.m1;
+    pt[3 * FS] = P.m2; pt[4 * FS] = P.e;  pt[5 * FS] = P.p;
+    pt[6 * FS] = P.ir; pt[7 * FS] = P.a;  pt[8 * FS] = P.sr;
+    L.cell[s] = cs[r];
+    L.sgn[s] = sa[r];
   }
 }
 
@@ -300,62 +240,27 @@ __device__ inline void to_global(const double (&Fl)[5], double *Fg) {
   else { Fg[3] = Fl[1]; Fg[2] = Fl[2]; Fg[1] = Fl[3]; }
 }
 
-// y-faces -> Fy [component][i + nx*(j + (ny+1)*k)]
+// z-faces with tracers, adding their part to the x/y part that fe_xyface_div left in out:
+// a block computes TF z-faces for TX columns and the TF-1 cells between them (tiles overlap
+// by one face),
+//   wdot = 0 - ((x/y part) + (Fz[k+1]-Fz[k])/dz).
+// The TF+5 fluid points of each column are converted once into LDS for the fluid fluxes;
+// then the tile's tracer values (one contiguous AoS read of a point's NC values per thread
+// and point, all issued together) take their place as [tracer][point] for the tracer
+// fluxes, and the fluxes of all 5+NC components go through LDS for the divergence.
 template <int NC, int TX, int TF>
 __global__ void __launch_bounds__(256)
-fe_yface(Grid g, Fields f, double *Fy) {
-  constexpr int TPTS = TF + 5;
-  __shared__ double s_pt[9 * TPTS * TX];
-  __shared__ int s_cell[TPTS * TX];
-  __shared__ double s_sgn[TPTS * TX];
-  const TileLds L{s_pt, s_cell, s_sgn};
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  const int nyp = g.ny + 1;
-  const int nxc = (g.nx + TX - 1) / TX, nfc = (nyp + TF - 1) / TF;
-  int b = blockIdx.x;
-  const int xc = b % nxc;
-  b /= nxc;
-  const int fc = b % nfc;
-  const int k = b / nfc;
-  const int j0 = fc * TF;
-  stage_tile<1, TX, TF>(g, f, L, xc * TX, k, j0);
-  __syncthreads();
-  const int i = xc * TX + tx, j = j0 + ty;
-  if (i >= g.nx || j > g.ny) return;
-  const double *pbase = L.pt + ty * TX + tx;
-  double Fl[5], alpha, un[6], Fg[5];
-  face_flux_lds(pbase, TPTS * TX, TX, g.gamma, Fl, alpha);
-  if (NC > 0) lds_velocities(pbase, TPTS * TX, TX, un);
-  to_global<1>(Fl, Fg);
-  const long nf = (long)g.nx * nyp * g.nz;
-  const long face = i + (long)g.nx * (j + (long)nyp * k);
-#pragma unroll
-  for (int a = 0; a < 5; a++) Fy[a * nf + face] = Fg[a];
-#pragma unroll 1
-  for (int v = 0; v < NC; v++) {
-    double cv[6];
-#pragma unroll
-    for (int l = 0; l < 6; l++)
-      cv[l] = f.chem[(long)NC * s_cell[(ty + l) * TX + tx] + v] * s_sgn[(ty + l) * TX + tx];
-    Fy[(5 + v) * nf + face] = tracer_face_flux(cv, un, alpha);
-  }
-}
-
-// z-faces with the divergence fused: a block computes TF z-faces per column and forms
-// wdot for the TF-1 cells between them (tiles overlap by one face):
-//   wdot = 0 - (((Fx[i+1]-Fx[i])/dx + (Fy[j+1]-Fy[j])/dy) + (Fz[k+1]-Fz[k])/dz)
-// XY: the x and y part is already in out (fe_xyface_div) and is read from there.
-template <int NC, int TX, int TF, bool XY>
-__global__ void __launch_bounds__(256)
-fe_zface_div(Grid g, Fields f, const double *Fx, const double *Fy, FieldsOut out, double idx,
-             double idy, double idz) {
+fe_zface_div(Grid g, Fields f, FieldsOut out, double idz) {
   constexpr int NV = 5 + NC;
   constexpr int TPTS = TF + 5;
   constexpr int TC = TF - 1;
-  constexpr int PT_SZ = 9 * TPTS * TX, FL_SZ = NV * TF * TX;
-  __shared__ double s_buf[PT_SZ > FL_SZ ? PT_SZ : FL_SZ];
-  __shared__ int s_cell[TPTS * TX];
-  __shared__ double s_sgn[TPTS * TX];
+  constexpr int NPT = TPTS * TX;
+  constexpr int PT_SZ = 9 * NPT, TR_SZ = NC * NPT, FL_SZ = NV * TF * TX;
+  constexpr int BUF = PT_SZ > FL_SZ ? (PT_SZ > TR_SZ ? PT_SZ : TR_SZ)
+                                    : (FL_SZ > TR_SZ ? FL_SZ : TR_SZ);
+  __shared__ double s_buf[BUF];
+  __shared__ int s_cell[NPT];
+  __shared__ double s_sgn[NPT];
   const TileLds L{s_buf, s_cell, s_sgn};
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int nxc = (g.nx + TX - 1) / TX, ncc = (g.nz + TC - 1) / TC;
@@ -369,46 +274,66 @@ fe_zface_div(Grid g, Fields f, const double *Fx, const double *Fy, FieldsOut out
   __syncthreads();
   const int i = xc * TX + tx, k = k0 + ty;
   const bool face_ok = i < g.nx && k <= g.nz;
-  double Fz[NV];
+  double Fz[NV], alpha = 0.0, un[6];
   if (face_ok) {
     const double *pbase = L.pt + ty * TX + tx;
-    double Fl[5], alpha, un[6];
-    face_flux_lds(pbase, TPTS * TX, TX, g.gamma, Fl, alpha);
-    if (NC > 0) lds_velocities(pbase, TPTS * TX, TX, un);
+    double Fl[5];
+    face_flux_lds(pbase, NPT, TX, g.gamma, Fl, alpha);
+    lds_velocities(pbase, NPT, TX, un);
     to_global<2>(Fl, Fz);
-#pragma unroll 1
+  }
+  // the x/y part of the thread's cell and the tracer values of its points (staged points
+  // only: stage_tile's bounds), all loaded now, so that their latency passes under the
+  // tracer staging and fluxes
+  const bool cell_ok = ty < TC && i < g.nx && k < g.nz;
+  const long cell = i + (long)g.nx * (j + (long)g.ny * k);
+  double dxy[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++)
+    dxy[v] = !cell_ok ? 0.0 : (v < 5 ? out.w[v][cell] : out.chem[(long)NC * cell + (v - 5)]);
+  constexpr int NR = (NPT + 255) / 256;
+  double tv[NR][NC];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int q = threadIdx.x + 256 * r;
+    const int row = q / TX, x = q - row * TX;
+    const bool staged = q < NPT && xc * TX + x < g.nx && k0 - 3 + row <= g.nz + 2;
+#pragma unroll
+    for (int v = 0; v < NC; v++)
+      tv[r][v] = staged ? f.chem[(long)NC * s_cell[q] + v] * s_sgn[q] : 0.0;
+  }
+  __syncthreads();   // the fluid points are dead: LDS takes the tracer values
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int q = threadIdx.x + 256 * r;
+    if (q < NPT)
+#pragma unroll
+      for (int v = 0; v < NC; v++) s_buf[v * NPT + q] = tv[r][v];
+  }
+  __syncthreads();
+  if (face_ok) {
+    double hp[5], hm[5];
+    tracer_split(un, alpha, hp, hm);
+#pragma unroll
     for (int v = 0; v < NC; v++) {
+      const double *c0 = s_buf + v * NPT + ty * TX + tx;
       double cv[6];
 #pragma unroll
-      for (int l = 0; l < 6; l++)
-        cv[l] = f.chem[(long)NC * s_cell[(ty + l) * TX + tx] + v] * s_sgn[(ty + l) * TX + tx];
-      Fz[5 + v] = tracer_face_flux(cv, un, alpha);
+      for (int l = 0; l < 6; l++) cv[l] = c0[l * TX];
+      Fz[5 + v] = tracer_face_flux(cv, hp, hm);
     }
   }
-  __syncthreads();   // the point tile is dead; reuse its LDS for the face fluxes
+  __syncthreads();   // the tracer values are dead; LDS takes the face fluxes
   if (face_ok) {
 #pragma unroll
     for (int v = 0; v < NV; v++) s_buf[(v * TF + ty) * TX + tx] = Fz[v];
   }
   __syncthreads();
-  if (ty >= TC || i >= g.nx || k >= g.nz) return;
-  const long cell = i + (long)g.nx * (j + (long)g.ny * k);
-  const int nxp = g.nx + 1, nyp = g.ny + 1;
-  const long nfx = (long)nxp * g.ny * g.nz, nfy = (long)g.nx * nyp * g.nz;
-  const long xf = (long)nxp * (j + (long)g.ny * k) + i;
-  const long yf = i + (long)g.nx * (j + (long)nyp * k);
+  if (!cell_ok) return;
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    double dxy;
-    if (XY) {
-      dxy = out.w[v][cell];
-    } else {
-      const double dx = (Fx[v * nfx + xf + 1] - Fx[v * nfx + xf]) * idx;
-      const double dy = (Fy[v * nfy + yf + g.nx] - Fy[v * nfy + yf]) * idy;
-      dxy = dx + dy;
-    }
     const double dz = (s_buf[(v * TF + ty + 1) * TX + tx] - s_buf[(v * TF + ty) * TX + tx]) * idz;
-    const double o = 0.0 - (dxy + dz);
+    const double o = 0.0 - (dxy[v] + dz);
     if (v < 5) out.w[v][cell] = o;
     else out.chem[(long)NC * cell + (v - 5)] = o;
   }
@@ -510,22 +435,77 @@ fe_zface_div_xy(Grid g, Fields f, FieldsOut out, double idz) {
   }
 }
 
-// x- and y-faces of one z-plane tile together, without tracers (nchem = 0): a block covers
-// XY_TX x XY_TY cells of plane k and forms (Fx[i+1]-Fx[i])/dx + (Fy[j+1]-Fy[j])/dy for them
-// into out (the z kernel then adds the z part), so the x and y face fluxes never go to
-// HBM. The points of the tile and of its x and y halos (a cross, 3 deep) are converted
-// once into LDS, keeping mx and my in place and both wave speeds |u|+c and |v|+c, so that
-// every point serves both directions; x-faces (XY_TX+1 per row) and y-faces (XY_TY+1 per
-// column) are 255 and 256 with the 16 x 15 tile: one of each per thread.
+// x- and y-faces of one z-plane tile together: a block covers XY_TX x XY_TY cells of plane k
+// and forms (Fx[i+1]-Fx[i])/dx + (Fy[j+1]-Fy[j])/dy for them into out (the z kernel then
+// adds the z part), so the x and y face fluxes never go to HBM. The points of the tile and
+// of its x and y halos (a cross, 3 deep) are converted once into LDS, keeping mx and my in
+// place and both wave speeds |u|+c and |v|+c, so that every point serves both directions;
+// x-faces (XY_TX+1 per row) and y-faces (XY_TY+1 per column) are 255 and 256 with the
+// 16 x 15 tile: one of each per thread.
+//
+// With NC tracers a second phase follows on the same faces: the tracer values of the cross
+// (each point's NC values one contiguous AoS read, issued before the fluid divergence) go
+// to LDS as [tracer][point], where the fluid points were; the normal velocities of the stencil points (u on the tile's rows, v
+// on its columns) are kept aside from the fluid staging, and each face's fluid alpha stays
+// in a register; each thread then forms its x- and y-face's tracer fluxes from LDS, and
+// the divergence goes through LDS in passes of at most XY_TP tracers.
 constexpr int XY_TX = 16, XY_TY = 15;
 constexpr int XY_PX = XY_TX + 6, XY_PY = XY_TY + 6;      // point window 22 x 21
 constexpr int XY_NP = XY_PX * XY_PY;
 constexpr int XY_NF = 10;     // r, mx, my, mz, e, p, 1/r, |u|+c, sqrt(r), |v|+c
+constexpr int XY_NXF = (XY_TX + 1) * XY_TY, XY_NYF = XY_TX * (XY_TY + 1);   // 255, 256
+constexpr int XY_LDS = XY_NF * XY_NP + 5 * XY_NYF;          // points + y-face fluxes
+constexpr int XY_TP = XY_LDS / (XY_NXF + XY_NYF);           // tracers per divergence pass: 11
 
+// Point q of the cross of the tile at (x0, y0) in plane k: its cell, ghost signs (all
+// fields, mx, my) and whether it is used (inside the cross and the extended grid) and a
+// cell of the tile (`own`), with the boundary mapping of ghost_map.
+struct XYPoint {
+  int cell;
+  double s_all, s_mx, s_my;
+  bool use, own, row, col;
+};
+
+__device__ inline XYPoint xy_point(const Grid &g, int x0, int y0, int k, int q) {
+  XYPoint P;
+  const int yy = q / XY_PX, xx = q - yy * XY_PX;
+  const int ex = x0 - 3 + xx, ey = y0 - 3 + yy;
+  P.row = yy >= 3 && yy < 3 + XY_TY && ey < g.ny;          // x-line point
+  P.col = xx >= 3 && xx < 3 + XY_TX && ex < g.nx;          // y-column point
+  int i = ex, j = ey;
+  P.s_all = P.s_mx = P.s_my = 1.0;
+  P.use = q < XY_NP && (P.row ? ex <= g.nx + 2 : (P.col && ey <= g.ny + 2));
+  P.own = P.row && P.col && ex >= 0;   // a cell of the tile: legal_state once
+  if (P.row) {
+    if (ex < 0 || ex >= g.nx) {
+      const GhostMap m = ghost_map(g, 0, ex);
+      i = m.src;
+      P.s_all = P.s_my = m.sgn_all;
+      P.s_mx = m.sgn_norm;
+    }
+  } else if (P.col) {
+    if (ey < 0 || ey >= g.ny) {
+      const GhostMap m = ghost_map(g, 1, ey);
+      j = m.src;
+      P.s_all = P.s_mx = m.sgn_all;
+      P.s_my = m.sgn_norm;
+    }
+  }
+  P.cell = P.use ? (int)(i + (long)g.nx * (j + (long)g.ny * k)) : 0;
+  return P;
+}
+
+template <int NC>
 __global__ void __launch_bounds__(256, 3)
 fe_xyface_div(Grid g, Fields f, FieldsOut out, double idx, double idy, unsigned *flag) {
-  __shared__ double sh[XY_NF * XY_NP];
-  __shared__ double sfy[5 * (XY_TY + 1) * XY_TX];    // y-face fluxes [5][XY_TY+1][XY_TX]
+  // points [XY_NF][XY_NP], then the y-face fluxes [5][XY_TY+1][XY_TX]; one array, so that the
+  // tracer phase can use all of it
+  __shared__ double lds[XY_LDS];
+  double *const sh = lds, *const sfy = lds + XY_NF * XY_NP;
+  // tracer stencils' normal velocities: u of the row points [XY_TY][XY_PX], then v of the
+  // column points [XY_PY][XY_TX]
+  constexpr int UN_X = XY_TY * XY_PX, UN_N = UN_X + XY_PY * XY_TX;
+  __shared__ double sun[NC > 0 ? UN_N : 1];
   const int t = threadIdx.x;
   const int nxb = (g.nx + XY_TX - 1) / XY_TX, nyb = (g.ny + XY_TY - 1) / XY_TY;
   // The hardware deals blocks to the 8 XCDs in turn, and each XCD has its own L2. Tiles are
@@ -544,58 +524,37 @@ fe_xyface_div(Grid g, Fields f, FieldsOut out, double idx, double idy, unsigned 
   // A thread's points (q = t + 256 r) are located first and all their loads issued before
   // any is converted, so that their latencies overlap.
   constexpr int NR = (XY_NP + 255) / 256;
-  int cq[NR];
-  double s_all[NR], s_mx[NR], s_my[NR];
-  bool use[NR], own[NR];
+  XYPoint pt[NR];
 #pragma unroll
-  for (int r = 0; r < NR; r++) {
-    const int q = t + 256 * r;
-    const int yy = q / XY_PX, xx = q - yy * XY_PX;
-    const int ex = x0 - 3 + xx, ey = y0 - 3 + yy;
-    const bool row = yy >= 3 && yy < 3 + XY_TY && ey < g.ny;          // x-line point
-    const bool col = xx >= 3 && xx < 3 + XY_TX && ex < g.nx;          // y-column point
-    int i = ex, j = ey;
-    s_all[r] = s_mx[r] = s_my[r] = 1.0;
-    use[r] = q < XY_NP && (row ? ex <= g.nx + 2 : (col && ey <= g.ny + 2));
-    own[r] = row && col && ex >= 0;   // a cell of the tile: legal_state once
-    if (row) {
-      if (ex < 0 || ex >= g.nx) {
-        const GhostMap m = ghost_map(g, 0, ex);
-        i = m.src;
-        s_all[r] = s_my[r] = m.sgn_all;
-        s_mx[r] = m.sgn_norm;
-      }
-    } else if (col) {
-      if (ey < 0 || ey >= g.ny) {
-        const GhostMap m = ghost_map(g, 1, ey);
-        j = m.src;
-        s_all[r] = s_mx[r] = m.sgn_all;
-        s_my[r] = m.sgn_norm;
-      }
-    }
-    cq[r] = use[r] ? (int)(i + (long)g.nx * (j + (long)g.ny * k)) : 0;
-  }
+  for (int r = 0; r < NR; r++) pt[r] = xy_point(g, x0, y0, k, t + 256 * r);
   double raw[NR][5];
 #pragma unroll
   for (int r = 0; r < NR; r++) {
 #pragma unroll
-    for (int a = 0; a < 5; a++) raw[r][a] = use[r] ? f.w[a][cq[r]] : 1.0;
+    for (int a = 0; a < 5; a++) raw[r][a] = pt[r].use ? f.w[a][pt[r].cell] : 1.0;
   }
 #pragma unroll
   for (int r = 0; r < NR; r++) {
-    if (!use[r]) continue;
-    const double rr = raw[r][0] * s_all[r], mx = raw[r][1] * s_mx[r], my = raw[r][2] * s_my[r];
-    const double mz = raw[r][3] * s_all[r], e = raw[r][4] * s_all[r];
+    if (!pt[r].use) continue;
+    const double rr = raw[r][0] * pt[r].s_all, mx = raw[r][1] * pt[r].s_mx;
+    const double my = raw[r][2] * pt[r].s_my;
+    const double mz = raw[r][3] * pt[r].s_all, e = raw[r][4] * pt[r].s_all;
     const double ir = fast_rcp(rr);
     const double p = (g.gamma - 1.0) * (e - 0.5 * (mx * mx + my * my + mz * mz) * ir);
     const double cs = sun_sqrt(g.gamma * p * ir);
-    double *pt = sh + t + 256 * r;
-    pt[0 * XY_NP] = rr; pt[1 * XY_NP] = mx; pt[2 * XY_NP] = my;
-    pt[3 * XY_NP] = mz; pt[4 * XY_NP] = e;  pt[5 * XY_NP] = p;
-    pt[6 * XY_NP] = ir; pt[7 * XY_NP] = fabs(mx * ir) + cs;
-    pt[8 * XY_NP] = sun_sqrt(rr);
-    pt[9 * XY_NP] = fabs(my * ir) + cs;
-    if (own[r]) {
+    const int q = t + 256 * r;
+    double *pp = sh + q;
+    pp[0 * XY_NP] = rr; pp[1 * XY_NP] = mx; pp[2 * XY_NP] = my;
+    pp[3 * XY_NP] = mz; pp[4 * XY_NP] = e;  pp[5 * XY_NP] = p;
+    pp[6 * XY_NP] = ir; pp[7 * XY_NP] = fabs(mx * ir) + cs;
+    pp[8 * XY_NP] = sun_sqrt(rr);
+    pp[9 * XY_NP] = fabs(my * ir) + cs;
+    if (NC > 0) {
+      const int yy = q / XY_PX, xx = q - yy * XY_PX;
+      if (pt[r].row) sun[(yy - 3) * XY_PX + xx] = mx * ir;
+      if (pt[r].col) sun[UN_X + yy * XY_TX + (xx - 3)] = my * ir;
+    }
+    if (pt[r].own) {
       unsigned bits = 0;
       if (!(rr > 0.0)) bits |= 1u;
       if (!(e > 0.0)) bits |= 2u;
@@ -609,35 +568,109 @@ fe_xyface_div(Grid g, Fields f, FieldsOut out, double idx, double idy, unsigned 
   const bool yok = x0 + cx < g.nx && y0 + fy <= g.ny && fy <= XY_TY;
   // x-face t: row ry, faces between columns x0+fx-1 and x0+fx (points xx = fx..fx+5)
   const int ry = t / (XY_TX + 1), fx = t % (XY_TX + 1);
-  const bool xok = t < (XY_TX + 1) * XY_TY && y0 + ry < g.ny && x0 + fx <= g.nx;
+  const bool xok = t < XY_NXF && y0 + ry < g.ny && x0 + fx <= g.nx;
+  double alpha_y = 0.0, alpha_x = 0.0;
   if (yok) {
-    double Fl[5], Fy[5], alpha;
-    face_flux_lds_f<2, 1, 9>(sh + fy * XY_PX + cx + 3, XY_NP, XY_PX, g.gamma, Fl, alpha);
+    double Fl[5], Fy[5];
+    face_flux_lds_f<2, 1, 9>(sh + fy * XY_PX + cx + 3, XY_NP, XY_PX, g.gamma, Fl, alpha_y);
     to_global<1>(Fl, Fy);
 #pragma unroll
     for (int a = 0; a < 5; a++) sfy[(a * (XY_TY + 1) + fy) * XY_TX + cx] = Fy[a];
   }
   double Fx[5];
-  if (xok) {
-    double alpha;
-    face_flux_lds_f<1, 2, 7>(sh + (ry + 3) * XY_PX + fx, XY_NP, 1, g.gamma, Fx, alpha);
-  }
+  if (xok) face_flux_lds_f<1, 2, 7>(sh + (ry + 3) * XY_PX + fx, XY_NP, 1, g.gamma, Fx, alpha_x);
   __syncthreads();                    // points dead: their LDS takes the x-face fluxes
   double *sfx = sh;                   // [5][XY_TY][XY_TX+1]
   if (xok)
 #pragma unroll
     for (int a = 0; a < 5; a++) sfx[(a * XY_TY + ry) * (XY_TX + 1) + fx] = Fx[a];
   __syncthreads();
-  const int ci = t % XY_TX, cj = t / XY_TX;
-  if (cj >= XY_TY || x0 + ci >= g.nx || y0 + cj >= g.ny) return;
-  const long cell = (x0 + ci) + (long)g.nx * ((y0 + cj) + (long)g.ny * k);
+  // the tracer values of the thread's cross points, loaded now so that their latency
+  // passes under the fluid divergence
+  double tv[NC > 0 ? NR : 1][NC > 0 ? NC : 1];
+  if (NC > 0) {
 #pragma unroll
-  for (int a = 0; a < 5; a++) {
-    const double dx = (sfx[(a * XY_TY + cj) * (XY_TX + 1) + ci + 1] -
-                       sfx[(a * XY_TY + cj) * (XY_TX + 1) + ci]) * idx;
-    const double dy = (sfy[(a * (XY_TY + 1) + cj + 1) * XY_TX + ci] -
-                       sfy[(a * (XY_TY + 1) + cj) * XY_TX + ci]) * idy;
-    out.w[a][cell] = dx + dy;
+    for (int r = 0; r < NR; r++) {
+      const XYPoint P = xy_point(g, x0, y0, k, t + 256 * r);
+#pragma unroll
+      for (int v = 0; v < NC; v++)
+        tv[r][v] = P.use ? f.chem[(long)NC * P.cell + v] * P.s_all : 0.0;
+    }
+  }
+  const int ci = t % XY_TX, cj = t / XY_TX;
+  const bool cok = cj < XY_TY && x0 + ci < g.nx && y0 + cj < g.ny;
+  const long cell = (x0 + ci) + (long)g.nx * ((y0 + cj) + (long)g.ny * k);
+  if (cok) {
+#pragma unroll
+    for (int a = 0; a < 5; a++) {
+      const double dx = (sfx[(a * XY_TY + cj) * (XY_TX + 1) + ci + 1] -
+                         sfx[(a * XY_TY + cj) * (XY_TX + 1) + ci]) * idx;
+      const double dy = (sfy[(a * (XY_TY + 1) + cj + 1) * XY_TX + ci] -
+                         sfy[(a * (XY_TY + 1) + cj) * XY_TX + ci]) * idy;
+      out.w[a][cell] = dx + dy;
+    }
+  }
+  if (NC == 0) return;
+  __syncthreads();                    // fluid fluxes read: LDS takes the tracer values
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int q = t + 256 * r;
+    if (q < XY_NP)
+#pragma unroll
+      for (int v = 0; v < NC; v++) sh[v * XY_NP + q] = tv[r][v];
+  }
+  __syncthreads();
+  double Fxt[NC > 0 ? NC : 1], Fyt[NC > 0 ? NC : 1];
+  if (yok) {
+    double un[6], hp[5], hm[5];
+#pragma unroll
+    for (int l = 0; l < 6; l++) un[l] = sun[UN_X + (fy + l) * XY_TX + cx];
+    tracer_split(un, alpha_y, hp, hm);
+#pragma unroll
+    for (int v = 0; v < NC; v++) {
+      const double *c0 = sh + v * XY_NP + fy * XY_PX + cx + 3;
+      double cv[6];
+#pragma unroll
+      for (int l = 0; l < 6; l++) cv[l] = c0[l * XY_PX];
+      Fyt[v] = tracer_face_flux(cv, hp, hm);
+    }
+  }
+  if (xok) {
+    double un[6], hp[5], hm[5];
+#pragma unroll
+    for (int l = 0; l < 6; l++) un[l] = sun[ry * XY_PX + fx + l];
+    tracer_split(un, alpha_x, hp, hm);
+#pragma unroll
+    for (int v = 0; v < NC; v++) {
+      const double *c0 = sh + v * XY_NP + (ry + 3) * XY_PX + fx;
+      double cv[6];
+#pragma unroll
+      for (int l = 0; l < 6; l++) cv[l] = c0[l];
+      Fxt[v] = tracer_face_flux(cv, hp, hm);
+    }
+  }
+  // divergence in passes of XY_TP tracers (one pass for up to 11): pass fluxes
+  // [v][x-faces | y-faces] in LDS, so that each cell's tracers are written together
+#pragma unroll
+  for (int v0 = 0; v0 < NC; v0 += XY_TP) {
+    __syncthreads();                  // tracer values (first pass) / last pass's fluxes read
+#pragma unroll
+    for (int v = v0; v < NC && v < v0 + XY_TP; v++) {
+      double *fl = sh + (v - v0) * (XY_NXF + XY_NYF);
+      if (xok) fl[ry * (XY_TX + 1) + fx] = Fxt[v];
+      if (yok) fl[XY_NXF + fy * XY_TX + cx] = Fyt[v];
+    }
+    __syncthreads();
+    if (cok) {
+#pragma unroll
+      for (int v = v0; v < NC && v < v0 + XY_TP; v++) {
+        const double *fl = sh + (v - v0) * (XY_NXF + XY_NYF);
+        const double dx = (fl[cj * (XY_TX + 1) + ci + 1] - fl[cj * (XY_TX + 1) + ci]) * idx;
+        const double dy = (fl[XY_NXF + (cj + 1) * XY_TX + ci] -
+                           fl[XY_NXF + cj * XY_TX + ci]) * idy;
+        out.chem[(long)NC * cell + v] = dx + dy;
+      }
+    }
   }
 }
 
@@ -670,37 +703,26 @@ static int launch_fe(const Grid &g, const Fields &f, const FieldsOut &o, FeScrat
     tm.end(PROF_PLAIN_DIV, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
   }
+  // x and y faces of each plane tile (fluid, then the tracers) in one kernel, then the z
+  // faces adding their part
   constexpr int TX = TILE_X, TF = TILE_F;
   const long nxc = (g.nx + TX - 1) / TX;
-  if (NC == 0) {
-    // x and y faces of each plane tile in one kernel, then the z faces adding their part
-    const long nxyb = ((long)(g.nx + XY_TX - 1) / XY_TX) * ((g.ny + XY_TY - 1) / XY_TY) * g.nz;
-    tm.begin(PROF_XYFACE_DIV, st);
-    hipLaunchKernelGGL(fe_xyface_div, dim3((unsigned)nxyb), dim3(256), 0, st, g, f, o,
-                       1.0 / g.dx, 1.0 / g.dy, flag);
-    tm.end(PROF_XYFACE_DIV, st);
+  const long nxyb = ((long)(g.nx + XY_TX - 1) / XY_TX) * ((g.ny + XY_TY - 1) / XY_TY) * g.nz;
+  tm.begin(PROF_XYFACE_DIV, st);
+  hipLaunchKernelGGL(fe_xyface_div<NC>, dim3((unsigned)nxyb), dim3(256), 0, st, g, f, o,
+                     1.0 / g.dx, 1.0 / g.dy, flag);
+  tm.end(PROF_XYFACE_DIV, st);
+  tm.begin(PROF_ZFACE_DIV, st);
+  if constexpr (NC == 0) {
     constexpr int ZC = 16 * ZFPT - 1;                      // cells per z tile
-    tm.begin(PROF_ZFACE_DIV, st);
     hipLaunchKernelGGL(fe_zface_div_xy<ZFPT>,
                        dim3((unsigned)(nxc * ((g.nz + ZC - 1) / ZC) * g.ny)), dim3(256), 0, st,
                        g, f, o, 1.0 / g.dz);
-    tm.end(PROF_ZFACE_DIV, st);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+  } else {
+    hipLaunchKernelGGL((fe_zface_div<NC, TX, TF>),
+                       dim3((unsigned)(nxc * ((g.nz + TF - 2) / (TF - 1)) * g.ny)),
+                       dim3(TX * TF), 0, st, g, f, o, 1.0 / g.dz);
   }
-  const long nchunk = (nxp + 63) / 64;
-  const long nrowblk = ((long)g.ny * g.nz + XROWS - 1) / XROWS;
-  tm.begin(PROF_XFACE, st);
-  hipLaunchKernelGGL(fe_xface<NC>, dim3((unsigned)(nchunk * nrowblk)), dim3(64 * XROWS), 0, st,
-                     g, f, s.Fx, flag);
-  tm.end(PROF_XFACE, st);
-  tm.begin(PROF_YFACE, st);
-  hipLaunchKernelGGL((fe_yface<NC, TX, TF>), dim3((unsigned)(nxc * ((nyp + TF - 1) / TF) * g.nz)),
-                     dim3(TX * TF), 0, st, g, f, s.Fy);
-  tm.end(PROF_YFACE, st);
-  tm.begin(PROF_ZFACE_DIV, st);
-  hipLaunchKernelGGL((fe_zface_div<NC, TX, TF, false>),
-                     dim3((unsigned)(nxc * ((g.nz + TF - 2) / (TF - 1)) * g.ny)), dim3(TX * TF),
-                     0, st, g, f, s.Fx, s.Fy, o, 1.0 / g.dx, 1.0 / g.dy, 1.0 / g.dz);
   tm.end(PROF_ZFACE_DIV, st);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -363,16 +363,27 @@ __device__ inline void lds_velocities(const double *base, int FS, int LS, double
   for (int l = 0; l < 6; l++) un[l] = base[FS + l * LS] * base[6 * FS + l * LS];
 }
 
-// Species (passive tracer) face flux for one component from its 6 point values c[l],
-// the point normal velocities un[l] and the fluid alpha (utilities.cpp:376-377,395,439,473):
-// component-wise WENO5 of the split fluxes, no characteristic projection.
-__device__ inline double tracer_face_flux(const double (&c)[6], const double (&un)[6],
-                                          double alpha) {
+// Species (passive tracer) face flux for one component from its 6 point values c[l]
+// (utilities.cpp:376-377,395,439,473): component-wise WENO5 of the split fluxes
+// (u c +- alpha c) / 2, no characteristic projection. The split fluxes are formed as
+// c[l] hp[l] and c[l+1] hm[l] with the face's split factors hp[l] = (u_l + alpha) / 2 and
+// hm[l] = (u_{l+1} - alpha) / 2 (tracer_split), which every tracer of the face shares.
+__device__ inline void tracer_split(const double (&un)[6], double alpha, double (&hp)[5],
+                                    double (&hm)[5]) {
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    hp[l] = 0.5 * (un[l] + alpha);
+    hm[l] = 0.5 * (un[l + 1] - alpha);
+  }
+}
+
+__device__ inline double tracer_face_flux(const double (&c)[6], const double (&hp)[5],
+                                          const double (&hm)[5]) {
   double fp[5], fm[5];
 #pragma unroll
   for (int l = 0; l < 5; l++) {
-    fp[l] = 0.5 * (un[l] * c[l] + alpha * c[l]);
-    fm[l] = 0.5 * (un[l + 1] * c[l + 1] - alpha * c[l + 1]);
+    fp[l] = c[l] * hp[l];
+    fm[l] = c[l + 1] * hm[l];
   }
   return weno5_pair(fp, fm);
 }
