@@ -327,8 +327,10 @@ int mvd_imex_evolve(mvd_context *ctx, double t0, double h, int nsteps, const mvd
  * be stiffly accurate (the last row of A sums to 1, as mvd_dirk_sdirk3). Ts/dTs end at the
  * forward run's final state. Returns 0, the number (> 0) of cells whose forward Newton
  * budget ran out (lam0 is still computed along the computed trajectory), or an error code.
+ * The cells run in the batched step's temperature order at y_0 (from 4096 cells on).
  * Device scratch held by the context: ceil(nsteps/ckpt_every) * 12
- * + ckpt_every * (10 * nstage + 1) doubles per cell. */
+ * + ckpt_every * (10 * nstage + 1) doubles and 22 bytes (the cell order) per cell, and
+ * 128 KiB. */
 int mvd_chem_dirk_adjoint(mvd_context *ctx, double t, double h, int nsteps, int ckpt_every,
                           const mvd_dirk_opts *o, double *y, const double *lamT, double *lam0);
 

@@ -1249,9 +1249,11 @@ constexpr int ADJ_REC = NSP + 2;     // y, Ts, dTs
 
 __global__ void __launch_bounds__(256)
 chem_adjoint_forward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, int nsteps, int K,
-                     double *y, double *ck, int *nfail) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                     const int *order, double *y, double *ck, int *nfail) {
+  // thread u takes the u-th cell of the temperature order; scratch is indexed by u
+  const long u = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const long i = order ? order[u] : u;
   const CellIn c = load_cell(p, i);
   double Ts = p.Ts[i], dTs = p.dTs[i];
   double z[NSP];
@@ -1262,11 +1264,11 @@ chem_adjoint_forward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, int
     if (st % K == 0) {
       double *rec = ck + (long)(st / K) * ADJ_REC * n;
 #pragma unroll
-      for (int s = 0; s < NSP; s++) rec[s * n + i] = z[s];
-      rec[NSP * n + i] = Ts;
-      rec[(NSP + 1) * n + i] = dTs;
+      for (int s = 0; s < NSP; s++) rec[s * n + u] = z[s];
+      rec[NSP * n + u] = Ts;
+      rec[(NSP + 1) * n + u] = dTs;
     }
-    ok = cell_step(cc, c, Ts, dTs, o, h, z, nullptr, n, i) && ok;
+    ok = cell_step(cc, c, Ts, dTs, o, h, z, nullptr, n, u) && ok;
   }
 #pragma unroll
   for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
@@ -1312,11 +1314,12 @@ __device__ inline void stage_adjoint_matrix(const ChemConst &cc, const CellIn &c
 
 __global__ void __launch_bounds__(256)
 chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, int nsteps, int K,
-                      const double *ck, double *Te, double *Ys, const double *lamT,
-                      double *lam0) {
+                      const int *order, const double *ck, double *Te, double *Ys,
+                      const double *lamT, double *lam0) {
 #pragma clang fp contract(off)
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const long u = (long)blockIdx.x * blockDim.x + threadIdx.x;   // as in chem_adjoint_forward
+  if (u >= n) return;
+  const long i = order ? order[u] : u;
   const CellIn c = load_cell(p, i);
   double lam[NSP];
 #pragma unroll
@@ -1329,15 +1332,15 @@ chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, in
     {
       double z[NSP];
 #pragma unroll
-      for (int s = 0; s < NSP; s++) z[s] = rec[s * n + i];
-      double Ts = rec[NSP * n + i], dTs = rec[(NSP + 1) * n + i];
+      for (int s = 0; s < NSP; s++) z[s] = rec[s * n + u];
+      double Ts = rec[NSP * n + u], dTs = rec[(NSP + 1) * n + u];
       for (int st = n0; st < n1; st++) {
-        (void)cell_step(cc, c, Ts, dTs, o, h, z, Ys + (long)(st - n0) * ns * NSP * n, n, i);
-        Te[(long)(st - n0) * n + i] = Ts;   // the warm start of the step's stage solves
+        (void)cell_step(cc, c, Ts, dTs, o, h, z, Ys + (long)(st - n0) * ns * NSP * n, n, u);
+        Te[(long)(st - n0) * n + u] = Ts;   // the warm start of the step's stage solves
       }
     }
     for (int st = n1 - 1; st >= n0; st--) {
-      const double Ts = Te[(long)(st - n0) * n + i];
+      const double Ts = Te[(long)(st - n0) * n + u];
       const double *Yst = Ys + (long)(st - n0) * ns * NSP * n;
       double nu[MAXSTAGE][NSP];
 #pragma unroll
@@ -1356,7 +1359,7 @@ chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, in
             for (int m = k + 1; m < MAXSTAGE; m++)
               if (m < ns) acc += dirk_coef(o, m, k) * nu[m][s];
             w[s] = acc;
-            Y[s] = Yst[(long)(k * NSP + s) * n + i];
+            Y[s] = Yst[(long)(k * NSP + s) * n + u];
           }
           double Mt[NSP][NSP], yb[NSP];
           stage_adjoint_matrix(cc, c, Ts, Y, o.rhs_scale, h, h * o.A[k * 4 + k], w, Mt, yb);
@@ -1377,9 +1380,18 @@ chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, in
   for (int s = 0; s < NSP; s++) lam0[NSP * i + s] = lam[s];
 }
 
+// Scratch: checkpoints | end temperatures | stage values | the cell order (order, keys,
+// temperatures at y_0, bucket counters of the ordering pass).
+static size_t adj_order_doubles(long n) {
+  const size_t b = align256((size_t)n * sizeof(int)) + align256((size_t)n * sizeof(short)) +
+                   align256((size_t)n * sizeof(double2)) +
+                   align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int));
+  return (b + 256) / sizeof(double);     // (+ the alignment of its start)
+}
+
 size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K, int nstage) {
   const long nckpt = (nsteps + K - 1) / K;
-  return (size_t)n * (nckpt * ADJ_REC + (long)K * (nstage * NSP + 1));
+  return (size_t)n * (nckpt * ADJ_REC + (long)K * (nstage * NSP + 1)) + adj_order_doubles(n);
 }
 
 int chem_adjoint_run(ChemState &s, const DirkDev &o, double h, int nsteps, int K, double *y,
@@ -1392,10 +1404,28 @@ int chem_adjoint_run(ChemState &s, const DirkDev &o, double h, int nsteps, int K
   const ChemPtrsD p = ptrs_d(s);
   int *nfail = s.counters;
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, 1L, nfail, 0);
+  // cells in the batched step's order at y_0 (temperature bucket, rate class), so that the
+  // cells of a wave take similar Newton paths; kept for the whole run
+  int *order = nullptr;
+  if (n >= TORD_MIN) {
+    const uintptr_t a = (uintptr_t)(Ys + (long)K * o.nstage * NSP * n);
+    char *w = (char *)((a + 255) & ~(uintptr_t)255);
+    order = (int *)w;
+    unsigned short *key = (unsigned short *)(w + align256((size_t)n * sizeof(int)));
+    double2 *tdt = (double2 *)((char *)key + align256((size_t)n * sizeof(short)));
+    int *cnt = (int *)((char *)tdt + align256((size_t)n * sizeof(double2)));
+    hipLaunchKernelGGL(fill_int, dim3(2 * TORD_NBK * TORD_PAD / 256), dim3(256), 0, st,
+                       (long)(2 * TORD_NBK * TORD_PAD), cnt, 0);
+    hipLaunchKernelGGL(tord_count, dim3((unsigned)((n + TORD_CBLOCK - 1) / TORD_CBLOCK)),
+                       dim3(TORD_CBLOCK), 0, st, n, s.cc, p, (const double *)y,
+                       (const double *)nullptr, h, o.rhs_scale, key, cnt, tdt);
+    hipLaunchKernelGGL(tord_scatter, dim3((unsigned)((n + TORD_BLOCK - 1) / TORD_BLOCK)),
+                       dim3(TORD_BLOCK), 0, st, n, (const unsigned short *)key, cnt, order);
+  }
   hipLaunchKernelGGL(chem_adjoint_forward, dim3(nbk), dim3(256), 0, st, n, s.cc, p, o, h, nsteps,
-                     K, y, ck, nfail);
+                     K, (const int *)order, y, ck, nfail);
   hipLaunchKernelGGL(chem_adjoint_backward, dim3(nbk), dim3(256), 0, st, n, s.cc, p, o, h,
-                     nsteps, K, (const double *)ck, Te, Ys, lamT, lam0);
+                     nsteps, K, (const int *)order, (const double *)ck, Te, Ys, lamT, lam0);
   if (hipGetLastError() != hipSuccess) return -1;
   if (hipMemcpyAsync(host_fail, nfail, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)

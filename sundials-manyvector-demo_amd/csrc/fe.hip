@@ -13,12 +13,14 @@
 //                    primitives staged in LDS), and the divergence of the 31 cells between
 //                    a tile's 32 faces (tiles overlap by one face), adding the x/y part
 //                    read from wdot at the kernel's start.
-// Fast path with tracers (two launches):
+// Fast path with tracers (three launches):
 //   fe_xyface_div<NC>: as without tracers, then the tracers on the same x- and y-faces
 //                    (their values staged in LDS as [tracer][point] after the fluid phase)
 //                    and their part of the divergence into wdot;
-//   fe_zface_div   : z-faces (tiles of 16 x-cells by 16 faces), fluid then tracers from
-//                    one LDS tile each, adding their part of the divergence.
+//   fe_zface_div_xy: the fluid z-faces, as without tracers;
+//   fe_zface_tracers: the tracer z-faces (tiles of 16 x-cells by 16 faces; per point the
+//                    normal velocity, wave speed and tracer values in LDS), adding their
+//                    part of the divergence.
 // The divergence is accumulated as -((dFx/dx + dFy/dy) + dFz/dz), the reference's
 // association (utilities.cpp:202-207).
 #include "mvd_device.hpp"
@@ -182,55 +184,6 @@ __device__ inline Pt to_point(const RawPt &q, double gamma) {
   return make_point(q.r, q.mn, q.m1, q.m2, q.e, gamma);
 }
 
-// z-faces with tracers: one thread per face; a block covers TX consecutive x-cells and TF
-// consecutive faces. The TF+5 points of each column are converted once into LDS (SoA,
-// conflict-free), with each point's cell and ghost sign for the tracer reads, then every
-// thread reads its 6-point stencil from LDS.
-struct TileLds {
-  double *pt;      // [9][TPTS][TX]
-  int *cell;       // [TPTS][TX]
-  double *sgn;     // [TPTS][TX]
-};
-
-// Blocks of 256 threads. A thread's points (s = t + 256 r) are located first and all their
-// loads issued before any is converted, so that their latencies overlap.
-template <int D, int TX, int TF>
-__device__ inline void stage_tile(const Grid &g, const Fields &f, const TileLds &L, int x0,
-                                  int t, int e0) {
-  constexpr int TPTS = TF + 5, FS = TPTS * TX, NR = (FS + 255) / 256;
-  const int n = (D == 1) ? g.ny : g.nz;
-  const int stride = (D == 1) ? g.nx : g.nx * g.ny;
-  int cs[NR];
-  double sa[NR], sn[NR];
-  bool use[NR];
-#pragma unroll
-  for (int r = 0; r < NR; r++) {
-    const int s = threadIdx.x + 256 * r;
-    const int row = s / TX, x = s - row * TX;
-    const int i = x0 + x;
-    const int e = e0 - 3 + row;
-    use[r] = s < FS && i < g.nx && e <= n + 2;
-    const int base = (D == 1) ? i + g.nx * g.ny * t : i + g.nx * t;
-    cs[r] = use[r] ? line_cell<D>(g, base, stride, n, e, sa[r], sn[r]) : 0;
-  }
-  RawPt raw[NR];
-#pragma unroll
-  for (int r = 0; r < NR; r++)
-    if (use[r]) raw[r] = load_raw<D>(f, cs[r], sa[r], sn[r]);
-#pragma unroll
-  for (int r = 0; r < NR; r++) {
-    if (!use[r]) continue;
-    const int s = threadIdx.x + 256 * r;
-    const Pt P = to_point(raw[r], g.gamma);
-    double *pt = L.pt + s;
-    pt[0 * FS] = P.r;  pt[1 * FS] = P.mn; pt[2 * FS] = P.m1;
-    pt[3 * FS] = P.m2; pt[4 * FS] = P.e;  pt[5 * FS] = P.p;
-    pt[6 * FS] = P.ir; pt[7 * FS] = P.a;  pt[8 * FS] = P.sr;
-    L.cell[s] = cs[r];
-    L.sgn[s] = sa[r];
-  }
-}
-
 // local (rho, m_normal, m_t1, m_t2, e) flux of direction D into global component order
 template <int D>
 __device__ inline void to_global(const double (&Fl)[5], double *Fg) {
@@ -240,29 +193,22 @@ __device__ inline void to_global(const double (&Fl)[5], double *Fg) {
   else { Fg[3] = Fl[1]; Fg[2] = Fl[2]; Fg[1] = Fl[3]; }
 }
 
-// z-faces with tracers, adding their part to the x/y part that fe_xyface_div left in out:
-// a block computes TF z-faces for TX columns and the TF-1 cells between them (tiles overlap
-// by one face),
-//   wdot = 0 - ((x/y part) + (Fz[k+1]-Fz[k])/dz).
-// The TF+5 fluid points of each column are converted once into LDS for the fluid fluxes;
-// then the tile's tracer values (one contiguous AoS read of a point's NC values per thread
-// and point, all issued together) take their place as [tracer][point] for the tracer
-// fluxes, and the fluxes of all 5+NC components go through LDS for the divergence.
-template <int NC, int TX, int TF>
+// Tracer z-faces, adding their part to the x/y part that fe_xyface_div<NC> left in
+// out.chem (the fluid z-faces are fe_zface_div_xy's). A block of 16 x 16 threads covers 16
+// x-cells of row j and 16 consecutive z-faces, i.e. the 15 cells between them. Per point
+// of the tile (21 per column) LDS holds the normal velocity w, the wave speed |w| + c
+// (make_point's, so the face's alpha is the fluid flux's bit for bit) and the NC tracer
+// values; each thread forms its face's alpha and split factors once and then the NC
+// tracer fluxes. Without the fluid flux's registers the kernel runs at a higher occupancy
+// than a fused fluid + tracer z kernel.
+template <int NC>
 __global__ void __launch_bounds__(256)
-fe_zface_div(Grid g, Fields f, FieldsOut out, double idz) {
-  constexpr int NV = 5 + NC;
-  constexpr int TPTS = TF + 5;
-  constexpr int TC = TF - 1;
-  constexpr int NPT = TPTS * TX;
-  constexpr int PT_SZ = 9 * NPT, TR_SZ = NC * NPT, FL_SZ = NV * TF * TX;
-  constexpr int BUF = PT_SZ > FL_SZ ? (PT_SZ > TR_SZ ? PT_SZ : TR_SZ)
-                                    : (FL_SZ > TR_SZ ? FL_SZ : TR_SZ);
-  __shared__ double s_buf[BUF];
-  __shared__ int s_cell[NPT];
-  __shared__ double s_sgn[NPT];
-  const TileLds L{s_buf, s_cell, s_sgn};
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+fe_zface_tracers(Grid g, Fields f, FieldsOut out, double idz) {
+  constexpr int TX = 16, TF = 16, TPTS = TF + 5, TC = TF - 1, NPT = TPTS * TX;
+  constexpr int NR = (NPT + 255) / 256;
+  constexpr int PT_SZ = (2 + NC) * NPT, FL_SZ = NC * TF * TX;
+  __shared__ double s_buf[PT_SZ > FL_SZ ? PT_SZ : FL_SZ];   // [w | speed | tracers][point]
+  const int t = threadIdx.x, tx = t % TX, ty = t / TX;
   const int nxc = (g.nx + TX - 1) / TX, ncc = (g.nz + TC - 1) / TC;
   int b = blockIdx.x;
   const int xc = b % nxc;
@@ -270,72 +216,79 @@ fe_zface_div(Grid g, Fields f, FieldsOut out, double idz) {
   const int cc = b % ncc;
   const int j = b / ncc;
   const int k0 = cc * TC;
-  stage_tile<2, TX, TF>(g, f, L, xc * TX, j, k0);
-  __syncthreads();
   const int i = xc * TX + tx, k = k0 + ty;
-  const bool face_ok = i < g.nx && k <= g.nz;
-  double Fz[NV], alpha = 0.0, un[6];
-  if (face_ok) {
-    const double *pbase = L.pt + ty * TX + tx;
-    double Fl[5];
-    face_flux_lds(pbase, NPT, TX, g.gamma, Fl, alpha);
-    lds_velocities(pbase, NPT, TX, un);
-    to_global<2>(Fl, Fz);
-  }
-  // the x/y part of the thread's cell and the tracer values of its points (staged points
-  // only: stage_tile's bounds), all loaded now, so that their latency passes under the
-  // tracer staging and fluxes
+  // the x/y part of the thread's cell, loaded first (its latency passes under the staging)
   const bool cell_ok = ty < TC && i < g.nx && k < g.nz;
   const long cell = i + (long)g.nx * (j + (long)g.ny * k);
-  double dxy[NV];
+  double dxy[NC];
 #pragma unroll
-  for (int v = 0; v < NV; v++)
-    dxy[v] = !cell_ok ? 0.0 : (v < 5 ? out.w[v][cell] : out.chem[(long)NC * cell + (v - 5)]);
-  constexpr int NR = (NPT + 255) / 256;
-  double tv[NR][NC];
+  for (int v = 0; v < NC; v++) dxy[v] = cell_ok ? out.chem[(long)NC * cell + v] : 0.0;
+  // points k0 - 3 .. k0 + TF + 1 of the 16 columns: located, loaded, then converted
+  {
+    const int stride = g.nx * g.ny;
+    int cs[NR];
+    double sa[NR], sn[NR];
+    bool use[NR];
 #pragma unroll
-  for (int r = 0; r < NR; r++) {
-    const int q = threadIdx.x + 256 * r;
-    const int row = q / TX, x = q - row * TX;
-    const bool staged = q < NPT && xc * TX + x < g.nx && k0 - 3 + row <= g.nz + 2;
+    for (int r = 0; r < NR; r++) {
+      const int s = t + 256 * r;
+      const int row = s / TX, x = s - row * TX;
+      const int e = k0 - 3 + row;
+      use[r] = s < NPT && xc * TX + x < g.nx && e <= g.nz + 2;
+      cs[r] = use[r] ? line_cell<2>(g, xc * TX + x + g.nx * j, stride, g.nz, e, sa[r], sn[r])
+                     : 0;
+    }
+    RawPt raw[NR];
+    double tv[NR][NC];
 #pragma unroll
-    for (int v = 0; v < NC; v++)
-      tv[r][v] = staged ? f.chem[(long)NC * s_cell[q] + v] * s_sgn[q] : 0.0;
-  }
-  __syncthreads();   // the fluid points are dead: LDS takes the tracer values
+    for (int r = 0; r < NR; r++) {
+      if (!use[r]) continue;
+      raw[r] = load_raw<2>(f, cs[r], sa[r], sn[r]);
 #pragma unroll
-  for (int r = 0; r < NR; r++) {
-    const int q = threadIdx.x + 256 * r;
-    if (q < NPT)
+      for (int v = 0; v < NC; v++) tv[r][v] = f.chem[(long)NC * cs[r] + v] * sa[r];
+    }
 #pragma unroll
-      for (int v = 0; v < NC; v++) s_buf[v * NPT + q] = tv[r][v];
+    for (int r = 0; r < NR; r++) {
+      if (!use[r]) continue;
+      const int s = t + 256 * r;
+      const Pt P = to_point(raw[r], g.gamma);
+      s_buf[s] = P.mn * P.ir;
+      s_buf[NPT + s] = P.a;
+#pragma unroll
+      for (int v = 0; v < NC; v++) s_buf[(2 + v) * NPT + s] = tv[r][v];
+    }
   }
   __syncthreads();
+  const bool face_ok = i < g.nx && k <= g.nz;
+  double Fz[NC];
   if (face_ok) {
+    const double *pb = s_buf + ty * TX + tx;
+    double alpha = 0.0, un[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++) alpha = fmax(alpha, pb[NPT + l * TX]);
+#pragma unroll
+    for (int l = 0; l < 6; l++) un[l] = pb[l * TX];
     double hp[5], hm[5];
     tracer_split(un, alpha, hp, hm);
 #pragma unroll
     for (int v = 0; v < NC; v++) {
-      const double *c0 = s_buf + v * NPT + ty * TX + tx;
       double cv[6];
 #pragma unroll
-      for (int l = 0; l < 6; l++) cv[l] = c0[l * TX];
-      Fz[5 + v] = tracer_face_flux(cv, hp, hm);
+      for (int l = 0; l < 6; l++) cv[l] = pb[(2 + v) * NPT + l * TX];
+      Fz[v] = tracer_face_flux(cv, hp, hm);
     }
   }
-  __syncthreads();   // the tracer values are dead; LDS takes the face fluxes
+  __syncthreads();   // the points are dead; LDS takes the face fluxes [tracer][face][x]
   if (face_ok) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) s_buf[(v * TF + ty) * TX + tx] = Fz[v];
+    for (int v = 0; v < NC; v++) s_buf[(v * TF + ty) * TX + tx] = Fz[v];
   }
   __syncthreads();
   if (!cell_ok) return;
 #pragma unroll
-  for (int v = 0; v < NV; v++) {
+  for (int v = 0; v < NC; v++) {
     const double dz = (s_buf[(v * TF + ty + 1) * TX + tx] - s_buf[(v * TF + ty) * TX + tx]) * idz;
-    const double o = 0.0 - (dxy[v] + dz);
-    if (v < 5) out.w[v][cell] = o;
-    else out.chem[(long)NC * cell + (v - 5)] = o;
+    out.chem[(long)NC * cell + v] = 0.0 - (dxy[v] + dz);
   }
 }
 
@@ -682,7 +635,7 @@ __global__ void reset_flag(unsigned *flag) { *flag = 0u; }
 
 static inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-constexpr int TILE_X = 16, TILE_F = 16;
+constexpr int TILE_X = 16, TILE_F = 16;   // fe_zface_tracers' tile
 constexpr int ZFPT = 2;     // z-faces per thread in fe_zface_div_xy
 
 template <int NC>
@@ -713,16 +666,14 @@ static int launch_fe(const Grid &g, const Fields &f, const FieldsOut &o, FeScrat
                      1.0 / g.dx, 1.0 / g.dy, flag);
   tm.end(PROF_XYFACE_DIV, st);
   tm.begin(PROF_ZFACE_DIV, st);
-  if constexpr (NC == 0) {
-    constexpr int ZC = 16 * ZFPT - 1;                      // cells per z tile
-    hipLaunchKernelGGL(fe_zface_div_xy<ZFPT>,
-                       dim3((unsigned)(nxc * ((g.nz + ZC - 1) / ZC) * g.ny)), dim3(256), 0, st,
-                       g, f, o, 1.0 / g.dz);
-  } else {
-    hipLaunchKernelGGL((fe_zface_div<NC, TX, TF>),
+  constexpr int ZC = 16 * ZFPT - 1;                        // cells per z tile
+  hipLaunchKernelGGL(fe_zface_div_xy<ZFPT>,
+                     dim3((unsigned)(nxc * ((g.nz + ZC - 1) / ZC) * g.ny)), dim3(256), 0, st,
+                     g, f, o, 1.0 / g.dz);
+  if constexpr (NC > 0)
+    hipLaunchKernelGGL(fe_zface_tracers<NC>,
                        dim3((unsigned)(nxc * ((g.nz + TF - 2) / (TF - 1)) * g.ny)),
                        dim3(TX * TF), 0, st, g, f, o, 1.0 / g.dz);
-  }
   tm.end(PROF_ZFACE_DIV, st);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -357,12 +357,6 @@ __device__ inline void face_flux_lds(const double *base, int FS, int LS, double 
   face_flux_lds_f<1, 2, 7>(base, FS, LS, gamma, F, alpha_out);
 }
 
-// Normal velocities of the 6 LDS stencil points (for the species part).
-__device__ inline void lds_velocities(const double *base, int FS, int LS, double (&un)[6]) {
-#pragma unroll
-  for (int l = 0; l < 6; l++) un[l] = base[FS + l * LS] * base[6 * FS + l * LS];
-}
-
 // Species (passive tracer) face flux for one component from its 6 point values c[l]
 // (utilities.cpp:376-377,395,439,473): component-wise WENO5 of the split fluxes
 // (u c +- alpha c) / 2, no characteristic projection. The split fluxes are formed as

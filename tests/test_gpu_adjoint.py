@@ -50,7 +50,10 @@ def energy_cotangent(oc, n):
     return lam.reshape(-1)
 
 
-@pytest.mark.parametrize("n,nsteps,K,h", [(512, 12, 5, 200.0), (1024, 50, 10, 50.0)])
+# 8192 cells: above the batched step's ordering threshold, so the adjoint runs its cells in
+# temperature order (thread u takes the u-th cell of the order)
+@pytest.mark.parametrize("n,nsteps,K,h", [(512, 12, 5, 200.0), (1024, 50, 10, 50.0),
+                                          (8192, 12, 5, 200.0)])
 def test_adjoint_matches_oracle(mvd, torch_gpu, n, nsteps, K, h):
     torch = torch_gpu
     print(f"[chem adjoint] n={n} steps={nsteps} ckpt={K} h={h}", flush=True)
