@@ -522,29 +522,46 @@ fe_xyface_div(Grid g, Fields f, FieldsOut out, double idx, double idy, unsigned 
   // x-face t: row ry, faces between columns x0+fx-1 and x0+fx (points xx = fx..fx+5)
   const int ry = t / (XY_TX + 1), fx = t % (XY_TX + 1);
   const bool xok = t < XY_NXF && y0 + ry < g.ny && x0 + fx <= g.nx;
-  double alpha_y = 0.0, alpha_x = 0.0;
   if (yok) {
-    double Fl[5], Fy[5];
+    double Fl[5], Fy[5], alpha_y;
     face_flux_lds_f<2, 1, 9>(sh + fy * XY_PX + cx + 3, XY_NP, XY_PX, g.gamma, Fl, alpha_y);
     to_global<1>(Fl, Fy);
 #pragma unroll
     for (int a = 0; a < 5; a++) sfy[(a * (XY_TY + 1) + fy) * XY_TX + cx] = Fy[a];
   }
   double Fx[5];
-  if (xok) face_flux_lds_f<1, 2, 7>(sh + (ry + 3) * XY_PX + fx, XY_NP, 1, g.gamma, Fx, alpha_x);
+  if (xok) {
+    double alpha_x;
+    face_flux_lds_f<1, 2, 7>(sh + (ry + 3) * XY_PX + fx, XY_NP, 1, g.gamma, Fx, alpha_x);
+  }
   __syncthreads();                    // points dead: their LDS takes the x-face fluxes
-  double *sfx = sh;                   // [5][XY_TY][XY_TX+1]
+  double *sfx = sh;                   // [5][XY_TY][XY_TX+1] (over fields 0-2)
   if (xok)
 #pragma unroll
     for (int a = 0; a < 5; a++) sfx[(a * XY_TY + ry) * (XY_TX + 1) + fx] = Fx[a];
   __syncthreads();
+  // the faces' alpha for the tracers, formed again (as the fluid flux forms it) from the
+  // wave speeds, which the x-face fluxes did not overwrite, rather than held through the
+  // fluid fluxes
+  double alpha_y = 0.0, alpha_x = 0.0;
+  if (NC > 0) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      alpha_y = fmax(alpha_y, sh[9 * XY_NP + (fy + l) * XY_PX + cx + 3]);
+      alpha_x = fmax(alpha_x, sh[7 * XY_NP + (ry + 3) * XY_PX + fx + l]);
+    }
+  }
   // the tracer values of the thread's cross points, loaded now so that their latency
   // passes under the fluid divergence
   double tv[NC > 0 ? NR : 1][NC > 0 ? NC : 1];
   if (NC > 0) {
+    // the points are located again rather than kept from the staging: made opaque, the
+    // tile origin keeps the compiler from holding the staging's values across the fluxes
+    int ox = x0, oy = y0, ok = k;
+    __asm__ volatile("" : "+s"(ox), "+s"(oy), "+s"(ok));
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      const XYPoint P = xy_point(g, x0, y0, k, t + 256 * r);
+      const XYPoint P = xy_point(g, ox, oy, ok, t + 256 * r);
 #pragma unroll
       for (int v = 0; v < NC; v++)
         tv[r][v] = P.use ? f.chem[(long)NC * P.cell + v] * P.s_all : 0.0;
