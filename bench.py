@@ -53,8 +53,6 @@ def kernel_flops(name, n, nchem=0):
     nv = 5 + nchem
     faces = (n + 1) * n * n
     cells = n ** 3
-    if name in ("fe_xface", "fe_yface"):
-        return faces * ff
     if name == "fe_xyface_div":
         return 2 * faces * ff + cells * nv * DIV_FLOPS_XY
     if name == "fe_zface_div":

@@ -112,8 +112,9 @@ int mvd_postprocess_step(mvd_context *ctx, double t, double *const *w);
 /* Kernel timing with HIP events on the context's stream (the reference times every RHS
  * and face flux with its Profile class, profiler.hpp:23-206). mvd_profile(ctx,1) resets
  * and enables; mvd_profile_read returns the number of slots and fills per-slot total
- * milliseconds and launch counts. Slots: 0 fe_xface, 1 fe_yface, 2 fe_zface_div,
- * 3 fe_plain_faces, 4 fe_plain_div, 5 fe_xyface_div. */
+ * milliseconds and launch counts. Slots: 0, 1 unused (kept so that the numbering stays),
+ * 2 the z-face kernels (fluid, then tracers when nchem > 0), 3 fe_plain_faces,
+ * 4 fe_plain_div, 5 fe_xyface_div (the x- and y-faces for every nchem). */
 int mvd_profile(mvd_context *ctx, int on);
 int mvd_profile_read(mvd_context *ctx, double *ms, long *count, int nslot);
 

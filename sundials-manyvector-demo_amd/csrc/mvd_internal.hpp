@@ -16,7 +16,8 @@ struct FieldsOut {
 };
 
 // Kernel timing slots (HIP events recorded on the launch stream around each kernel).
-enum { PROF_XFACE = 0, PROF_YFACE, PROF_ZFACE_DIV, PROF_PLAIN_FACES, PROF_PLAIN_DIV,
+// (slots 0 and 1 are unused; the numbering is part of mvd_profile_read's interface)
+enum { PROF_UNUSED0 = 0, PROF_UNUSED1, PROF_ZFACE_DIV, PROF_PLAIN_FACES, PROF_PLAIN_DIV,
        PROF_XYFACE_DIV, PROF_NSLOT };
 
 struct KernelTimer {

@@ -431,7 +431,8 @@ class Context:
     def fslow(self, w, wdot, t=0.0):
         return _check("mvd_fslow", lib().mvd_fslow(self.h, t, dptrs(w), dptrs(wdot)))
 
-    PROFILE_SLOTS = ("fe_xface", "fe_yface", "fe_zface_div", "fe_plain_faces", "fe_plain_div",
+    # slots 0 and 1 are unused; slot 2 times the z-face kernels (fluid, then tracers)
+    PROFILE_SLOTS = ("unused0", "unused1", "fe_zface_div", "fe_plain_faces", "fe_plain_div",
                      "fe_xyface_div")
 
     def profile(self, on=True):
