@@ -341,47 +341,6 @@ __device__ inline void chem_rhs_species(const double (&y)[NSP], const double (&k
          R.r18;
 }
 
-// Scalar with one tangent (forward-mode derivative), for the gradient of the gas-energy
-// row in the vector-Jacobian product; all operations without FMA contraction.
-struct Dual1 {
-  double v, d;
-};
-__device__ inline Dual1 operator+(Dual1 a, Dual1 b) {
-#pragma clang fp contract(off)
-  return {a.v + b.v, a.d + b.d};
-}
-__device__ inline Dual1 operator-(Dual1 a, Dual1 b) {
-#pragma clang fp contract(off)
-  return {a.v - b.v, a.d - b.d};
-}
-__device__ inline Dual1 operator-(Dual1 a) { return {-a.v, -a.d}; }
-__device__ inline Dual1 operator*(Dual1 a, Dual1 b) {
-#pragma clang fp contract(off)
-  return {a.v * b.v, a.d * b.v + a.v * b.d};
-}
-__device__ inline Dual1 operator/(Dual1 a, Dual1 b) {
-#pragma clang fp contract(off)
-  const double q = a.v / b.v;
-  return {q, (a.d - q * b.d) / b.v};
-}
-__device__ inline Dual1 operator+(double a, Dual1 b) { return {a + b.v, b.d}; }
-__device__ inline Dual1 operator+(Dual1 a, double b) { return {a.v + b, a.d}; }
-__device__ inline Dual1 operator-(double a, Dual1 b) { return {a - b.v, -b.d}; }
-__device__ inline Dual1 operator-(Dual1 a, double b) { return {a.v - b, a.d}; }
-__device__ inline Dual1 operator*(double a, Dual1 b) {
-#pragma clang fp contract(off)
-  return {a * b.v, a * b.d};
-}
-__device__ inline Dual1 operator*(Dual1 a, double b) {
-#pragma clang fp contract(off)
-  return {a.v * b, a.d * b};
-}
-__device__ inline Dual1 operator/(double a, Dual1 b) {
-#pragma clang fp contract(off)
-  const double q = a / b.v;
-  return {q, (-q * b.d) / b.v};
-}
-
 template <class R>
 __device__ inline R chem_rhs_energy_t(const R (&y)[NSP], const R (&k)[NRATE], R T, double z,
                                       const CellAux &ax) {
@@ -412,6 +371,101 @@ __device__ inline double chem_rhs_energy(const double (&y)[NSP], const double (&
   return chem_rhs_energy_t<double>(y, k, T, z, ax);
 }
 
+// Gradient of the gas-energy row f_9 = chem_rhs_energy_t(y, k, T) by one reverse sweep of
+// its expression: g[s] = d f_9 / d y_s (s = 0..8; zero for the species it does not read)
+// and g[9] = d f_9 / dT through the rates (dk = dk/dT) and the Compton term's explicit T.
+// The same partial derivatives as forward tangents along each direction, formed once.
+__device__ inline void chem_energy_grad(const double (&y)[NSP], const double (&k)[NRATE],
+                                        const double (&dk)[NRATE], double T, double z,
+                                        const CellAux &ax, double (&g)[NSP]) {
+#pragma clang fp contract(off)
+  const double H2_1 = y[0], H_1 = y[2], H_2 = y[3], He_1 = y[5], He_2 = y[6], He_3 = y[7],
+               de = y[8];
+  const double c = ax.cie, h2o = ax.h2o, md = ax.mdensity;
+  double kb[NRATE];                                 // d f_9 / d k_r
+#pragma unroll
+  for (int r = 0; r < NRATE; r++) kb[r] = 0.0;
+  double b0 = 0.0, b2 = 0.0, b3 = 0.0, b5 = 0.0, b6 = 0.0, b7 = 0.0, b8 = 0.0;
+  // -2.01588 H2_1 k_CIECO cie mdensity
+  b0 += -2.0158800000000001 * k[C_CIECO] * c * md;
+  kb[C_CIECO] += -2.0158800000000001 * H2_1 * c * md;
+  // -H2_1 cie k_H2LTE h2o / (k_H2LTE / gS + 1)
+  {
+    const double gS = H2_1 * k[C_GAH2] + H_1 * k[C_GAHI] + H_2 * k[C_GAHP] + He_1 * k[C_GAHE] +
+                      de * k[C_GAEL];
+    const double D = k[C_H2LTE] / gS + 1.0, iD = 1.0 / D;
+    const double N = H2_1 * c * k[C_H2LTE] * h2o;
+    const double Db = N * iD * iD;                  // d(-N/D)/dD
+    const double gSb = Db * (-k[C_H2LTE] / (gS * gS));
+    b0 += -(c * k[C_H2LTE] * h2o) * iD;
+    kb[C_H2LTE] += -(H2_1 * c * h2o) * iD + Db / gS;
+    b0 += gSb * k[C_GAH2];  kb[C_GAH2] += gSb * H2_1;
+    b2 += gSb * k[C_GAHI];  kb[C_GAHI] += gSb * H_1;
+    b3 += gSb * k[C_GAHP];  kb[C_GAHP] += gSb * H_2;
+    b5 += gSb * k[C_GAHE];  kb[C_GAHE] += gSb * He_1;
+    b8 += gSb * k[C_GAEL];  kb[C_GAEL] += gSb * de;
+  }
+  // atomic and ionic line cooling, recombination: -X k cie de (and de^2) terms
+  b2 += -(k[C_CEHI] + k[C_CIHI]) * c * de;
+  b8 += -(k[C_CEHI] + k[C_CIHI]) * c * H_1;
+  kb[C_CEHI] += -H_1 * c * de;
+  kb[C_CIHI] += -H_1 * c * de;
+  b3 += -c * de * k[C_REHII];
+  b8 += -H_2 * c * k[C_REHII];
+  kb[C_REHII] += -H_2 * c * de;
+  b5 += -k[C_CIHEI] * c * de;
+  b8 += -He_1 * k[C_CIHEI] * c;
+  kb[C_CIHEI] += -He_1 * c * de;
+  {
+    const double sa = k[C_CEHEII] + k[C_CIHEII] + k[C_REHEII1] + k[C_REHEII2];
+    const double sb = k[C_CEHEI] + k[C_CIHEIS];
+    b6 += -c * (de * sa + (de * de) * sb);
+    b8 += -He_2 * c * (sa + 2.0 * de * sb);
+    const double ka = -He_2 * c * de, kq = -He_2 * c * (de * de);
+    kb[C_CEHEII] += ka; kb[C_CIHEII] += ka; kb[C_REHEII1] += ka; kb[C_REHEII2] += ka;
+    kb[C_CEHEI] += kq;  kb[C_CIHEIS] += kq;
+  }
+  b7 += -c * de * k[C_REHEIII];
+  b8 += -He_3 * c * k[C_REHEIII];
+  kb[C_REHEIII] += -He_3 * c * de;
+  // bremsstrahlung
+  {
+    const double S = H_2 + He_2 + 4.0 * He_3;
+    kb[C_BREM] += -c * de * S;
+    b8 += -k[C_BREM] * c * S;
+    const double sb = -k[C_BREM] * c * de;
+    b3 += sb; b6 += sb; b7 += 4.0 * sb;
+  }
+  // Compton: -cie k_COMPTON de (z+1)^4 (T - 2.73 z - 2.73)
+  const double zp1 = z + 1.0;
+  const double zp4 = (zp1 * zp1) * (zp1 * zp1);
+  const double dTc = T - 2.73 * z - 2.73;
+  kb[C_COMPTON] += -c * de * zp4 * dTc;
+  b8 += -c * k[C_COMPTON] * zp4 * dTc;
+  double Tb = -c * k[C_COMPTON] * de * zp4;
+  // H2 formation heating / cooling: 0.5 / (k_NCRN / ncrD + 1) Q
+  {
+    const double ncrD = H2_1 * k[C_NCRD2] + H_1 * k[C_NCRD1];
+    const double E = k[C_NCRN] / ncrD + 1.0, iE = 1.0 / E;
+    const double Q = (-H2_1) * H_1 * k[C_H2MCOOL] + (H_1 * H_1 * H_1) * k[C_H2MHEAT];
+    const double Qb = 0.5 * iE, Eb = -0.5 * Q * iE * iE;
+    const double ncrDb = Eb * (-k[C_NCRN] / (ncrD * ncrD));
+    kb[C_NCRN] += Eb / ncrD;
+    b0 += Qb * (-H_1 * k[C_H2MCOOL]) + ncrDb * k[C_NCRD2];
+    b2 += Qb * (-H2_1 * k[C_H2MCOOL] + 3.0 * (H_1 * H_1) * k[C_H2MHEAT]) + ncrDb * k[C_NCRD1];
+    kb[C_H2MCOOL] += Qb * (-H2_1 * H_1);
+    kb[C_H2MHEAT] += Qb * (H_1 * H_1 * H_1);
+    kb[C_NCRD2] += ncrDb * H2_1;
+    kb[C_NCRD1] += ncrDb * H_1;
+  }
+#pragma unroll
+  for (int r = C_BREM; r < NRATE; r++) Tb += kb[r] * dk[r];
+#pragma unroll
+  for (int s = 0; s < NSP; s++) g[s] = 0.0;
+  g[0] = b0; g[2] = b2; g[3] = b3; g[5] = b5; g[6] = b6; g[7] = b7; g[8] = b8;
+  g[NSP - 1] = Tb;
+}
+
 // ---- vector-Jacobian product of the RHS --------------------------------------------
 
 // Reactions in the order of Reactions (r01..r19, r21, r22): rate index (the same position in
@@ -438,8 +492,8 @@ __device__ constexpr signed char CHEM_STO[NSP - 1][NREACT] = {
 // units) with the temperature T = T(y), the root of the temperature equation
 // g(T, y) = T kb nsum(T, y) / (density(y) mh) - ge = 0 (so dT/dy = -(dg/dy) / (dg/dT), with
 // dg/dT from the slopes of the interpolated gamma tables). The species rows are
-// differentiated in reverse through the reactions, the gas-energy row in forward mode
-// along each species it depends on and along T. `clamped`: T sits on a clamp bound, so it
+// differentiated in reverse through the reactions, the gas-energy row by a reverse sweep
+// of its expression (chem_energy_grad). `clamped`: T sits on a clamp bound, so it
 // does not depend on y.
 //
 // In two parts: chem_vjp_prepare forms what does not depend on vp (reaction rates and
@@ -449,7 +503,7 @@ __device__ constexpr signed char CHEM_STO[NSP - 1][NREACT] = {
 // one row at a time, for many cotangents at one state).
 struct ChemVjpPre {
   double k[NREACT], dk[NREACT];   // reaction rates and T-derivatives (reaction order)
-  double e9[NSP];                 // d f_9 / d y_s along H2_1, H_1, H_2, He_1..3, de; [9]: d/dT
+  double e9[NSP];                 // d f_9 / d y_s (H2_1, H_1, H_2, He_1..3, de); [9]: d/dT
   double dgdT, dgdy[NSP - 1];     // temperature path (not formed when clamped)
   bool clamped;
 };
@@ -464,24 +518,9 @@ __device__ inline void chem_vjp_prepare(const ChemConst &c, const double (&y)[NS
     P.k[j] = k[j];
     P.dk[j] = dk[j];
   }
-  // gas-energy row: forward tangents along H2_1, H_1, H_2, He_1, He_2, He_3, de and T
-#pragma unroll
-  for (int s = 0; s < NSP; s++) P.e9[s] = 0.0;
-  {
-    Dual1 yd[NSP], kd[NRATE];
-    constexpr int dirs[8] = {0, 2, 3, 5, 6, 7, 8, NSP};
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int s = dirs[q];
-#pragma unroll
-      for (int u = 0; u < NSP; u++) yd[u] = Dual1{y[u], u == s ? 1.0 : 0.0};
-#pragma unroll
-      for (int r = 0; r < NRATE; r++) kd[r] = Dual1{k[r], s == NSP ? dk[r] : 0.0};
-      const Dual1 f9 = chem_rhs_energy_t<Dual1>(yd, kd, Dual1{T, s == NSP ? 1.0 : 0.0},
-                                                c.z, ax);
-      P.e9[s == NSP ? NSP - 1 : s] = f9.d;   // the T tangent in slot 9 (ge has none)
-    }
-  }
+  // gas-energy row: its gradient along H2_1, H_1, H_2, He_1, He_2, He_3, de and T (in
+  // slot 9: ge has none)
+  chem_energy_grad(y, k, dk, T, c.z, ax, P.e9);
   P.clamped = clamped;
   P.dgdT = 1.0;
 #pragma unroll
