@@ -34,6 +34,10 @@ constexpr double W_C116 = 1.833333333333333333333333333333333333333;
 
 // WENO5-JS reconstruction from f[0..4] (side 0: the f+ formula over points i-3..i+1,
 // side 1: the mirrored f- formula over points i-2..i+2) and its gradient g = dr/df.
+// With the normalised weights o_k = w_k / sum w: r = sum o_k q_k and
+//   dr/df = sum o_k dq_k/df + sum_k kap_k db_k/df,   kap_k = -2 o_k (q_k - r) / e_k,
+// db_k/df = 2 (13/12) A_k dA_k/df + (B_k / 2) dB_k/df; the gradient is written out term by
+// term (no products with the zero entries of the stencil derivatives).
 __device__ inline double weno5_grad(const double (&f)[5], int side, double (&g)[5]) {
   const double A1 = f[2] - 2.0 * f[3] + f[4], B1 = 3.0 * f[2] - 4.0 * f[3] + f[4];
   const double A2 = f[1] - 2.0 * f[2] + f[3], B2 = f[1] - f[3];
@@ -45,39 +49,36 @@ __device__ inline double weno5_grad(const double (&f)[5], int side, double (&g)[
   const double e1 = W_EPS + b1, e2 = W_EPS + b2, e3 = W_EPS + b3;
   const double ie1 = fast_rcp(e1), ie2 = fast_rcp(e2), ie3 = fast_rcp(e3);
   const double w1 = d1 * ie1 * ie1, w2 = 0.6 * ie2 * ie2, w3 = d3 * ie3 * ie3;
+  // candidate stencils q_k and their coefficients (dq_k/df over the points they use)
   double q1, q2, q3;
-  double dq1[5] = {0, 0, 0, 0, 0}, dq2[5] = {0, 0, 0, 0, 0}, dq3[5] = {0, 0, 0, 0, 0};
+  double c10, c11, c12, c21, c22, c23, c32, c33, c34;   // dq3/df0..2, dq2/df1..3, dq1/df2..4
   if (side == 0) {
-    q1 = W_C13 * f[2] + W_C56 * f[3] - W_C16 * f[4];
-    q2 = -W_C16 * f[1] + W_C56 * f[2] + W_C13 * f[3];
-    q3 = W_C13 * f[0] - W_C76 * f[1] + W_C116 * f[2];
-    dq1[2] = W_C13; dq1[3] = W_C56; dq1[4] = -W_C16;
-    dq2[1] = -W_C16; dq2[2] = W_C56; dq2[3] = W_C13;
-    dq3[0] = W_C13; dq3[1] = -W_C76; dq3[2] = W_C116;
+    c10 = W_C13; c11 = -W_C76; c12 = W_C116;
+    c21 = -W_C16; c22 = W_C56; c23 = W_C13;
+    c32 = W_C13; c33 = W_C56; c34 = -W_C16;
   } else {
-    q1 = W_C116 * f[2] - W_C76 * f[3] + W_C13 * f[4];
-    q2 = W_C13 * f[1] + W_C56 * f[2] - W_C16 * f[3];
-    q3 = -W_C16 * f[0] + W_C56 * f[1] + W_C13 * f[2];
-    dq1[2] = W_C116; dq1[3] = -W_C76; dq1[4] = W_C13;
-    dq2[1] = W_C13; dq2[2] = W_C56; dq2[3] = -W_C16;
-    dq3[0] = -W_C16; dq3[1] = W_C56; dq3[2] = W_C13;
+    c10 = -W_C16; c11 = W_C56; c12 = W_C13;
+    c21 = W_C13; c22 = W_C56; c23 = -W_C16;
+    c32 = W_C116; c33 = -W_C76; c34 = W_C13;
   }
+  q1 = c32 * f[2] + c33 * f[3] + c34 * f[4];
+  q2 = c21 * f[1] + c22 * f[2] + c23 * f[3];
+  q3 = c10 * f[0] + c11 * f[1] + c12 * f[2];
   const double iS = fast_rcp(w1 + w2 + w3);
-  const double r = (q1 * w1 + q2 * w2 + q3 * w3) * iS;
-  // dw_i/df = -2 w_i / e_i * db_i/df
-  const double k1 = -2.0 * w1 * ie1 * (q1 - r) * iS;
-  const double k2 = -2.0 * w2 * ie2 * (q2 - r) * iS;
-  const double k3 = -2.0 * w3 * ie3 * (q3 - r) * iS;
-  const double db1[5] = {0.0, 0.0, 2.0 * W_BC * A1 + 1.5 * B1, -4.0 * W_BC * A1 - 2.0 * B1,
-                         2.0 * W_BC * A1 + 0.5 * B1};
-  const double db2[5] = {0.0, 2.0 * W_BC * A2 + 0.5 * B2, -4.0 * W_BC * A2,
-                         2.0 * W_BC * A2 - 0.5 * B2, 0.0};
-  const double db3[5] = {2.0 * W_BC * A3 + 0.5 * B3, -4.0 * W_BC * A3 - 2.0 * B3,
-                         2.0 * W_BC * A3 + 1.5 * B3, 0.0, 0.0};
-#pragma unroll
-  for (int j = 0; j < 5; j++)
-    g[j] = (w1 * dq1[j] + w2 * dq2[j] + w3 * dq3[j]) * iS + k1 * db1[j] + k2 * db2[j] +
-           k3 * db3[j];
+  const double o1 = w1 * iS, o2 = w2 * iS, o3 = w3 * iS;
+  const double r = q1 * o1 + q2 * o2 + q3 * o3;
+  const double k1 = -2.0 * o1 * ie1 * (q1 - r);
+  const double k2 = -2.0 * o2 * ie2 * (q2 - r);
+  const double k3 = -2.0 * o3 * ie3 * (q3 - r);
+  // kap_k db_k/df = a_k dA_k/df + c_k dB_k/df
+  const double a1 = (2.0 * W_BC) * k1 * A1, h1 = 0.5 * k1 * B1;   // dA1 (0,0,1,-2,1), dB1 (0,0,3,-4,1)
+  const double a2 = (2.0 * W_BC) * k2 * A2, h2 = 0.5 * k2 * B2;   // dA2 (0,1,-2,1,0), dB2 (0,1,0,-1,0)
+  const double a3 = (2.0 * W_BC) * k3 * A3, h3 = 0.5 * k3 * B3;   // dA3 (1,-2,1,0,0), dB3 (1,-4,3,0,0)
+  g[0] = o3 * c10 + a3 + h3;
+  g[1] = o2 * c21 + o3 * c11 + a2 + h2 - 2.0 * a3 - 4.0 * h3;
+  g[2] = o1 * c32 + o2 * c22 + o3 * c12 + a1 + 3.0 * h1 - 2.0 * a2 + a3 + 3.0 * h3;
+  g[3] = o1 * c33 + o2 * c23 - 2.0 * a1 - 4.0 * h1 + a2 - h2;
+  g[4] = o1 * c34 + a1 + h1;
   return r;
 }
 
@@ -106,19 +107,22 @@ __device__ inline void face_prim(const double (&s)[6][5], double gamma, FacePrim
 }
 
 // One tracer: t[6] the stencil values, Fbc its flux cotangent; tb[6] receives the tracer
-// adjoint, ulb and alphab accumulate.
+// adjoint, ulb and alphab accumulate. The split flux fs = (ul t +- alpha t) / 2 is formed as
+// t hs with the point's split factor hs = (ul +- alpha) / 2 (ha = alpha / 2), whose
+// derivatives are hs (t), t / 2 (ul) and +-t / 2 (alpha).
 __device__ inline void tracer_vjp(const double (&t)[6], double Fbc, const FacePrim &P,
-                                  double (&tb)[6], double (&ulb)[6], double &alphab) {
+                                  double ha, double (&tb)[6], double (&ulb)[6],
+                                  double &alphab) {
 #pragma unroll
   for (int l = 0; l < 6; l++) tb[l] = 0.0;
 #pragma unroll
   for (int sd = 0; sd < 2; sd++) {
-    double fa[5], g[5];
+    double fa[5], hs[5], g[5];
 #pragma unroll
     for (int l = 0; l < 5; l++) {
       const int pt = l + sd;
-      fa[l] = sd == 0 ? 0.5 * (P.ul[pt] * t[pt] + P.alpha * t[pt])
-                      : 0.5 * (P.ul[pt] * t[pt] - P.alpha * t[pt]);
+      hs[l] = fma(0.5, P.ul[pt], sd == 0 ? ha : -ha);
+      fa[l] = t[pt] * hs[l];
     }
     (void)weno5_grad(fa, sd, g);
     const double sg = sd == 0 ? 0.5 : -0.5;
@@ -126,10 +130,10 @@ __device__ inline void tracer_vjp(const double (&t)[6], double Fbc, const FacePr
     for (int l = 0; l < 5; l++) {
       const int pt = l + sd;
       const double fsb = Fbc * g[l];
-      // fs = 0.5 (ul t +- alpha t)
-      ulb[pt] += 0.5 * fsb * t[pt];
-      tb[pt] += 0.5 * fsb * P.ul[pt] + sg * P.alpha * fsb;
-      alphab += sg * fsb * t[pt];
+      const double ft = fsb * t[pt];
+      ulb[pt] += 0.5 * ft;
+      alphab += sg * ft;
+      tb[pt] += fsb * hs[l];
     }
   }
 }
@@ -399,7 +403,7 @@ __device__ inline void face_vjp(const Grid &g, const Fields &w, const Fields &vc
         load_tracer(c + 1);
       }
       __asm__ volatile("" ::: "memory");
-      tracer_vjp(t, Fbc, P, tb, ulb, alphab);
+      tracer_vjp(t, Fbc, P, 0.5 * P.alpha, tb, ulb, alphab);
 #pragma unroll
       for (int l = 0; l < 6; l++) tb[l] *= sa[l];
       sink.put(5 + c, tb, pre);
