@@ -331,11 +331,6 @@ __device__ inline int gcomp(int d, int a) {
   return a;
 }
 
-__device__ inline long face_index(const Grid &g, int d, long i, long j, long k) {
-  return d == 0 ? i + (long)(g.nx + 1) * (j + (long)g.ny * k)
-       : d == 1 ? i + (long)g.nx * (j + (long)(g.ny + 1) * k)
-                : i + (long)g.nx * (j + (long)g.ny * k);
-}
 
 
 }  // namespace
@@ -523,7 +518,10 @@ fe_vjp_sweep_x(Grid g, Fields w, Fields vc, FieldsOut out, double *side, int lps
 }
 
 // Ghost contributions of each x-line into their source cells (one thread per line and
-// component, pairs in a fixed order).
+// component, pairs in a fixed order). When the six ghost points e = -3..-1, n..n+2 have six
+// distinct source cells (lines of six cells or more), each source takes its pairs' sums in
+// registers, in the pairs' order, and is read and written once (all loads issued
+// together): the same sums as one read-modify-write per pair.
 template <int NC>
 __global__ void __launch_bounds__(256)
 fe_vjp_fix_x(Grid g, const double *side, FieldsOut out) {
@@ -533,6 +531,48 @@ fe_vjp_fix_x(Grid g, const double *side, FieldsOut out) {
   if (tid >= (long)nlines * NV) return;
   const int line = (int)(tid % nlines), qq = (int)(tid / nlines);
   const int n = g.nx;
+  const long ncell = (long)n * nlines;
+  int src[6];
+#pragma unroll
+  for (int e = 0; e < 6; e++) src[e] = ghost_map(g, 0, e < 3 ? e - 3 : n + (e - 3)).src;
+  bool distinct = n >= 6;
+#pragma unroll
+  for (int a = 0; a < 6; a++)
+#pragma unroll
+    for (int b = a + 1; b < 6; b++) distinct = distinct && src[a] != src[b];
+  if (distinct) {
+    // ghost e = f - 3 + l: e = -3 (0,0); -2 (0,1) (1,0); -1 (0,2) (1,1) (2,0);
+    // n (n-2,5) (n-1,4) (n,3); n+1 (n-1,5) (n,4); n+2 (n,5), in pair order
+    const auto sv = [&](int f, int l) {
+      return side[((long)xghost_pair(n, f, l) * NV + qq) * nlines + line];
+    };
+    double add[6];
+    add[0] = sv(0, 0);
+    add[1] = sv(0, 1); add[2] = sv(0, 2);
+    const double a10 = sv(1, 0), a11 = sv(1, 1), a20 = sv(2, 0);
+    add[3] = sv(n - 2, 5); add[4] = sv(n - 1, 5); add[5] = sv(n, 5);
+    const double b14 = sv(n - 1, 4), b03 = sv(n, 3), b04 = sv(n, 4);
+    long cell[6];
+    double old[6];
+#pragma unroll
+    for (int e = 0; e < 6; e++) {
+      cell[e] = src[e] + (long)n * line;
+      old[e] = qq < 5 ? out.w[qq][cell[e]] : out.chem[(long)(qq - 5) * ncell + cell[e]];
+    }
+    double v[6];
+    v[0] = old[0] + add[0];
+    v[1] = (old[1] + add[1]) + a10;
+    v[2] = ((old[2] + add[2]) + a11) + a20;
+    v[3] = ((old[3] + add[3]) + b14) + b03;
+    v[4] = (old[4] + add[4]) + b04;
+    v[5] = old[5] + add[5];
+#pragma unroll
+    for (int e = 0; e < 6; e++) {
+      if (qq < 5) out.w[qq][cell[e]] = v[e];
+      else out.chem[(long)(qq - 5) * ncell + cell[e]] = v[e];
+    }
+    return;
+  }
   for (int f = 0; f <= n; f++) {
     if (f > 2 && f < n - 2) { f = n - 3; continue; }
 #pragma unroll 1
@@ -657,15 +697,41 @@ fe_vjp_fix_yz(Grid g, const double *side, FieldsOut out, int nseg) {
   const int i = (int)(col % g.nx), t = (int)(col / g.nx);
   const long rstride = D == 1 ? g.nx : (long)g.nx * g.ny;
   const long cbase = D == 1 ? i + (long)g.nx * g.ny * t : i + (long)g.nx * t;
+  const long ncell = (long)g.nx * g.ny * g.nz;
   for (int seg = 0; seg < nseg; seg++) {
     const int c0 = (int)((long)seg * n / nseg), c1 = (int)((long)(seg + 1) * n / nseg);
+    // when a segment's six side rows land in six distinct cells (segments of at least
+    // three rows away from short columns), all its reads are issued together
+    int src[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) src[k] = ghost_map(g, D, k < 3 ? c0 - 3 + k : c1 + (k - 3)).src;
+    bool distinct = true;
+#pragma unroll
+    for (int a = 0; a < 6; a++)
+#pragma unroll
+      for (int b = a + 1; b < 6; b++) distinct = distinct && src[a] != src[b];
+    if (distinct) {
+      long cell[6];
+      double add[6], old[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        cell[k] = cbase + (long)src[k] * rstride;
+        add[k] = side[(((long)seg * 6 + k) * NV + qq) * ncol + col];
+        old[k] = qq < 5 ? out.w[qq][cell[k]] : out.chem[(long)(qq - 5) * ncell + cell[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        if (qq < 5) out.w[qq][cell[k]] = old[k] + add[k];
+        else out.chem[(long)(qq - 5) * ncell + cell[k]] = old[k] + add[k];
+      }
+      continue;
+    }
 #pragma unroll 1
     for (int k = 0; k < 6; k++) {
       const int e = k < 3 ? c0 - 3 + k : c1 + (k - 3);
       const int src = ghost_map(g, D, e).src;
       const long cell = cbase + (long)src * rstride;
-      out_put(out, (long)g.nx * g.ny * g.nz, qq, cell,
-              side[(((long)seg * 6 + k) * NV + qq) * ncol + col], true);
+      out_put(out, ncell, qq, cell, side[(((long)seg * 6 + k) * NV + qq) * ncol + col], true);
     }
   }
 }
