@@ -73,8 +73,8 @@ int mvd_last_state_flags(mvd_context *ctx);
  * wdot. The derivative of the Lax-Friedrichs speed max() follows its first maximum.
  * Keeps no face buffer. On first use it allocates its scratch: a side buffer for the stencil
  * adjoints that leave an x-line or a y/z column segment (12*(5+nchem) doubles per x-line,
- * 6*(5+nchem) per column segment) and, with tracers, three tracer-major planes of
- * nchem*nx*ny*nz doubles each (w, v and the tracer adjoint). */
+ * 6*(5+nchem) per column segment) and, with tracers, two tracer-major planes of
+ * nchem*nx*ny*nz doubles each (w and the tracer adjoint). */
 int mvd_fe_vjp(mvd_context *ctx, double t, const double *const *w, const double *const *v,
                double *const *wbar);
 

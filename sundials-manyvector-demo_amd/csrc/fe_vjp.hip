@@ -14,8 +14,9 @@
 //   fe_vjp_fix_x/_yz - add the contributions that left a line or segment (ghost points,
 //                      rows of neighbouring segments), held in a small side buffer.
 // The x sweep stores wbar, the others add. No atomics: the result is deterministic.
-// Tracers are handled tracer-major inside (cells_to_planes / planes_to_cells around the
-// sweeps); the interface keeps the cell-major layout.
+// The tracers of w and wbar are handled tracer-major inside (cells_to_planes /
+// planes_to_cells around the sweeps); the cotangent's, read at two cells per face, stay
+// cell-major; the interface keeps the cell-major layout.
 #include "mvd_device.hpp"
 #include "mvd_internal.hpp"
 #include "../../include/mvd_rhs.h"
@@ -381,8 +382,8 @@ __device__ inline void face_vjp(const Grid &g, const Fields &w, const Fields &vc
       const long pl = (long)c * ncell;
 #pragma unroll
       for (int l = 0; l < 6; l++) tn[l] = w.chem[pl + cell[l]];
-      rn = vc.chem[pl + cr];
-      ln = vc.chem[pl + cl];
+      rn = vc.chem[(long)NC * cr + c];   // the cotangent stays cell-major (two cells a face)
+      ln = vc.chem[(long)NC * cl + c];
     };
     auto pre_next = sink.pre(5);
     load_tracer(0);
@@ -799,7 +800,7 @@ static VjpPlan vjp_plan(const Grid &g) {
   return p;
 }
 
-// Scratch: the side buffer, then (with tracers) tracer planes of w, v and wbar.
+// Scratch: the side buffer, then (with tracers) tracer planes of w and wbar.
 template <int NC>
 static int fe_vjp_run(const Grid &g, const Fields &w, const Fields &v, const FieldsOut &out,
                       double *buf, hipStream_t st) {
@@ -813,11 +814,9 @@ static int fe_vjp_run(const Grid &g, const Fields &w, const Fields &v, const Fie
   FieldsOut op = out;
   const unsigned tb = (unsigned)((ncell + 255) / 256);
   if constexpr (NC > 0) {
-    double *pw = buf + p.side, *pv = pw + NC * ncell, *po = pv + NC * ncell;
+    double *pw = buf + p.side, *po = pw + NC * ncell;
     hipLaunchKernelGGL(cells_to_planes<NC>, dim3(tb), dim3(256), 0, st, w.chem, pw, ncell);
-    hipLaunchKernelGGL(cells_to_planes<NC>, dim3(tb), dim3(256), 0, st, v.chem, pv, ncell);
     wp.chem = pw;
-    vp.chem = pv;
     op.chem = po;
   }
   hipLaunchKernelGGL(fe_vjp_sweep_x<NC>, dim3((unsigned)((nlines + p.lps - 1) / p.lps)),
@@ -841,7 +840,7 @@ static int fe_vjp_run(const Grid &g, const Fields &w, const Fields &v, const Fie
 }
 
 size_t fe_vjp_buffer_doubles(const Grid &g) {
-  return vjp_plan(g).side + (size_t)3 * g.nchem * g.nx * g.ny * g.nz;
+  return vjp_plan(g).side + (size_t)2 * g.nchem * g.nx * g.ny * g.nz;
 }
 
 int fe_vjp_launch(const Grid &g, const Fields &w, const Fields &v, const FieldsOut &out,
