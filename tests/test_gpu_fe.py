@@ -52,7 +52,8 @@ BCS = {
 @pytest.mark.parametrize("bcname", list(BCS))
 @pytest.mark.parametrize("dims,nchem", [((12, 9, 7), 0), ((8, 6, 10), 2), ((7, 8, 6), 10),
                                         ((3, 5, 4), 0), ((37, 33, 9), 0), ((9, 7, 11), 1),
-                                        ((17, 12, 5), 4), ((6, 9, 8), 6), ((70, 5, 4), 10)])
+                                        ((17, 12, 5), 4), ((6, 9, 8), 6), ((70, 5, 4), 10),
+                                        ((40, 33, 35), 10), ((33, 31, 17), 4)])
 def test_fe_plain_bitwise_and_fast(mvd, torch_gpu, bcname, dims, nchem):
     torch = torch_gpu
     bc = BCS[bcname]

@@ -103,7 +103,11 @@ def test_chem_vjp_conservation_cotangents(mvd, torch_gpu):
                                            ((70, 5, 4), 10, (0,) * 6),
                                            # several segments per y and z column and
                                            # several x-lines per wave
-                                           ((18, 24, 16), 1, (1, 3, 0, 0, 3, 1))])
+                                           ((18, 24, 16), 1, (1, 3, 0, 0, 3, 1)),
+                                           # x-lines of five cells: two periodic ghost
+                                           # points share a source cell (the x fix-up's
+                                           # one-pair-at-a-time path)
+                                           ((5, 9, 7), 2, (0,) * 6)])
 def test_fe_vjp_matches_oracle(mvd, torch_gpu, dims, nchem, bc):
     torch = torch_gpu
     print(f"[fe vjp] dims={dims} nchem={nchem} bc={bc}", flush=True)
