@@ -468,8 +468,8 @@ def main():
                          "kernel), and nothing else")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the fe() halo exchange design is described in DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU: there is no multi-GPU path (a multi-GPU fe() would "
+              "exchange 3-deep halos between 3D blocks of the grid); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 

@@ -370,11 +370,16 @@ int dirk_fit_block(int block, int nstage, size_t static_lds, size_t limit) {
 // exactly. A cell that fails at the finest size keeps its last accepted state and counts
 // as failed. Without SUBSTEP a failed cell keeps its last Newton iterate (the plain
 // solver's behaviour).
+// nsteps: steps of size h per cell in this launch (a fixed-step run with constant forcing,
+// such as the fast steps of one multirate slow stage); each cell runs them back to back,
+// its state and temperature staying in registers, every step exactly as a launch of its
+// own would run it (a failed step counts and the next starts from the state it kept).
+// With nsteps > 1 the statistics are per cell over all its steps (failures: failed steps).
 template <bool FORCED, bool SUBSTEP>
 __global__ void __launch_bounds__(256)
 chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double *y,
                   const double *force, const int *order, const double2 *tdt, int *queue,
-                  int *rows) {
+                  int *rows, int nsteps) {
 #pragma clang fp contract(off)
   // block statistics in LDS (the per-cell updates are LDS atomics; one row per block).
   // Dynamic LDS: the gamma tables of the temperature solve (32 KB; the ten-step Newton loop
@@ -442,8 +447,8 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     if (!SUBSTEP && lane == 0) q = atomicAdd(queue, 64) + nwaves * 64;
     // (the substepping variants, which hold more state, solve it again: skipping there
     // costs registers that are worth more than the solve)
-    bool setup = true, ok = true, solved = !SUBSTEP && tdt != nullptr;
-    int st = 0, it = 0, setups = 0, nit = 0, nsetup = 0;
+    bool setup = true, solved = !SUBSTEP && tdt != nullptr;
+    int st = 0, it = 0, setups = 0, nit = 0, nsetup = 0, nst = 0, nfail = 0;
     double hs = h;                                         // current (sub)step
     double hg = h * o.A[0];                                // SDIRK: one diagonal value
     int halv = 0, done = 0;                                // substep level, units done
@@ -484,35 +489,38 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
       for (int s = 0; s < NSP; s++) z[s] += d[s];
       nit++;
       it++;
+      bool step_end = false;
       if (wrms_norm(d, z, o.rtol, o.atol) <= o.ntol) {
         if (++st == o.nstage) {
-          if (!SUBSTEP) break;
-          done += 1 << (o.max_halvings - halv);
-          if (done >= (1 << o.max_halvings)) break;
-          // substep accepted: it becomes the start of the next one
+          if (SUBSTEP) done += 1 << (o.max_halvings - halv);
+          step_end = !SUBSTEP || done >= (1 << o.max_halvings);
+          if (!step_end) {
+            // substep accepted: it becomes the start of the next one
 #pragma unroll
-          for (int s = 0; s < NSP; s++) { yst[s * 64] = z[s]; r[s] = z[s] + hs * 0.0; }
-          Ts0 = Ts;
-          dTs0 = dTs;
-          st = 0;
+            for (int s = 0; s < NSP; s++) { yst[s * 64] = z[s]; r[s] = z[s] + hs * 0.0; }
+            Ts0 = Ts;
+            dTs0 = dTs;
+            st = 0;
+            it = 0;
+            setups = 0;
+            setup = true;
+            continue;
+          }
+        } else {
+          // F_{st-1} = (z - r)/hg (ARKODE's implicit-stage derivative); the next stage's r
+#pragma unroll
+          for (int s = 0; s < NSP; s++) Fst[((st - 1) * NSP + s) * 64] = (z[s] - r[s]) / hg;
+#pragma unroll
+          for (int s = 0; s < NSP; s++) {
+            double a = 0.0;
+#pragma unroll
+            for (int j = 0; j < MAXSTAGE - 1; j++)
+              if (j < st) a += dirk_coef(o, st, j) * Fst[(j * NSP + s) * 64];
+            r[s] = yst[s * 64] + hs * a;                   // the (sub)step start
+          }
           it = 0;
           setups = 0;
-          setup = true;
-          continue;
         }
-        // F_{st-1} = (z - r)/hg (ARKODE's implicit-stage derivative); the next stage's r
-#pragma unroll
-        for (int s = 0; s < NSP; s++) Fst[((st - 1) * NSP + s) * 64] = (z[s] - r[s]) / hg;
-#pragma unroll
-        for (int s = 0; s < NSP; s++) {
-          double a = 0.0;
-#pragma unroll
-          for (int j = 0; j < MAXSTAGE - 1; j++)
-            if (j < st) a += dirk_coef(o, st, j) * Fst[(j * NSP + s) * 64];
-          r[s] = yst[s * 64] + hs * a;                     // the (sub)step start
-        }
-        it = 0;
-        setups = 0;
       } else if (it == o.max_iters) {
         if (++setups > o.max_setups) {
           if (SUBSTEP) {
@@ -534,10 +542,27 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
               continue;
             }
           }
-          ok = false;
-          break;
+          nfail++;                                         // the step failed
+          step_end = true;
+        } else {
+          setup = true;
         }
+      }
+      if (step_end) {
+        if (++nst >= nsteps) break;
+        // the next step from the state the cell kept (as the next launch would start it)
+#pragma unroll
+        for (int s = 0; s < NSP; s++) { yst[s * 64] = z[s]; r[s] = z[s] + h * 0.0; }
+        Ts0 = Ts;
+        dTs0 = dTs;
+        st = 0;
+        it = 0;
+        setups = 0;
         setup = true;
+        hs = h;
+        hg = h * o.A[0];
+        halv = 0;
+        done = 0;
       }
     }
     {
@@ -550,7 +575,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     atomicAdd(&bs[STAT_ITERS], nit);
     atomicMax(&bs[STAT_MAXIT], nit);
     atomicAdd(&bs[STAT_SETUPS], nsetup);
-    if (!ok) atomicAdd(&bs[STAT_FAILED], 1);
+    if (nfail) atomicAdd(&bs[STAT_FAILED], nfail);
     atomicAdd(&bs[STAT_HIST + (nit < NHIST - 1 ? nit : NHIST - 1)], 1);
     }
     next = __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));
@@ -893,7 +918,7 @@ tord_scatter(long n, const unsigned short *key, int *cnt, int *order) {
 // running offsets | keys | temperatures at y_n).
 struct BatchedPlan {
   void (*kern)(long, ChemConst, ChemPtrsD, DirkDev, double, double *, const double *,
-               const int *, const double2 *, int *, int *);
+               const int *, const double2 *, int *, int *, int);
   int block, wpb;
   long grid, nrow;
   bool sorted;
@@ -940,7 +965,7 @@ static int batched_plan(long n, const DirkDev &o, int grid_waves, int block, boo
 // kernel's finishing ticket).
 static int batched_enqueue(const BatchedPlan &p, ChemState &s, const DirkDev &o, double h,
                            double *y, const double *force, bool reuse_order, char *work,
-                           int *counters, hipStream_t st) {
+                           int *counters, hipStream_t st, int nsteps = 1) {
   const long n = s.n;
   int *rows = (int *)(work);
   int *order = p.sorted ? (int *)(work + p.b_rows) : nullptr;
@@ -964,18 +989,21 @@ static int batched_enqueue(const BatchedPlan &p, ChemState &s, const DirkDev &o,
   const bool sub = o.max_halvings > 0;
   hipLaunchKernelGGL(p.kern, dim3((unsigned)p.grid), dim3(p.block), lds, st, n, s.cc,
                      ptrs_d(s), o, h, y, force, (const int *)order,
-                     sub ? nullptr : (const double2 *)tdt, counters, rows);
+                     sub ? nullptr : (const double2 *)tdt, counters, rows, nsteps);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
-                        int grid_waves, int block, const double *force, bool reuse_order) {
+                        int grid_waves, int block, const double *force, bool reuse_order,
+                        int nsteps) {
+  if (nsteps < 1) return -1;
   BatchedPlan p;
   if (batched_plan(s.n, o, grid_waves, block, force != nullptr, p)) return -1;
   const void *work0 = s.work;
   if (ensure_work(s, p.bytes())) return -1;
   if (s.work != work0) reuse_order = false;                // a new buffer holds no order
-  return batched_enqueue(p, s, o, h, y, force, reuse_order, (char *)s.work, s.counters, st);
+  return batched_enqueue(p, s, o, h, y, force, reuse_order, (char *)s.work, s.counters, st,
+                         nsteps);
 }
 
 // ------------------------------------------------------------------------------------

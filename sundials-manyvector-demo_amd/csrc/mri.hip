@@ -9,20 +9,25 @@
 // reference's fslow, which also overwrites Y_{i-1}'s total energy), the constant forcing
 // r_i = sum_{j<i} ((a_ij - a_{i-1,j}) / (c_i - c_{i-1})) F_j, then fixed fast steps over
 // (c_i - c_{i-1}) h, as many as the fast step bound hmax asks (mri_fast_steps): the fluid
-// fields, which have no fast dynamics, advance by dt * r_i; the chemistry by one forced
-// step of the batched per-cell DIRK kernel.
+// fields, which have no fast dynamics, advance by dt * r_i per step; the chemistry by
+// forced steps of the batched per-cell DIRK kernel. The forcing is constant over the
+// stage and the cells are independent, so one launch of each runs all the stage's fast
+// steps: every fluid value takes its nfast additions in turn, every chemistry cell its
+// nfast steps back to back with its state in registers (the same results as one launch
+// per fast step).
 //
 // Everything stays on the device: the state and the three slow derivatives live in
 // contiguous ManyVector blocks ([5 fields][chem]); the first slow step runs eagerly (it
 // allocates the DIRK scratch), then one slow step -- three fslow evaluations, the forcing
-// combinations and 3*nfast fast steps -- is captured as one hipGraph and replayed. fe()'s
-// legal-state bits and the failed-cell counts of every fast step accumulate on the device
-// and are read back once.
+// combinations and the three stages' fast steps -- is captured as one hipGraph and
+// replayed. fe()'s legal-state bits and the failed fast cell steps accumulate on the
+// device and are read back once.
 //
 // Forcing units. The reference's fslow returns the chemistry rates in physical units (the
 // advection of the unscaled species) and the total-energy rate, in code units, in the gas
 // energy slot, and MRIStep adds them unchanged to the fast stepper's Dengo-scaled
-// variables. That is reproduced by default. With `consistent` the
+// variables (multirate_chem_hydro_main.cpp:996-1083 with :922-946). That is reproduced by
+// default. With `consistent` the
 // chemistry part of each slow RHS is first converted to the scaled variables the fast
 // stepper integrates: species rates divided by their scale, the gas-energy slot times
 // EnergyUnits (fslow sets et = ge / EnergyUnits) divided by the ge scale.
@@ -40,6 +45,19 @@ constexpr double KW_A[4][4] = {{0.0, 0.0, 0.0, 0.0},
                                {1.0 / 6.0, 3.0 / 10.0, 8.0 / 15.0, 0.0}};
 
 __global__ void mri_add_int(const int *src, int *dst) { *dst += *src; }
+
+// nsteps fast steps of the fluid fields: y += dt r, one addition per step in turn (the
+// values of nsteps linear combinations 1 y + dt r)
+__global__ void __launch_bounds__(256)
+mri_fluid_steps(long len, double *y, const double *r, double dt, int nsteps) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const double dr = dt * r[i];
+  double v = y[i];
+  for (int k = 0; k < nsteps; k++) v = 1.0 * v + dr;
+  y[i] = v;
+}
 __global__ void mri_zero(unsigned *flag, int *nfail) {
   *flag = 0u;
   *nfail = 0;
@@ -101,17 +119,14 @@ int enqueue_slow_step(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_
     // fast steps
     const int nfast = mri_fast_steps(i, h, hmax);
     const double dt = dc * h / nfast;
-    const double c2[2] = {1.0, dt};
-    const double *X2[2] = {m.Y, m.R};
-    for (int k = 0; k < nfast; k++) {
-      if (linear_combination_launch(2, c2, X2, m.Y, 5 * n, st)) return -1;   // fluid
-      // the cells' temperature order is taken at the stage's first fast step and kept for
-      // the others (temperatures move little over one slow stage)
-      if (dirk_batched_launch(ch, o, dt, Y[5], st, waves, block, m.R + 5 * n, k > 0))
-        return -1;
-      hipLaunchKernelGGL(mri_add_int, dim3(1), dim3(1), 0, st,
-                         (const int *)(ch.counters + 1 + STAT_FAILED), m.nfail);
-    }
+    hipLaunchKernelGGL(mri_fluid_steps, dim3((unsigned)((5 * n + 255) / 256)), dim3(256), 0,
+                       st, 5 * n, m.Y, (const double *)m.R, dt, nfast);
+    // the cells' temperature order is taken at the stage's start and kept for all its fast
+    // steps (temperatures move little over one slow stage)
+    if (dirk_batched_launch(ch, o, dt, Y[5], st, waves, block, m.R + 5 * n, false, nfast))
+      return -1;
+    hipLaunchKernelGGL(mri_add_int, dim3(1), dim3(1), 0, st,
+                       (const int *)(ch.counters + 1 + STAT_FAILED), m.nfail);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -28,10 +28,11 @@ struct DirkDev {
 
 // force (nullable): constant forcing added to the scaled RHS, [cell][10]. reuse_order:
 // keep the cell order of the previous launch (same cells, work buffer not reallocated
-// since) instead of sorting again.
+// since) instead of sorting again. nsteps: steps of size h run per cell in this one
+// launch (the same states as nsteps launches; statistics per cell over all its steps).
 int dirk_batched_launch(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st,
                         int grid_waves, int block, const double *force = nullptr,
-                        bool reuse_order = false);
+                        bool reuse_order = false, int nsteps = 1);
 // The same step with its work buffer and counters in the library's per-GPU scratch array
 // instead of the context's buffer (no device allocation); at most NOALLOC_MAX_CELLS cells
 // and NOALLOC_MAX_ROWS blocks, else MVD_BAD_ARGUMENT. *counters_out: the counters it used
