@@ -100,8 +100,9 @@ int mvd_RankLocalLS_Free(SUNLinearSolver S);
  * the inner ARKStep. Evolve(t0, tout, y): ceil((tout - t0)/hmax_fast) equal fixed steps of
  * y' = ffast(t, y); the MRI forcing must be constant over the step (nforcing == 1, as for
  * MRIStep's MIS methods such as the default Knoth-Wolke 3; else -1): the fluid subvectors
- * take y_q += dt F_q per step, the chemistry one forced batched DIRK step (options fast,
- * rhs_scale = TimeUnits, per-cell step rejection per fast.max_halvings). Returns 0, the
+ * take y_q += dt F_q per step (mvd_nvec_axpy_steps), the chemistry forced batched DIRK
+ * steps (options fast, rhs_scale = TimeUnits, per-cell step rejection per
+ * fast.max_halvings), all of them in one mvd_chem_dirk_steps launch. Returns 0, the
  * number (> 0, recoverable) of failed fast cell steps, or < 0. FullRhs: ffast with the
  * forcing disabled (any mode). Reset(tR, yR): the stepper's time becomes tR; it keeps no
  * other history (the reference's ARKStepReset clears the inner integrator's step history

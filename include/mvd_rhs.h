@@ -156,6 +156,13 @@ int mvd_nvec_linear_combination(mvd_context *ctx, int nv, const double *c,
                                 const double *const *X, double *z, long len);
 /* N_VConst: z[0..len) = c (device array). */
 int mvd_nvec_const(mvd_context *ctx, double c, double *z, long len);
+/* nsteps fixed explicit steps of y' = x with x constant (the fluid subvectors of the
+ * multirate inner stepper, which have no fast dynamics): y = 1 y + a x, nsteps times, each
+ * value's additions in turn (bitwise nsteps mvd_nvec_linear_combination calls with
+ * c = {1, a}, X = {y, x}), in one launch. No counterpart in the reference; for the inner
+ * stepper's Evolve (multirate_chem_hydro_main.cpp:1130-1160). */
+int mvd_nvec_axpy_steps(mvd_context *ctx, int nsteps, double a, const double *x, double *y,
+                        long len);
 /* N_VWrmsNorm of a ManyVector of nsub (<= 8) subvectors: sqrt(sum_q sum_i (x_q[i] w_q[i])^2 /
  * sum_q len[q]), deterministic; x, w host arrays of device pointers, len host array. */
 int mvd_nvec_wrms_norm(mvd_context *ctx, int nsub, const double *const *x,
@@ -269,6 +276,18 @@ int mvd_chem_dirk_step_noalloc(mvd_context *ctx, double t, double h, const mvd_d
  * multirate_chem_hydro_main.cpp:922-946), batched kernel only. */
 int mvd_chem_dirk_step_forced(mvd_context *ctx, double t, double h, const mvd_dirk_opts *o,
                               double *y, const double *force, mvd_dirk_stats *stats);
+/* nsteps >= 1 fixed steps of size h of y' = rhs_scale*f(y) (+ force, nullable, constant over
+ * all of them), batched kernel only, in one launch: each cell runs its steps back to back
+ * with its state in registers, every step exactly as one mvd_chem_dirk_step(_forced) call
+ * would run it (a failed step counts and the next starts from the state it kept), so y and
+ * the temperatures end bitwise as after nsteps such calls. The inner stepper's Evolve of the
+ * multirate driver (multirate_chem_hydro_main.cpp:1130-1160: fixed steps within a slow
+ * stage, one forcing). Returns the number of failed cell steps over all steps, or an error
+ * code; stats are per cell over all its steps (newton_iters, setups and the histogram of
+ * per-cell totals, failed_cells = failed cell steps). */
+int mvd_chem_dirk_steps(mvd_context *ctx, double t, double h, int nsteps,
+                        const mvd_dirk_opts *o, double *y, const double *force,
+                        mvd_dirk_stats *stats);
 /* Multirate evolution of the reacting flow, device-resident (configuration 5; the driver
  * multirate_chem_hydro_main.cpp:529-555 with MRIStep's default third-order coupling, MIS
  * with Knoth-Wolke 3, and a fixed slow step): nslow slow steps of size h on w (the six

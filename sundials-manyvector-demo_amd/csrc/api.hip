@@ -468,6 +468,15 @@ int mvd_nvec_const(mvd_context *ctx, double c, double *z, long len) {
   return MVD_SUCCESS;
 }
 
+int mvd_nvec_axpy_steps(mvd_context *ctx, int nsteps, double a, const double *x, double *y,
+                        long len) {
+  if (!ctx || nsteps < 0 || len < 0 || (len > 0 && (!x || !y))) return MVD_BAD_ARGUMENT;
+  if (mvd::axpy_steps_launch(nsteps, a, x, y, len, ctx->stream) ||
+      !dev_ok(hipStreamSynchronize(ctx->stream)))
+    return MVD_DEVICE_ERROR;
+  return MVD_SUCCESS;
+}
+
 int mvd_nvec_wrms_norm(mvd_context *ctx, int nsub, const double *const *x,
                        const double *const *w, const long *len, double *out) {
   if (!ctx || nsub < 1 || nsub > mvd::WRMS_MAXSUB || !x || !w || !len || !out)
@@ -584,8 +593,10 @@ int default_waves(mvd_context *ctx) {
 }  // namespace
 
 static int dirk_step_common(mvd_context *ctx, double h, const mvd_dirk_opts *o, double *y,
-                            int plain, const double *force, mvd_dirk_stats *stats) {
-  if (!ctx || !y || !(h > 0)) return MVD_BAD_ARGUMENT;
+                            int plain, const double *force, mvd_dirk_stats *stats,
+                            int nsteps = 1) {
+  if (!ctx || !y || !(h > 0) || nsteps < 1) return MVD_BAD_ARGUMENT;
+  if (plain && nsteps != 1) return MVD_UNSUPPORTED;
   if (!ctx->chem.ready) return MVD_NOT_READY;
   if (plain && force) return MVD_UNSUPPORTED;
   mvd::DirkDev d;
@@ -597,7 +608,7 @@ static int dirk_step_common(mvd_context *ctx, double h, const mvd_dirk_opts *o, 
   (void)hipEventRecord(e0, ctx->stream);
   rc = plain ? mvd::dirk_plain_run(ctx->chem, d, h, y, ctx->stream)
              : mvd::dirk_batched_launch(ctx->chem, d, h, y, ctx->stream, default_waves(ctx),
-                                        ctx->dirk_block, force);
+                                        ctx->dirk_block, force, false, nsteps);
   (void)hipEventRecord(e1, ctx->stream);
   if (rc) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return MVD_DEVICE_ERROR; }
   int *hc = ctx->chem.host_counters;
@@ -661,6 +672,12 @@ int mvd_chem_dirk_step_forced(mvd_context *ctx, double, double h, const mvd_dirk
                               double *y, const double *force, mvd_dirk_stats *stats) {
   if (!force) return MVD_BAD_ARGUMENT;
   return dirk_step_common(ctx, h, o, y, 0, force, stats);
+}
+
+int mvd_chem_dirk_steps(mvd_context *ctx, double, double h, int nsteps,
+                        const mvd_dirk_opts *o, double *y, const double *force,
+                        mvd_dirk_stats *stats) {
+  return dirk_step_common(ctx, h, o, y, 0, force, stats, nsteps);
 }
 
 int mvd_mri_evolve(mvd_context *ctx, double, double h, int nslow, double hmax_fast,

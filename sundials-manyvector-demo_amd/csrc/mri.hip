@@ -46,18 +46,6 @@ constexpr double KW_A[4][4] = {{0.0, 0.0, 0.0, 0.0},
 
 __global__ void mri_add_int(const int *src, int *dst) { *dst += *src; }
 
-// nsteps fast steps of the fluid fields: y += dt r, one addition per step in turn (the
-// values of nsteps linear combinations 1 y + dt r)
-__global__ void __launch_bounds__(256)
-mri_fluid_steps(long len, double *y, const double *r, double dt, int nsteps) {
-#pragma clang fp contract(off)
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
-  const double dr = dt * r[i];
-  double v = y[i];
-  for (int k = 0; k < nsteps; k++) v = 1.0 * v + dr;
-  y[i] = v;
-}
 __global__ void mri_zero(unsigned *flag, int *nfail) {
   *flag = 0u;
   *nfail = 0;
@@ -119,8 +107,7 @@ int enqueue_slow_step(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_
     // fast steps
     const int nfast = mri_fast_steps(i, h, hmax);
     const double dt = dc * h / nfast;
-    hipLaunchKernelGGL(mri_fluid_steps, dim3((unsigned)((5 * n + 255) / 256)), dim3(256), 0,
-                       st, 5 * n, m.Y, (const double *)m.R, dt, nfast);
+    if (axpy_steps_launch(nfast, dt, m.R, m.Y, 5 * n, st)) return -1;   // fluid
     // the cells' temperature order is taken at the stage's start and kept for all its fast
     // steps (temperatures move little over one slow stage)
     if (dirk_batched_launch(ch, o, dt, Y[5], st, waves, block, m.R + 5 * n, false, nfast))

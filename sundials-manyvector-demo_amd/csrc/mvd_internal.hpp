@@ -91,6 +91,9 @@ int exchange_launch(const Grid &g, const Fields &f, double *const *recv, hipStre
 int pack1d_launch(const Grid &g, const Fields &f, int d, long ncell, const int *ijk,
                   double *out, int *scratch, hipStream_t st, int *host_bad);
 
+// y = 1 y + a x, nsteps times in turn (one launch)
+int axpy_steps_launch(int nsteps, double a, const double *x, double *y, long len,
+                      hipStream_t st);
 int linear_combination_launch(int nv, const double *c, const double *const *X, double *z,
                               long len, hipStream_t st);
 int wrms_norm_launch(int nsub, const double *const *x, const double *const *w, const long *len,

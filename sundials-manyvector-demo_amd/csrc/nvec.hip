@@ -177,6 +177,27 @@ lin_comb(long len, int nv, LcArgs a, double *z, int add) {
   z[i] = acc;
 }
 
+// y = 1 y + a x, nsteps times, each value's additions in turn: the values of nsteps
+// lin_comb launches with c = {1, a}, X = {y, x}
+__global__ void __launch_bounds__(256)
+axpy_steps(long len, int nsteps, double a, const double *x, double *y) {
+#pragma clang fp contract(off)
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const double ax = a * x[i];
+  double v = y[i];
+  for (int k = 0; k < nsteps; k++) v = 1.0 * v + ax;
+  y[i] = v;
+}
+
+int axpy_steps_launch(int nsteps, double a, const double *x, double *y, long len,
+                      hipStream_t st) {
+  if (len <= 0 || nsteps <= 0) return 0;
+  hipLaunchKernelGGL(axpy_steps, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, len,
+                     nsteps, a, x, y);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
 int linear_combination_launch(int nv, const double *c, const double *const *X, double *z,
                               long len, hipStream_t st) {
   if (len <= 0) return 0;

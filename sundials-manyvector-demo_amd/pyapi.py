@@ -100,6 +100,7 @@ def lib():
         "mvd_chem_jac_scaled": ([vp, C.c_double, C.c_double, _P, _P], C.c_int),
         "mvd_chem_lsetup_matrix": ([vp, _P], C.c_int),
         "mvd_nvec_const": ([vp, C.c_double, _P, C.c_long], C.c_int),
+        "mvd_nvec_axpy_steps": ([vp, C.c_int, C.c_double, _P, _P, C.c_long], C.c_int),
         "mvd_mri_fast_steps": ([C.c_int, C.c_double, C.c_double], C.c_int),
         "mvd_postprocess_step": ([vp, C.c_double, _PP], C.c_int),
         "mvd_chem_state": ([vp, C.c_int], _P),
@@ -127,6 +128,8 @@ def lib():
         "mvd_chem_dirk_get_shape": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "mvd_chem_dirk_step_forced": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, _P,
                                        C.POINTER(DirkStats)], C.c_int),
+        "mvd_chem_dirk_steps": ([vp, C.c_double, C.c_double, C.c_int, C.POINTER(DirkOpts), _P,
+                                 _P, C.POINTER(DirkStats)], C.c_int),
         "mvd_mri_evolve": ([vp, C.c_double, C.c_double, C.c_int, C.c_double,
                             C.POINTER(DirkOpts), C.c_int, C.POINTER(_P)], C.c_int),
         "mvd_ark_ark324": ([C.POINTER(ArkTable)], C.c_int),
@@ -499,6 +502,23 @@ class Context:
         if rc < 0:
             _check("mvd_chem_dirk_step_forced", rc)
         return rc
+
+    def dirk_steps(self, y, h, nsteps, force=None, opts=None, t=0.0):
+        """mvd_chem_dirk_steps: nsteps steps of size h per cell in one launch (force
+        optional). Returns (failed cell steps, stats)."""
+        o = opts if opts is not None else sdirk3()
+        st = DirkStats()
+        rc = lib().mvd_chem_dirk_steps(self.h, t, h, int(nsteps), C.byref(o), dptr(y),
+                                       dptr(force) if force is not None else None,
+                                       C.byref(st))
+        if rc < 0:
+            _check("mvd_chem_dirk_steps", rc)
+        return rc, st.as_dict()
+
+    def nvec_axpy_steps(self, nsteps, a, x, y):
+        """mvd_nvec_axpy_steps: y = 1 y + a x, nsteps times in turn."""
+        _check("mvd_nvec_axpy_steps", lib().mvd_nvec_axpy_steps(self.h, int(nsteps), a,
+                                                                dptr(x), dptr(y), y.numel()))
 
     def dirk_adjoint(self, y, h, nsteps, ckpt_every, lamT, lam0, opts=None, t=0.0):
         """Checkpointed adjoint of nsteps DIRK steps: y advanced in place, lam0 filled with

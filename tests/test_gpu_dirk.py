@@ -216,3 +216,64 @@ def test_dirk_set_shape_rejects_bad_shapes(mvd, torch_gpu):
             ctx.dirk_set_shape(block, waves)
         assert e.value.code == mvd.MVD_BAD_ARGUMENT
     ctx.close()
+
+
+@pytest.mark.parametrize("variant", ["plain", "forced", "substep", "failing"])
+def test_dirk_steps_equal_repeated_steps(mvd, torch_gpu, variant):
+    """mvd_chem_dirk_steps runs nsteps steps per cell in one launch, each cell's steps back
+    to back: states and temperatures equal nsteps single-step calls bit for bit, and the
+    failed count is the sum of the single calls' (failed cell steps). `failing` takes a
+    Newton budget so small that cells fail and carry their kept state into the next step;
+    `substep` rejects and halves per cell; `forced` adds a constant forcing."""
+    torch = torch_gpu
+    n, seed, nsteps = 6000, 41, 5
+    print(f"[dirk steps] n={n} variant={variant} nsteps={nsteps}", flush=True)
+    rng = np.random.default_rng(seed)
+    force = torch.from_numpy(rng.standard_normal(10 * n) * 1e-5).to("cuda") \
+        if variant == "forced" else None
+    kw = {"plain": {}, "forced": {}, "substep": dict(max_halvings=4, max_iters=2, max_setups=0),
+          "failing": dict(max_iters=1, max_setups=0)}[variant]
+    h = {"plain": 300.0, "forced": 300.0, "substep": 2e4, "failing": 1e5}[variant]
+    y, _ = problems.chem_cells(n, seed=seed)
+    res = []
+    for fused in (False, True):
+        ctx = mvd.Context(mvd.make_params(4, 4, 4, 10))
+        ctx.chem_setup(orc.load_tables(), n)
+        yd = torch.from_numpy(y.copy()).to("cuda")
+        ctx.chem_prepare(yd)
+        o = mvd.sdirk3(**kw)
+        if fused:
+            failed, st = ctx.dirk_steps(yd, h, nsteps, force, o)
+        else:
+            failed = 0
+            for _ in range(nsteps):
+                if force is not None:
+                    failed += ctx.dirk_step_forced(yd, h, force, o)
+                else:
+                    failed += ctx.dirk_step(yd, h, o)[0]
+        res.append((failed, yd.cpu().numpy(), ctx.chem_state("Ts", torch).cpu().numpy(),
+                    ctx.chem_state("dTs_ge", torch).cpu().numpy()))
+        ctx.close()
+    (f0, y0, T0, d0), (f1, y1, T1, d1) = res
+    if variant == "failing":
+        assert f0 > 0
+    else:
+        assert np.all(np.isfinite(y0))
+    assert f1 == f0
+    assert np.array_equal(y1, y0) and np.array_equal(T1, T0) and np.array_equal(d1, d0)
+
+
+def test_nvec_axpy_steps_equals_repeated_combinations(mvd, torch_gpu):
+    torch = torch_gpu
+    ctx = mvd.Context(mvd.make_params(4, 4, 4, 10))
+    rng = np.random.default_rng(7)
+    x = torch.from_numpy(rng.standard_normal(100003)).to("cuda")
+    y0 = torch.from_numpy(rng.standard_normal(100003) * 1e3).to("cuda")
+    a, k = 1.7e-3, 37
+    ya = y0.clone()
+    ctx.nvec_axpy_steps(k, a, x, ya)
+    yb = y0.clone()
+    for _ in range(k):
+        ctx.linear_combination([1.0, a], [yb, x], yb)
+    assert torch.equal(ya, yb)
+    ctx.close()
