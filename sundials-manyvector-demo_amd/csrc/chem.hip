@@ -13,6 +13,15 @@ __global__ void __launch_bounds__(256) chem_init(long n, double *Ts, double *dTs
   if (i < n) { Ts[i] = 1000.0; dTs[i] = 0.0; }
 }
 
+// The bin-major pair copy of the rate tables (ChemConst::tabp).
+__global__ void __launch_bounds__(256) chem_pair_tables(const double *tab, double2 *tabp,
+                                                        int nbins) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)nbins * NRATE) return;
+  const int b = (int)(e / NRATE), r = (int)(e % NRATE);
+  tabp[e] = make_double2(tab[r * TLEN + b], tab[r * TLEN + b + 1]);
+}
+
 __global__ void __launch_bounds__(256) chem_init_scale(long m, double *scale, double *iscale) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) { scale[i] = 1.0; iscale[i] = 1.0; }
@@ -71,7 +80,7 @@ chem_rhs_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *ydot
   p.Ts[i] = Ts;
   p.dTs[i] = dTs;
   double k[NRATE];
-  chem_rates_only(cc, Ts, k);
+  chem_rates_only_pairs<0, NRATE>(cc, Ts, k);
   const CellAux ax{p.md[i], p.imd[i], p.cie[i], p.h2o[i]};
   double f[NSP];
   chem_rhs_physical(y, k, Ts, cc.z, ax, f);
@@ -85,7 +94,10 @@ chem_rhs_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *ydot
 
 // Vector-Jacobian product of calculate_rhs_cvklu in the scaled variables:
 // ybar = (d ydot / d y)^T v, with the temperature a function of y (solved from the warm start
-// like the RHS, which it also updates); optionally ydot itself.
+// like the RHS, which it also updates); optionally ydot itself. The rates come from the
+// rate-major tables, all of them in one batch of loads: with one wave per SIMD the kernel
+// is bound by the latency of these gathers, and for cells of unrelated temperatures this
+// form measured faster than the pair table.
 __global__ void __launch_bounds__(256)
 chem_vjp_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, const double *v,
                 double *ybar, double *ydot) {
@@ -115,7 +127,7 @@ chem_vjp_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, const double
   for (int s = 0; s < NSP; s++) vp[s] = v[NSP * i + s] * p.iscale[NSP * i + s];
   vp[9] *= ax.inv_mdensity;
   const bool clamped = !(Ts > cc.tmin && Ts < cc.tmax);
-  chem_vjp_physical(cc, y, Ts, clamped, ax, vp, yb);
+  chem_vjp_physical<false>(cc, y, Ts, clamped, ax, vp, yb);
 #pragma unroll
   for (int s = 0; s < NSP; s++) ybar[NSP * i + s] = yb[s] * p.scale[NSP * i + s];
 }
@@ -156,7 +168,7 @@ chem_jac_kernel(long n, ChemConst cc, ChemPtrs p, const double *yd, double *J, d
   }
   const double T = p.Ts[i];
   double k[NRATE], dk[NRATE];
-  chem_rates(cc, T, k, dk);
+  chem_rates_pairs<0, NRATE>(cc, T, k, dk);
   const CellAux ax{p.md[i], p.imd[i], p.cie[i], p.h2o[i]};
   double Jp[NSP][NSP];
   chem_jac_physical(y, k, dk, T, cc.z, p.dTs[i], ax, Jp);
@@ -179,6 +191,7 @@ void chem_free(ChemState &s) {
                     s.cie_oda, s.h2_oda, s.work};
   for (double *a : arrs)
     if (a) (void)hipFree(a);
+  if (s.tabp) (void)hipFree(s.tabp);
   if (s.counters) (void)hipFree(s.counters);
   if (s.host_counters) (void)hipHostFree(s.host_counters);
   s = ChemState{};
@@ -189,6 +202,7 @@ int chem_setup(ChemState &s, const double *tables_host, long n, double z, hipStr
   chem_free(s);
   s.gen = gen + 1;
   bool ok = hipMalloc(&s.tab, sizeof(double) * NTAB * TLEN) == hipSuccess &&
+            hipMalloc(&s.tabp, sizeof(double2) * (TLEN - 1) * NRATE) == hipSuccess &&
             hipMalloc(&s.scale, sizeof(double) * NSP * n) == hipSuccess &&
             hipMalloc(&s.inv_scale, sizeof(double) * NSP * n) == hipSuccess &&
             hipMalloc(&s.Ts, sizeof(double) * n) == hipSuccess &&
@@ -213,6 +227,9 @@ int chem_setup(ChemState &s, const double *tables_host, long n, double z, hipStr
   s.cc.dbin = (std::log(s.cc.tmax) - std::log(s.cc.tmin)) / s.cc.nbins;
   s.cc.idbin = (double)(1.0L / (long double)s.cc.dbin);   // long-double quotient as the reference
   s.cc.lb = std::log(s.cc.tmin);
+  s.cc.tabp = s.tabp;
+  hipLaunchKernelGGL(chem_pair_tables, dim3(nb((long)s.cc.nbins * NRATE)), dim3(256), 0, st,
+                     (const double *)s.tab, s.tabp, s.cc.nbins);
   s.cc.z = z;
   hipLaunchKernelGGL(chem_init, dim3(nb(n)), dim3(256), 0, st, n, s.Ts, s.dTs_ge);
   hipLaunchKernelGGL(chem_init_scale, dim3(nb(n * NSP)), dim3(256), 0, st, n * NSP, s.scale,

@@ -44,6 +44,8 @@ __device__ inline CellIn load_cell(const ChemPtrsD &p, long i) {
 // calculate_rhs_cvklu for one cell in scaled variables, times srhs; updates Ts/dTs. The
 // temperature solve reads the gamma tables from gt (global memory or an LDS copy).
 // With solve_t false, Ts/dTs already hold the solution at y and the solve is skipped.
+// PAIRS: the rates come from the bin-major pair table (same values).
+template <bool PAIRS = false>
 __device__ inline void cell_rhs(const ChemConst &cc, const GammaTabs &gt, const CellIn &c,
                                 double &Ts, double &dTs, const double (&y)[NSP], double srhs,
                                 double (&f)[NSP], bool solve_t = true) {
@@ -57,10 +59,12 @@ __device__ inline void cell_rhs(const ChemConst &cc, const GammaTabs &gt, const 
   double k[NRATE];
 #pragma unroll
   for (int r = 0; r < NRATE; r++) k[r] = 0.0;
-  chem_rates_only_range<0, C_BREM>(cc, Ts, k);
+  if (PAIRS) chem_rates_only_pairs<0, C_BREM>(cc, Ts, k);
+  else chem_rates_only_range<0, C_BREM>(cc, Ts, k);
   chem_rhs_species(ya, k, f);
   __asm__ volatile("" ::: "memory");
-  chem_rates_only_range<C_BREM, NRATE>(cc, Ts, k);
+  if (PAIRS) chem_rates_only_pairs<C_BREM, NRATE>(cc, Ts, k);
+  else chem_rates_only_range<C_BREM, NRATE>(cc, Ts, k);
   f[9] = chem_rhs_energy(ya, k, Ts, cc.z, c.ax);
 #pragma unroll
   for (int s = 0; s < NSP; s++) f[s] *= c.is(s);
@@ -69,9 +73,10 @@ __device__ inline void cell_rhs(const ChemConst &cc, const GammaTabs &gt, const 
   for (int s = 0; s < NSP; s++) f[s] *= srhs;
 }
 
+template <bool PAIRS = false>
 __device__ inline void cell_rhs(const ChemConst &cc, const CellIn &c, double &Ts, double &dTs,
                                 const double (&y)[NSP], double srhs, double (&f)[NSP]) {
-  cell_rhs(cc, gamma_tabs(cc), c, Ts, dTs, y, srhs, f);
+  cell_rhs<PAIRS>(cc, gamma_tabs(cc), c, Ts, dTs, y, srhs, f);
 }
 
 // A cell's 10x10 matrix in memory: element (r, c) at b[r*rs + c*cs]. The batched solver
@@ -290,14 +295,14 @@ __device__ inline void cell_matrix_regs(const ChemConst &cc, const CellIn &c, do
   for (int r = 0; r < NSP; r++)
 #pragma unroll
     for (int q = 0; q < NSP; q++) M[r][q] = 0.0;
-  chem_rates_range<0, C_BREM>(cc, Ts, k, dk);
+  chem_rates_pairs<0, C_BREM>(cc, Ts, k, dk);
   chem_jac_species(ya, k, dk, dTs, M);
 #pragma unroll
   for (int r = 0; r < NSP - 1; r++)
 #pragma unroll
     for (int q = 0; q < NSP; q++)
       M[r][q] = (r == q ? 1.0 : 0.0) - hg * (srhs * (M[r][q] * (c.is(r) * c.s(q))));
-  chem_rates_range<C_BREM, NRATE>(cc, Ts, k, dk);
+  chem_rates_pairs<C_BREM, NRATE>(cc, Ts, k, dk);
   chem_jac_energy(ya, k, dk, Ts, cc.z, dTs, c.ax, M);
 #pragma unroll
   for (int q = 0; q < NSP; q++)
@@ -466,7 +471,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
                      CellAux{cl[(2 * NSP) * 64], cl[(2 * NSP + 1) * 64], cl[(2 * NSP + 2) * 64],
                              cl[(2 * NSP + 3) * 64]}};
       double f[NSP];
-      cell_rhs(cc, gt, c, Ts, dTs, z, o.rhs_scale, f, !solved);
+      cell_rhs<true>(cc, gt, c, Ts, dTs, z, o.rhs_scale, f, !solved);
       solved = false;
       if (setup) {
         cell_matrix_regs(cc, c, Ts, dTs, z, hg, o.rhs_scale, lu);
@@ -845,7 +850,7 @@ tord_count(long n, ChemConst cc, ChemPtrsD p, const double *y, const double *for
     double kr[NRATE], f[NSP];
 #pragma unroll
     for (int r = 0; r < NRATE; r++) kr[r] = 0.0;
-    chem_rates_only_range<0, C_BREM>(cc, T, kr);
+    chem_rates_only_pairs<0, C_BREM>(cc, T, kr);
     chem_rhs_species(ya, kr, f);
     double m = 0.0;
 #pragma unroll
@@ -1222,7 +1227,7 @@ __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, doub
   double yn[NSP], z[NSP], f[NSP], F[MAXSTAGE][NSP];
 #pragma unroll
   for (int s = 0; s < NSP; s++) yn[s] = z[s] = y[s];
-  cell_rhs(cc, c, Ts, dTs, yn, o.rhs_scale, f);      // refresh Ts at y_n for J
+  cell_rhs<true>(cc, c, Ts, dTs, yn, o.rhs_scale, f);      // refresh Ts at y_n for J
   double a[NSP][NSP];
   unsigned long long pv = 0;
   bool ok = true;
@@ -1246,7 +1251,7 @@ __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, doub
       bool conv = false;
       while (!conv) {
         for (int it = 0; it < o.max_iters; it++) {
-          cell_rhs(cc, c, Ts, dTs, z, o.rhs_scale, f);
+          cell_rhs<true>(cc, c, Ts, dTs, z, o.rhs_scale, f);
           double d[NSP];
 #pragma unroll
           for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
@@ -1257,7 +1262,7 @@ __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, doub
         }
         if (conv) break;
         if (++setups > o.max_setups) { ok = false; break; }
-        cell_rhs(cc, c, Ts, dTs, z, o.rhs_scale, f);   // temperature at the iterate
+        cell_rhs<true>(cc, c, Ts, dTs, z, o.rhs_scale, f);   // temperature at the iterate
         cell_matrix_regs(cc, c, Ts, dTs, z, hg, o.rhs_scale, a);
         pv = lu_factor_regs(a);
       }
@@ -1319,7 +1324,7 @@ __device__ inline void stage_adjoint_matrix(const ChemConst &cc, const CellIn &c
   chem_temperature(cc, ya, T, dT);
   const bool clamped = !(T > cc.tmin && T < cc.tmax);
   ChemVjpPre P;                  // rates, gas-energy tangents, temperature path: once
-  chem_vjp_prepare(cc, ya, T, clamped, c.ax, P);
+  chem_vjp_prepare<true>(cc, ya, T, clamped, c.ax, P);
   double acc[NSP];
 #pragma unroll
   for (int q = 0; q < NSP; q++) acc[q] = 0.0;

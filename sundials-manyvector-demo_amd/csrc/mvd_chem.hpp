@@ -31,6 +31,11 @@ enum {
 // Table grid constants (cvklu_setup_data, raja_primordial_network.cpp:191-195)
 struct ChemConst {
   const double *tab;    // device, NTAB x TLEN
+  // the NRATE rate tables again, bin-major with both interpolation ends of a bin side by
+  // side: tabp[b * NRATE + r] = (tab[r][b], tab[r][b + 1]), b < nbins. One cell's rates at
+  // one temperature are then NRATE consecutive 16-byte loads (about six cache lines)
+  // instead of 2 NRATE loads from NRATE tables.
+  const double2 *tabp;
   double dbin, idbin, lb, tmin, tmax, z;
   int nbins;
 };
@@ -39,6 +44,7 @@ struct ChemState {
   bool ready = false;
   long n = 0;
   double *tab = nullptr;
+  double2 *tabp = nullptr;
   ChemConst cc{};
   double *scale = nullptr, *inv_scale = nullptr, *Ts = nullptr, *dTs_ge = nullptr;
   double *mdensity = nullptr, *inv_mdensity = nullptr, *cie_oda = nullptr, *h2_oda = nullptr;
@@ -260,6 +266,43 @@ __device__ inline void chem_rates_only_range(const ChemConst &c, double T, doubl
   for (int r = R0; r < R1; r++) {
     const double a0 = c.tab[r * TLEN + b], a1 = c.tab[r * TLEN + b + 1];
     k[r] = a0 + Tdef * (a1 - a0);
+  }
+}
+
+// chem_rates_range and chem_rates_only_range from the bin-major pair table: the same
+// values (the same table entries and arithmetic), fewer and denser loads.
+template <int R0, int R1>
+__device__ inline void chem_rates_pairs(const ChemConst &c, double T, double (&k)[NRATE],
+                                        double (&dk)[NRATE]) {
+#pragma clang fp contract(off)
+  const double lT = log(T);
+  const int b = chem_bin(c, lT);
+  const double t1 = c.lb + (double)b * c.dbin;
+  const double t2 = c.lb + (double)(b + 1) * c.dbin;
+  const double Tdef = (lT - t1) / (t2 - t1);
+  const double Tfactor = (1.0 / T) / (t2 - t1);
+  const double2 *row = c.tabp + (long)b * NRATE;
+#pragma unroll
+  for (int r = R0; r < R1; r++) {
+    const double2 a = row[r];
+    k[r] = a.x + Tdef * (a.y - a.x);
+    dk[r] = (a.y - a.x) * Tfactor;
+  }
+}
+
+template <int R0, int R1>
+__device__ inline void chem_rates_only_pairs(const ChemConst &c, double T, double (&k)[NRATE]) {
+#pragma clang fp contract(off)
+  const double lT = log(T);
+  const int b = chem_bin(c, lT);
+  const double t1 = c.lb + (double)b * c.dbin;
+  const double t2 = c.lb + (double)(b + 1) * c.dbin;
+  const double Tdef = (lT - t1) / (t2 - t1);
+  const double2 *row = c.tabp + (long)b * NRATE;
+#pragma unroll
+  for (int r = R0; r < R1; r++) {
+    const double2 a = row[r];
+    k[r] = a.x + Tdef * (a.y - a.x);
   }
 }
 
@@ -508,11 +551,14 @@ struct ChemVjpPre {
   bool clamped;
 };
 
+// PAIRS: the rates from the bin-major pair table (same values).
+template <bool PAIRS>
 __device__ inline void chem_vjp_prepare(const ChemConst &c, const double (&y)[NSP], double T,
                                         bool clamped, const CellAux &ax, ChemVjpPre &P) {
 #pragma clang fp contract(off)
   double k[NRATE], dk[NRATE];
-  chem_rates(c, T, k, dk);
+  if (PAIRS) chem_rates_pairs<0, NRATE>(c, T, k, dk);
+  else chem_rates(c, T, k, dk);
 #pragma unroll
   for (int j = 0; j < NREACT; j++) {
     P.k[j] = k[j];
@@ -626,11 +672,12 @@ __device__ inline void chem_vjp_row(const ChemVjpPre &P, const double (&y)[NSP],
   chem_vjp_temperature(P, Tb, row);
 }
 
+template <bool PAIRS>
 __device__ inline void chem_vjp_physical(const ChemConst &c, const double (&y)[NSP], double T,
                                          bool clamped, const CellAux &ax,
                                          const double (&vp)[NSP], double (&yab)[NSP]) {
   ChemVjpPre P;
-  chem_vjp_prepare(c, y, T, clamped, ax, P);
+  chem_vjp_prepare<PAIRS>(c, y, T, clamped, ax, P);
   chem_vjp_apply(P, y, vp, yab);
 }
 
