@@ -280,7 +280,9 @@ static inline unsigned stats_blocks(long nrow) {
 // batched path
 // ------------------------------------------------------------------------------------
 
-// M = I - hg * srhs * J into registers (same expressions and order as cell_matrix)
+// M = I - hg * srhs * J into registers (same expressions and order as cell_matrix). The
+// rates come from the rate-major tables here: with the factors held in registers, the pair
+// table's wider loads cost this kernel spills, and it measured faster this way.
 __device__ inline void cell_matrix_regs(const ChemConst &cc, const CellIn &c, double Ts,
                                         double dTs, const double (&y)[NSP], double hg,
                                         double srhs, double (&M)[NSP][NSP]) {
@@ -295,14 +297,14 @@ __device__ inline void cell_matrix_regs(const ChemConst &cc, const CellIn &c, do
   for (int r = 0; r < NSP; r++)
 #pragma unroll
     for (int q = 0; q < NSP; q++) M[r][q] = 0.0;
-  chem_rates_pairs<0, C_BREM>(cc, Ts, k, dk);
+  chem_rates_range<0, C_BREM>(cc, Ts, k, dk);
   chem_jac_species(ya, k, dk, dTs, M);
 #pragma unroll
   for (int r = 0; r < NSP - 1; r++)
 #pragma unroll
     for (int q = 0; q < NSP; q++)
       M[r][q] = (r == q ? 1.0 : 0.0) - hg * (srhs * (M[r][q] * (c.is(r) * c.s(q))));
-  chem_rates_pairs<C_BREM, NRATE>(cc, Ts, k, dk);
+  chem_rates_range<C_BREM, NRATE>(cc, Ts, k, dk);
   chem_jac_energy(ya, k, dk, Ts, cc.z, dTs, c.ax, M);
 #pragma unroll
   for (int q = 0; q < NSP; q++)
