@@ -33,8 +33,8 @@ struct ChemConst {
   const double *tab;    // device, NTAB x TLEN
   // the NRATE rate tables again, bin-major with both interpolation ends of a bin side by
   // side: tabp[b * NRATE + r] = (tab[r][b], tab[r][b + 1]), b < nbins. One cell's rates at
-  // one temperature are then NRATE consecutive 16-byte loads (about six cache lines)
-  // instead of 2 NRATE loads from NRATE tables.
+  // one temperature are then NRATE consecutive 16-byte loads over about six cache lines
+  // instead of one line in each of NRATE tables.
   const double2 *tabp;
   double dbin, idbin, lb, tmin, tmax, z;
   int nbins;
