@@ -325,6 +325,11 @@ __device__ inline long cell_idx(const Grid &g, long i, long j, long k) {
   return i + (long)g.nx * (j + (long)g.ny * k);
 }
 
+// x times -1 when neg (the sign bit flipped: bitwise x * -1.0), else x
+__device__ inline double flip(double x, unsigned neg) {
+  return __longlong_as_double(__double_as_longlong(x) ^ ((long long)neg << 63));
+}
+
 // direction-local component a -> global field index
 __device__ inline int gcomp(int d, int a) {
   if (a == 1) return 1 + d;
@@ -355,7 +360,7 @@ __device__ inline void face_vjp(const Grid &g, const Fields &w, const Fields &vc
   const long cl = hl ? cell_idx(g, i - (d == 0), j - (d == 1), k - (d == 2)) : cr;
   const double mr = hr ? 1.0 : 0.0, ml = hl ? 1.0 : 0.0;
   long cell[6];
-  double sa[6], sn[6];
+  unsigned na = 0, nn = 0;     // bit l: point l's ghost sign is -1 (all / normal)
   double s[6][5];
 #pragma unroll
   for (int l = 0; l < 6; l++) {
@@ -363,11 +368,11 @@ __device__ inline void face_vjp(const Grid &g, const Fields &w, const Fields &vc
     int ii = i, jj = j, kk = k;
     if (d == 0) ii = m.src; else if (d == 1) jj = m.src; else kk = m.src;
     cell[l] = cell_idx(g, ii, jj, kk);
-    sa[l] = m.sgn_all;
-    sn[l] = m.sgn_norm;
+    na |= (m.sgn_all < 0.0 ? 1u : 0u) << l;
+    nn |= (m.sgn_norm < 0.0 ? 1u : 0u) << l;
 #pragma unroll
     for (int a = 0; a < 5; a++)
-      s[l][a] = w.w[gcomp(d, a)][cell[l]] * (a == 1 ? sn[l] : sa[l]);
+      s[l][a] = flip(w.w[gcomp(d, a)][cell[l]], ((a == 1 ? nn : na) >> l) & 1u);
   }
   FacePrim P;
   face_prim(s, g.gamma, P);
@@ -391,7 +396,7 @@ __device__ inline void face_vjp(const Grid &g, const Fields &w, const Fields &vc
     for (int c = 0; c < NC; c++) {
       double t[6], tb[6];
 #pragma unroll
-      for (int l = 0; l < 6; l++) t[l] = tn[l] * sa[l];
+      for (int l = 0; l < 6; l++) t[l] = flip(tn[l], (na >> l) & 1u);
       const double Fbc = (mr * rn - ml * ln) * idelta;
       const auto pre = pre_next;
       if (c + 1 < NC) {
@@ -401,7 +406,7 @@ __device__ inline void face_vjp(const Grid &g, const Fields &w, const Fields &vc
       __asm__ volatile("" ::: "memory");
       tracer_vjp(t, Fbc, P, 0.5 * P.alpha, tb, ulb, alphab);
 #pragma unroll
-      for (int l = 0; l < 6; l++) tb[l] *= sa[l];
+      for (int l = 0; l < 6; l++) tb[l] = flip(tb[l], (na >> l) & 1u);
       sink.put(5 + c, tb, pre);
     }
   }
@@ -419,7 +424,7 @@ __device__ inline void face_vjp(const Grid &g, const Fields &w, const Fields &vc
     const auto pre = sink.pre(q);
     double x[6];
 #pragma unroll
-    for (int l = 0; l < 6; l++) x[l] = sb[l][a] * (a == 1 ? sn[l] : sa[l]);
+    for (int l = 0; l < 6; l++) x[l] = flip(sb[l][a], ((a == 1 ? nn : na) >> l) & 1u);
     sink.put(q, x, pre);
   }
 }
