@@ -232,8 +232,14 @@ typedef struct {
   double A[16];            /* row-major 4x4 Butcher matrix */
   double c[4];
   double rtol, atol;       /* WRMS weights 1/(rtol|z|+atol) of the Newton correction */
-  double newton_tol;       /* converged when the WRMS norm of the correction <= newton_tol */
-  int max_iters;           /* Newton iterations before the Jacobian is re-evaluated (1..100) */
+  double newton_tol;       /* ARKODE's test (arkStep_NlsConvTest, SUNDIALS 6.2.0): iteration m
+                              of a stage solve converged when min(crate, 1) del / newton_tol
+                              <= 1, del the WRMS norm of its correction, crate = max(0.3
+                              crate, del / del_prev) for m > 0 (1 after every Jacobian
+                              setup, carried from stage to stage) */
+  int max_iters;           /* Newton iterations before the Jacobian is re-evaluated (1..100);
+                              a correction above 2.3 times the previous one (m > 0) ends
+                              the iterations early the same way */
   int max_setups;          /* Jacobian re-evaluations per stage before a cell fails (0..20) */
   double rhs_scale;        /* 1 for frhs, TimeUnits for ffast/fimpl */
   int max_halvings;        /* per-cell step rejection (batched solver only; 0..20, 0 = off):

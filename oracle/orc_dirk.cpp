@@ -10,10 +10,19 @@
 //   * stage i solves z - h a_ii f(z) = r_i,  r_i = y_n + h sum_{j<i} a_ij F_j;
 //   * Newton with M = I - h a_ii s_rhs J, J = calculate_jacobian_cvklu at y_n (after f(y_n)
 //     refreshed the temperature), factored once per step by 10x10 LU with partial pivoting
-//     and reused (modified Newton); if a stage reaches max_iters, J is re-evaluated at the
-//     current iterate, refactored and the stage restarted from it (up to max_setups times);
+//     and reused (modified Newton); if a stage reaches max_iters or its corrections grow, J
+//     is re-evaluated at the current iterate, refactored and the stage restarted from it
+//     (up to max_setups times);
 //   * predictor: the previous stage value (y_n for the first stage);
-//   * convergence: WRMS norm of the correction, weights 1/(rtol|z|+atol), <= newton_tol;
+//   * convergence: ARKODE's test (arkStep_NlsConvTest of ARKODE 5.2 in SUNDIALS 6.2.0, the
+//     version the reference pins, README.md:209): with del the WRMS norm of the correction,
+//     weights 1/(rtol|z|+atol) of the updated iterate (ARKODE weighs by the step start
+//     y_n; with those weights the multirate blast's fast steps fail where a species starts
+//     near zero), and m the iteration of the stage solve (0 first),
+//     crate = max(0.3 crate, del/del_prev) for m > 0,
+//     converged when min(crate, 1) del / newton_tol <= 1, diverging (a failure, as
+//     max_iters) when m > 0 and del > 2.3 del_prev; crate starts at 1 at every setup and
+//     carries over from stage to stage;
 //   * F_i = (z_i - r_i)/(h a_ii) (ARKODE's implicit-stage derivative), y_{n+1} = z_s.
 #include <cmath>
 #include <cstring>
@@ -107,6 +116,18 @@ double wrms(const double (&d)[NSP], const double (&z)[NSP], double rtol, double 
   return std::sqrt(s / NSP);
 }
 
+// ARKODE's convergence-rate update and test (crdown 0.3; SUNMAX / SUNMIN as ternaries)
+double conv_rate(double crate, double del, double delp) {
+  const double a = 0.3 * crate, b = del / delp;
+  return a > b ? a : b;
+}
+
+bool conv_ok(double crate, double del, double tol) {
+  return (crate < 1.0 ? crate : 1.0) * del / tol <= 1.0;
+}
+
+bool conv_diverging(double del, double delp) { return del > 2.3 * delp; }
+
 }  // namespace
 
 extern "C" int orc_lu(const double *Min, double *LUout, int *piv, const double *b, double *x) {
@@ -138,6 +159,7 @@ static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[N
   double M[NSP][NSP];
   int piv[NSP];
   bool ok_cell = true;
+  double crate = 1.0, delp = 0.0;     // convergence-rate estimate, last correction norm
   nit = 0;
   for (int st = 0; st < ns && ok_cell; st++) {
     const double hg = hstep * opt->A[st * 4 + st];
@@ -164,13 +186,18 @@ static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[N
         lu_solve(M, piv, d);
         for (int s = 0; s < NSP; s++) z[s] += d[s];
         nit++;
-        if (wrms(d, z, opt->rtol, opt->atol) <= opt->newton_tol) { conv = true; break; }
+        const double del = wrms(d, z, opt->rtol, opt->atol);
+        if (it > 0) crate = conv_rate(crate, del, delp);
+        if (conv_ok(crate, del, opt->newton_tol)) { conv = true; break; }
+        if (it > 0 && conv_diverging(del, delp)) break;
+        delp = del;
       }
       if (conv) break;
       if (++setups > opt->max_setups) { ok_cell = false; break; }
       cell_rhs(cl, z, opt->rhs_scale, f);             // temperature at the iterate
       cell_matrix(cl, z, hg, opt->rhs_scale, M);
       lu_factor(M, piv);
+      crate = 1.0;
     }
     for (int s = 0; s < NSP; s++) F[st][s] = (z[s] - r[s]) / hg;
     if (Y)

@@ -200,6 +200,29 @@ __device__ inline double wrms_norm(const double (&d)[NSP], const double (&z)[NSP
   return sqrt(s / NSP);
 }
 
+// ARKODE's Newton convergence test (arkStep_NlsConvTest, ARKODE 5.2 / SUNDIALS 6.2.0, the
+// version the reference builds against): with del the WRMS norm of the correction
+// (weights of the updated iterate, as wrms_norm; ARKODE weighs by the step start, with which
+// fast steps of the multirate blast fail where a species starts near zero) and m the
+// iteration of the stage solve (0 first), the rate
+// estimate crate = max(0.3 crate, del / del_prev) for m > 0 (1 after every setup, carried
+// from stage to stage); converged when min(crate, 1) del / tol <= 1; diverging, handled as
+// a failed iteration budget, when m > 0 and del > 2.3 del_prev.
+__device__ inline double conv_rate(double crate, double del, double delp) {
+  const double a = 0.3 * crate, b = del / delp;
+  return a > b ? a : b;
+}
+
+__device__ inline bool conv_ok(double crate, double del, double tol) {
+#pragma clang fp contract(off)
+  return (crate < 1.0 ? crate : 1.0) * del / tol <= 1.0;
+}
+
+__device__ inline bool conv_diverging(double del, double delp) {
+#pragma clang fp contract(off)
+  return del > 2.3 * delp;
+}
+
 // a_{st,j} with constant indices only (no dynamic indexing of the by-value argument)
 __device__ inline double dirk_coef(const DirkDev &o, int st, int j) {
   double v = 0.0;
@@ -456,6 +479,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     // costs registers that are worth more than the solve)
     bool setup = true, solved = !SUBSTEP && tdt != nullptr;
     int st = 0, it = 0, setups = 0, nit = 0, nsetup = 0, nst = 0, nfail = 0;
+    double crate = 1.0, delp = 0.0;                        // Newton rate estimate, last norm
     double hs = h;                                         // current (sub)step
     double hg = h * o.A[0];                                // SDIRK: one diagonal value
     int halv = 0, done = 0;                                // substep level, units done
@@ -481,6 +505,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
         nsetup++;
         setup = false;
         it = 0;
+        crate = 1.0;
         continue;
       }
       double d[NSP];
@@ -497,7 +522,11 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
       nit++;
       it++;
       bool step_end = false;
-      if (wrms_norm(d, z, o.rtol, o.atol) <= o.ntol) {
+      const double del = wrms_norm(d, z, o.rtol, o.atol);
+      if (it > 1) crate = conv_rate(crate, del, delp);
+      const bool diverging = it > 1 && conv_diverging(del, delp);
+      delp = del;
+      if (conv_ok(crate, del, o.ntol)) {
         if (++st == o.nstage) {
           if (SUBSTEP) done += 1 << (o.max_halvings - halv);
           step_end = !SUBSTEP || done >= (1 << o.max_halvings);
@@ -528,7 +557,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
           it = 0;
           setups = 0;
         }
-      } else if (it == o.max_iters) {
+      } else if (it == o.max_iters || diverging) {
         if (++setups > o.max_setups) {
           if (SUBSTEP) {
             // back to the substep start
@@ -621,12 +650,16 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
 // plain (reference-structured) kernels
 // ------------------------------------------------------------------------------------
 
-// f = s * rhs(z) for the active cells (updates Ts/dTs of those cells)
+// Plain-path cell states (`active`): 0 done with the stage (converged or failed), 1
+// iterating, 2 diverged and waiting for the re-setup.
+constexpr int PLAIN_ITERATING = 1 << 1, PLAIN_UNCONVERGED = (1 << 1) | (1 << 2);
+
+// f = s * rhs(z) for the cells whose state is in `mask` (updates Ts/dTs of those cells)
 __global__ void __launch_bounds__(256)
 plain_rhs(long n, ChemConst cc, ChemPtrsD p, double srhs, const double *z, double *f,
-          const int *active) {
+          const int *active, int mask) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !active[i]) return;
+  if (i >= n || !((mask >> active[i]) & 1)) return;
   const CellIn c = load_cell(p, i);
   double Ts = p.Ts[i], dTs = p.dTs[i];
   double zz[NSP], ff[NSP];
@@ -689,15 +722,16 @@ plain_stage_init(long n, int st, double h, DirkDev o, const double *yn, const do
   active[i] = failed[i] ? 0 : 1;
 }
 
-// Newton update (getrs + z += d + WRMS test) of the active cells; converged cells are
-// deactivated, the others counted (one atomic per wave: uniform address)
+// Newton update (getrs + z += d + ARKODE's convergence test, iteration m of the stage
+// solve) of the iterating cells; converged cells are done, diverging ones wait for the
+// re-setup; the still iterating (count[0]) and the diverged (count[1]) are counted.
 __global__ void __launch_bounds__(256)
 plain_newton(long n, double hg, DirkDev o, double *LU, const unsigned long long *piv,
              const double *f, const double *r, double *z, int *active, int *cell_iters,
-             int *nactive) {
+             double *crate, double *delp, int m, int *count) {
 #pragma clang fp contract(off)
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !active[i]) return;
+  if (i >= n || active[i] != 1) return;
   double d[NSP], zz[NSP];
 #pragma unroll
   for (int s = 0; s < NSP; s++) {
@@ -711,8 +745,27 @@ plain_newton(long n, double hg, DirkDev o, double *LU, const unsigned long long 
     z[NSP * i + s] = zz[s];
   }
   cell_iters[i] += 1;
-  if (wrms_norm(d, zz, o.rtol, o.atol) <= o.ntol) active[i] = 0;
-  else atomicAdd(nactive, 1);
+  const double del = wrms_norm(d, zz, o.rtol, o.atol), dp = delp[i];
+  double cr = crate[i];
+  if (m > 0) cr = conv_rate(cr, del, dp);
+  crate[i] = cr;
+  delp[i] = del;
+  if (conv_ok(cr, del, o.ntol)) active[i] = 0;
+  else if (m > 0 && conv_diverging(del, dp)) {
+    active[i] = 2;
+    atomicAdd(count + 1, 1);
+  } else {
+    atomicAdd(count, 1);
+  }
+}
+
+// after a setup: its cells iterate again, with the rate estimate back at 1
+__global__ void __launch_bounds__(256)
+plain_resume(long n, int *active, double *crate) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !active[i]) return;
+  active[i] = 1;
+  crate[i] = 1.0;
 }
 
 // stage end: F_st = (z - r)/hg
@@ -1067,10 +1120,10 @@ int dirk_batched_launch_noalloc(ChemState &s, const DirkDev &o, double h, double
 int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st) {
   const long n = s.n;
   if (n <= 0 || n > (1L << 30)) return -1;
-  // scratch: yn, z, f, r, F[MAXSTAGE-1], LU blocks | pivots | active, iters, setups,
-  // failed, count | per-wave statistics rows
-  const size_t b_dbl = align256(((size_t)n * NSP * (4 + (MAXSTAGE - 1)) + (size_t)n * NSP * NSP) *
-                                sizeof(double));
+  // scratch: yn, z, f, r, F[MAXSTAGE-1], LU blocks, rate estimate, last correction norm |
+  // pivots | active, iters, setups, failed, counts | per-wave statistics rows
+  const size_t b_dbl = align256(((size_t)n * NSP * (4 + (MAXSTAGE - 1)) + (size_t)n * NSP * NSP +
+                                 (size_t)2 * n) * sizeof(double));
   const size_t b_piv = align256((size_t)n * sizeof(unsigned long long));
   const size_t b_int = align256(((size_t)n * 4 + 64) * sizeof(int));
   const long nrow = (n + 63) / 64;
@@ -1079,6 +1132,7 @@ int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStrea
   char *w = (char *)s.work;
   double *yn = (double *)w, *z = yn + n * NSP, *f = z + n * NSP, *r = f + n * NSP;
   double *acc = r + n * NSP, *LU = acc + (long)(MAXSTAGE - 1) * n * NSP;
+  double *crate = LU + (long)NSP * NSP * n, *delp = crate + n;
   unsigned long long *piv = (unsigned long long *)(w + b_dbl);
   int *active = (int *)(w + b_dbl + b_piv), *iters = active + n, *csetups = iters + n,
       *failed = csetups + n, *nact = failed + n;
@@ -1091,11 +1145,12 @@ int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStrea
   hipLaunchKernelGGL(fill_int, dim3(nb(n)), dim3(256), 0, st, n, active, 1);
   // temperature at y_n, Jacobian at y_n, factor (the first stage's a_ii)
   hipLaunchKernelGGL(plain_rhs, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, o.rhs_scale, z, f,
-                     active);
+                     active, PLAIN_UNCONVERGED);
   hipLaunchKernelGGL(plain_jac, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, h * o.A[0],
                      o.rhs_scale, z, LU, active);
   hipLaunchKernelGGL(plain_getrf, dim3(nb(n)), dim3(256), 0, st, n, LU, piv, active);
   hipLaunchKernelGGL(count_setups, dim3(nb(n)), dim3(256), 0, st, n, active, csetups);
+  hipLaunchKernelGGL(plain_resume, dim3(nb(n)), dim3(256), 0, st, n, active, crate);
   int *hcnt = s.host_counters;
   for (int stg = 0; stg < o.nstage; stg++) {
     const double hg = h * o.A[stg * 4 + stg];
@@ -1104,17 +1159,20 @@ int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStrea
     int setups = 0;
     for (;;) {
       bool all_done = false;
+      hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, 2L, nact, 0);
       for (int it = 0; it < o.max_iters; it++) {
         hipLaunchKernelGGL(plain_rhs, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, o.rhs_scale, z,
-                           f, active);
+                           f, active, PLAIN_ITERATING);
         hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, 1L, nact, 0);
         hipLaunchKernelGGL(plain_newton, dim3(nb(n)), dim3(256), 0, st, n, hg, o, LU, piv, f, r,
-                           z, active, iters, nact);
-        // convergence test on the host, as the reference's Newton loop does
-        if (hipMemcpyAsync(hcnt, nact, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                           z, active, iters, crate, delp, it, nact);
+        // convergence test on the host, as the reference's Newton loop does: stop when no
+        // cell iterates; done when none diverged either
+        if (hipMemcpyAsync(hcnt, nact, 2 * sizeof(int), hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
           return -1;
-        if (*hcnt == 0) { all_done = true; break; }
+        if (hcnt[0] == 0) { all_done = hcnt[1] == 0; break; }
       }
       if (all_done) break;
       if (++setups > o.max_setups) {
@@ -1122,11 +1180,12 @@ int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStrea
         break;
       }
       hipLaunchKernelGGL(plain_rhs, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, o.rhs_scale, z, f,
-                         active);
+                         active, PLAIN_UNCONVERGED);
       hipLaunchKernelGGL(plain_jac, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, hg, o.rhs_scale,
                          z, LU, active);
       hipLaunchKernelGGL(plain_getrf, dim3(nb(n)), dim3(256), 0, st, n, LU, piv, active);
       hipLaunchKernelGGL(count_setups, dim3(nb(n)), dim3(256), 0, st, n, active, csetups);
+      hipLaunchKernelGGL(plain_resume, dim3(nb(n)), dim3(256), 0, st, n, active, crate);
     }
     if (stg + 1 < o.nstage)
       hipLaunchKernelGGL(plain_stage_finish, dim3(nb(n)), dim3(256), 0, st, n, stg, hg, z, r,
@@ -1233,6 +1292,7 @@ __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, doub
   double a[NSP][NSP];
   unsigned long long pv = 0;
   bool ok = true;
+  double crate = 1.0, delp = 0.0;                     // Newton rate estimate, last norm
 #pragma unroll
   for (int st = 0; st < MAXSTAGE; st++) {
     if (st < o.nstage && ok) {
@@ -1260,13 +1320,18 @@ __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, doub
           lu_solve_regs(a, pv, d);
 #pragma unroll
           for (int s = 0; s < NSP; s++) z[s] += d[s];
-          if (wrms_norm(d, z, o.rtol, o.atol) <= o.ntol) { conv = true; break; }
+          const double del = wrms_norm(d, z, o.rtol, o.atol);
+          if (it > 0) crate = conv_rate(crate, del, delp);
+          if (conv_ok(crate, del, o.ntol)) { conv = true; break; }
+          if (it > 0 && conv_diverging(del, delp)) break;
+          delp = del;
         }
         if (conv) break;
         if (++setups > o.max_setups) { ok = false; break; }
         cell_rhs<true>(cc, c, Ts, dTs, z, o.rhs_scale, f);   // temperature at the iterate
         cell_matrix_regs(cc, c, Ts, dTs, z, hg, o.rhs_scale, a);
         pv = lu_factor_regs(a);
+        crate = 1.0;
       }
 #pragma unroll
       for (int s = 0; s < NSP; s++) F[st][s] = (z[s] - r[s]) / hg;
