@@ -36,6 +36,12 @@ DIV_FLOPS_PER_VAR = 9
 DIV_FLOPS_XY, DIV_FLOPS_Z = 5, 4
 BYTES_PER_CELL_FE = 16 * 5          # read 5 fields + write 5 fields, FP64, nchem = 0
 PEAK_FP64_TFLOPS = 78.6             # MI355X FP64 (vector and matrix) peak
+# Newton controls of the reference's inputs (ARKODE's convergence test): the chemistry test
+# (input_primordial_ode.txt:73-74: maxniters 5, nlconvcoef 0.001) for the batched
+# chemistry step, the multirate blast (input_primordial_blast_mr.txt:70-71: maxniters 10,
+# nlconvcoef 0.01) for the multirate run
+CHEM_NEWTON = {"newton_tol": 1e-3, "max_iters": 5}
+MRI_NEWTON = {"newton_tol": 1e-2, "max_iters": 10}
 PEAK_HBM_GBS = 8000.0               # MI355X HBM3E
 
 
@@ -112,7 +118,7 @@ def bench_chem(api, problems, torch, n, seed, steps, h=1e3):
     ctx.chem_prepare(y0)
     Ts0 = ctx.chem_state("Ts", torch)
     dT0 = ctx.chem_state("dTs_ge", torch)
-    opts = api.sdirk3()
+    opts = api.sdirk3(**CHEM_NEWTON)
     yy = y0.clone()
     tuned_ms = ctx.dirk_tune(yy, h, opts)          # launch-shape autotuner (untimed)
 
@@ -141,7 +147,8 @@ def bench_chem(api, problems, torch, n, seed, steps, h=1e3):
     flops = nrhs * rhs_fl + b_st["setups"] * (jac_fl + lu_fl) + b_st["newton_iters"] * solve_fl
     out = {
         "workload": f"{round(n ** (1 / 3))}^3 = {n} independent primordial-chemistry cells, "
-                    f"one SDIRK3 step dt={h:g} s, seed {seed}",
+                    f"one SDIRK3 step dt={h:g} s, Newton tol {CHEM_NEWTON['newton_tol']:g} / "
+                    f"{CHEM_NEWTON['max_iters']} iterations, seed {seed}",
         "batched_ms": b_ms, "plain_ms": p_ms,
         "batched_cells_per_s": n / (b_ms * 1e-3), "plain_cells_per_s": n / (p_ms * 1e-3),
         "speedup_batched_vs_plain": p_ms / b_ms,
@@ -357,7 +364,7 @@ def bench_mri(api, problems, torch, n, nslow, cpu=True, cpu_n=16):
         ctx.chem_prepare(wd[5])
         return ctx, wd, fluid, chem
 
-    fast = api.sdirk3(rhs_scale=units[2], max_halvings=10)
+    fast = api.sdirk3(rhs_scale=units[2], max_halvings=10, **MRI_NEWTON)
     ctx, wd, _, _ = setup(n)
     h = min(ctx.stability(wd), h0)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -372,7 +379,8 @@ def bench_mri(api, problems, torch, n, nslow, cpu=True, cpu_n=16):
     nfast = sum(api.mri_fast_steps(i, h, hmax) for i in (2, 3, 4))
     out = {"workload": f"multirate primordial blast {n}^3, 10 species, reflecting, "
                        f"{nslow} slow steps of {h:g} (MIS Knoth-Wolke 3), {nfast} SDIRK3 fast "
-                       f"steps each (hmax {hmax:g}), consistent forcing units",
+                       f"steps each (hmax {hmax:g}, Newton tol {MRI_NEWTON['newton_tol']:g} / "
+                       f"{MRI_NEWTON['max_iters']} iterations), consistent forcing units",
            "slow_step": h, "fast_steps_per_slow_step": nfast, "failed_fast_cell_steps": failed,
            "finite": ok, "gpu_ms": ms,
            "cell_slow_steps_per_s": n ** 3 * nslow / (ms * 1e-3)}
@@ -389,8 +397,8 @@ def bench_mri(api, problems, torch, n, nslow, cpu=True, cpu_n=16):
         g = orc.make_grid(m, m, m, 10, (3,) * 6, gamma=5 / 3)
         orc.set_threads(1)
         t0 = time.perf_counter()
-        orc.mri_evolve(g, oc, eu, orc.dirk_opts(rhs_scale=units[2], max_halvings=10), hs, 1, hmax,
-                       wo, consistent=True)
+        orc.mri_evolve(g, oc, eu, orc.dirk_opts(rhs_scale=units[2], max_halvings=10, **MRI_NEWTON),
+                       hs, 1, hmax, wo, consistent=True)
         dt = time.perf_counter() - t0
         diffs = []
         for q in range(6):
@@ -421,7 +429,7 @@ def cpu_chem_baseline(n_sample, seed, threads, reps, h=1e3):
         ch.prepare(ys)
         ch.extra()
         t0 = time.perf_counter()
-        failed, _ = orc.dirk_step(ch, orc.dirk_opts(), h, ys)
+        failed, _ = orc.dirk_step(ch, orc.dirk_opts(**CHEM_NEWTON), h, ys)
         total += time.perf_counter() - t0
         assert failed == 0
         del ch

@@ -1,13 +1,15 @@
 """The benchmark configurations at their stated sizes, through the C ABI.
 
 C2  256^3 blast, periodic, nchem = 0: the fast fe() kernels against the OpenMP oracle.
-C3  64^3 = 262144 independent chemistry cells, one SDIRK3 step of 1e3 s: the batched kernel
+C3  64^3 = 262144 independent chemistry cells, one SDIRK3 step of 1e3 s with the chemistry
+    input's Newton controls (tolerance 1e-3, 5 iterations): the batched kernel
     (default launch shape, which at this size takes chunks from the work queue) against the
     oracle, states to 1e-10 and the Newton histogram identical.
 C4  128^3 with 10 species: dot-product tests of the fe() and chemistry vector-Jacobian
     products against central differences of the forward kernels on random directions.
 C5  the multirate primordial blast with the configuration's options (reflecting walls, slow
-    step min(CFL 0.3 bound, 0.05), fast steps of at most 1e-4 with per-cell step rejection):
+    step min(CFL 0.3 bound, 0.05), fast steps of at most 1e-4 with per-cell step rejection,
+    the input's Newton controls: tolerance 1e-2, 10 iterations):
     one slow step at 16^3 against the oracle, and two slow steps at 128^3 checked for a
     finite state, no failed cells and positive density and pressure.
 """
@@ -27,6 +29,10 @@ spec.loader.exec_module(problems)
 pytestmark = pytest.mark.gpu
 UNITS = (3.0e70, 3.0857e30, 1.0e11)
 EU = UNITS[0] / UNITS[1] / UNITS[2] / UNITS[2]
+# Newton controls of the reference's inputs: input_primordial_ode.txt:73-74 (C3) and
+# input_primordial_blast_mr.txt:70-71 (C5)
+C3_NEWTON = {"newton_tol": 1e-3, "max_iters": 5}
+C5_NEWTON = {"newton_tol": 1e-2, "max_iters": 10}
 
 
 def test_c2_fe_256_matches_oracle(mvd, torch_gpu):
@@ -57,14 +63,14 @@ def test_c3_batched_chemistry_262144_matches_oracle(mvd, torch_gpu):
     oc.prepare(yo)
     oc.extra()
     orc.set_threads(orc.max_threads())
-    failed_o, iters_o = orc.dirk_step(oc, orc.dirk_opts(), 1e3, yo)
+    failed_o, iters_o = orc.dirk_step(oc, orc.dirk_opts(**C3_NEWTON), 1e3, yo)
     ctx = mvd.Context(mvd.make_params(4, 4, 4, 10))
     ctx.chem_setup(orc.load_tables(), n)
     yd = torch.from_numpy(y.copy()).to("cuda")
     ctx.chem_prepare(yd)
     block, waves = ctx.dirk_get_shape()
     assert (n + 63) // 64 > waves            # more chunks than waves: the queue runs
-    failed_b, st = ctx.dirk_step(yd, 1e3)
+    failed_b, st = ctx.dirk_step(yd, 1e3, mvd.sdirk3(**C3_NEWTON))
     assert failed_o == failed_b == 0
     got = yd.cpu().numpy().reshape(n, 10)
     ref = yo.reshape(n, 10)
@@ -188,7 +194,7 @@ def test_c5_multirate_16_one_slow_step_matches_oracle(mvd, torch_gpu):
     dt_cfl = ctx.stability(wd)
     assert dt_cfl == orc.stability_dt(g, [a.copy() for a in fluid] + [chem.copy()], 0.3)
     h = min(dt_cfl, H0)
-    fast = mvd.sdirk3(rhs_scale=UNITS[2], max_halvings=10)
+    fast = mvd.sdirk3(rhs_scale=UNITS[2], max_halvings=10, **C5_NEWTON)
     assert ctx.mri_evolve(wd, h, 1, HMAX, fast, consistent=True) == 0
     oc = orc.Chem(m ** 3)
     y = chem.copy()
@@ -196,7 +202,8 @@ def test_c5_multirate_16_one_slow_step_matches_oracle(mvd, torch_gpu):
     oc.extra()
     wo = [a.copy() for a in fluid] + [y]
     orc.set_threads(orc.max_threads())
-    assert orc.mri_evolve(g, oc, EU, orc.dirk_opts(rhs_scale=UNITS[2], max_halvings=10), h, 1,
+    assert orc.mri_evolve(g, oc, EU, orc.dirk_opts(rhs_scale=UNITS[2], max_halvings=10,
+                                                   **C5_NEWTON), h, 1,
                           HMAX, wo, consistent=True) == 0
     for q in (0, 4):
         assert normwise_err(wd[q].cpu().numpy(), wo[q]) <= 1e-8, q
@@ -216,7 +223,7 @@ def test_c5_multirate_128_two_slow_steps_properties(mvd, torch_gpu):
     print(f"[C5] multirate {m}^3 two slow steps", flush=True)
     ctx, wd, _, _ = c5_setup(mvd, torch, m)
     h = min(ctx.stability(wd), H0)
-    fast = mvd.sdirk3(rhs_scale=UNITS[2], max_halvings=10)
+    fast = mvd.sdirk3(rhs_scale=UNITS[2], max_halvings=10, **C5_NEWTON)
     mass0 = float(wd[0].sum())
     assert ctx.mri_evolve(wd, h, 2, HMAX, fast, consistent=True) == 0
     for q in range(6):
