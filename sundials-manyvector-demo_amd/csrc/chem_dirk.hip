@@ -1309,29 +1309,30 @@ __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, doub
         for (int j = 0; j < st; j++) acc += o.A[st * 4 + j] * F[j][s];
         r[s] = yn[s] + h * acc;
       }
-      int setups = 0;
-      bool conv = false;
-      while (!conv) {
-        for (int it = 0; it < o.max_iters; it++) {
-          cell_rhs<true>(cc, c, Ts, dTs, z, o.rhs_scale, f);
-          double d[NSP];
+      // the stage's Newton iterations; after max_iters or a growing correction, a setup at
+      // the iterate and the iterations start again (up to max_setups times). One loop: the
+      // nested form of the same control flow made this kernel spill ~200 VGPRs.
+      int setups = 0, it = 0;
+      for (;;) {
+        cell_rhs<true>(cc, c, Ts, dTs, z, o.rhs_scale, f);
+        double d[NSP];
 #pragma unroll
-          for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
-          lu_solve_regs(a, pv, d);
+        for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * f[s] - r[s]);
+        lu_solve_regs(a, pv, d);
 #pragma unroll
-          for (int s = 0; s < NSP; s++) z[s] += d[s];
-          const double del = wrms_norm(d, z, o.rtol, o.atol);
-          if (it > 0) crate = conv_rate(crate, del, delp);
-          if (conv_ok(crate, del, o.ntol)) { conv = true; break; }
-          if (it > 0 && conv_diverging(del, delp)) break;
-          delp = del;
-        }
-        if (conv) break;
+        for (int s = 0; s < NSP; s++) z[s] += d[s];
+        const double del = wrms_norm(d, z, o.rtol, o.atol);
+        if (it > 0) crate = conv_rate(crate, del, delp);
+        if (conv_ok(crate, del, o.ntol)) break;
+        const bool diverging = it > 0 && conv_diverging(del, delp);
+        delp = del;
+        if (++it < o.max_iters && !diverging) continue;
         if (++setups > o.max_setups) { ok = false; break; }
         cell_rhs<true>(cc, c, Ts, dTs, z, o.rhs_scale, f);   // temperature at the iterate
         cell_matrix_regs(cc, c, Ts, dTs, z, hg, o.rhs_scale, a);
         pv = lu_factor_regs(a);
         crate = 1.0;
+        it = 0;
       }
 #pragma unroll
       for (int s = 0; s < NSP; s++) F[st][s] = (z[s] - r[s]) / hg;
