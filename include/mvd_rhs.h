@@ -247,6 +247,13 @@ typedef struct {
                               up to this many times, the stand-in for the fast integrator's
                               step-size control; a cell that fails at the finest size keeps
                               its last accepted state */
+  int max_steps_between_setups; /* ARKODE's msbp (its default 20): within one
+                              mvd_chem_dirk_steps call (and the multirate driver's fast
+                              steps) a step starts with a Jacobian setup at y_n only when
+                              this many steps have passed since the cell's last setup, or
+                              its previous step failed or was halved; the others start
+                              Newton with the factors they hold. 0 or 1: every step (the
+                              single-step entry points always set up) */
 } mvd_dirk_opts;
 
 typedef struct {

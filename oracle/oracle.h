@@ -22,8 +22,13 @@ typedef struct {
   int max_iters, max_setups;
   double rhs_scale;      /* 1 (frhs) or TimeUnits (ffast / fimpl) */
   int max_halvings;      /* per-cell step rejection (0: off) */
+  int max_steps_between_setups;   /* Jacobian setups in a multi-step call (0, 1: every step) */
 } orc_dirk;
 int orc_chem_dirk_step(void *chem, const orc_dirk *opt, double h, double *y, int *iters);
+/* nsteps DIRK steps of every cell in one call (the setup rule of max_steps_between_setups),
+ * with an optional constant forcing [cell][10]; iters (nullable) totals per cell */
+int orc_chem_dirk_steps_forced(void *chem, const orc_dirk *opt, double h, int nsteps,
+                               double *y, const double *force, int *iters);
 /* discrete adjoint of nsteps DIRK steps (orc_dirk.cpp); exact scaled Jacobian of one cell
  * (orc_chem.cpp) */
 int orc_chem_dirk_adjoint(void *chem, const orc_dirk *opt, double h, int nsteps, double *y,

@@ -147,23 +147,41 @@ extern "C" int orc_lu(const double *Min, double *LUout, int *piv, const double *
   return 0;
 }
 
+// A cell's Newton matrix factors and convergence-rate estimate kept from one step to the
+// next within a multi-step call (ARKODE's lsetup frequency, max_steps_between_setups).
+struct Carry {
+  double M[NSP][NSP];
+  int piv[NSP];
+  double crate;
+};
+
 // One DIRK step of one cell; y (10, scaled) is overwritten with y_{n+1}, the cell's Ts/dTs
 // follow its RHS evaluations. Y (may be null) receives the stage values. Returns false when
-// the Newton budget ran out; nit counts the Newton iterations.
+// the Newton budget ran out; nit counts the Newton iterations. With reuse, the step starts
+// Newton with the factors and rate estimate in cy instead of f(y_n) and a setup at y_n; cy
+// (may be null) receives the step's final factors and estimate, set (may be null) whether
+// the step set up.
 static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[NSP],
-                      double (*Y)[NSP], int &nit, const double *force = nullptr) {
+                      double (*Y)[NSP], int &nit, const double *force = nullptr,
+                      Carry *cy = nullptr, bool reuse = false, bool *set = nullptr) {
   const int ns = opt->nstage;
   double yn[NSP], z[NSP], F[4][NSP], f[NSP];
   for (int s = 0; s < NSP; s++) yn[s] = z[s] = y[s];
-  cell_rhs(cl, yn, opt->rhs_scale, f);               // refresh Ts at y_n for J
   double M[NSP][NSP];
   int piv[NSP];
-  bool ok_cell = true;
+  bool ok_cell = true, did_setup = !reuse;
   double crate = 1.0, delp = 0.0;     // convergence-rate estimate, last correction norm
+  if (reuse) {
+    std::memcpy(M, cy->M, sizeof(M));
+    std::memcpy(piv, cy->piv, sizeof(piv));
+    crate = cy->crate;
+  } else {
+    cell_rhs(cl, yn, opt->rhs_scale, f);             // refresh Ts at y_n for J
+  }
   nit = 0;
   for (int st = 0; st < ns && ok_cell; st++) {
     const double hg = hstep * opt->A[st * 4 + st];
-    if (st == 0) {
+    if (st == 0 && !reuse) {
       cell_matrix(cl, yn, hg, opt->rhs_scale, M);
       lu_factor(M, piv);
     }
@@ -198,12 +216,19 @@ static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[N
       cell_matrix(cl, z, hg, opt->rhs_scale, M);
       lu_factor(M, piv);
       crate = 1.0;
+      did_setup = true;
     }
     for (int s = 0; s < NSP; s++) F[st][s] = (z[s] - r[s]) / hg;
     if (Y)
       for (int s = 0; s < NSP; s++) Y[st][s] = z[s];
   }
   for (int s = 0; s < NSP; s++) y[s] = z[s];
+  if (cy) {
+    std::memcpy(cy->M, M, sizeof(M));
+    std::memcpy(cy->piv, piv, sizeof(piv));
+    cy->crate = crate;
+  }
+  if (set) *set = did_setup;
   return ok_cell;
 }
 
@@ -212,18 +237,28 @@ static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[N
 // warm start) and retries in halves, up to max_halvings times; accepted substeps count in
 // units of h / 2^max_halvings. A cell that fails at the finest size keeps its last accepted
 // state. Without it, the plain step (a failed cell keeps its last Newton iterate).
+// With a carry (multi-step calls): the first attempt may reuse the carried factors (reuse),
+// every later attempt and substep sets up (its h differs or its predecessor failed); set
+// reports whether any attempt set up, halved whether the step was split.
 static bool step_cell_sub(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[NSP],
-                          int &nit, const double *force) {
-  if (opt->max_halvings <= 0) return step_cell(cl, opt, hstep, y, nullptr, nit, force);
+                          int &nit, const double *force, Carry *cy = nullptr,
+                          bool reuse = false, bool *set = nullptr, bool *halved = nullptr) {
+  if (halved) *halved = false;
+  if (opt->max_halvings <= 0) return step_cell(cl, opt, hstep, y, nullptr, nit, force, cy, reuse, set);
   const int mh = opt->max_halvings;
   int halv = 0, done = 0;
+  bool any_set = false, first = true;
   nit = 0;
   while (done < (1 << mh)) {
     double z[NSP];
     for (int s = 0; s < NSP; s++) z[s] = y[s];
     const double Ts0 = cl.Ts, dTs0 = cl.dTs;
     int k = 0;
-    const bool ok = step_cell(cl, opt, std::ldexp(hstep, -halv), z, nullptr, k, force);
+    bool ds = false;
+    const bool ok = step_cell(cl, opt, std::ldexp(hstep, -halv), z, nullptr, k, force, cy,
+                              first && reuse, &ds);
+    first = false;
+    any_set = any_set || ds;
     nit += k;
     if (ok) {
       for (int s = 0; s < NSP; s++) y[s] = z[s];
@@ -231,10 +266,16 @@ static bool step_cell_sub(Cell &cl, const orc_dirk *opt, double hstep, double (&
     } else {
       cl.Ts = Ts0;
       cl.dTs = dTs0;
-      if (halv == mh) return false;
+      if (halv == mh) {
+        if (set) *set = any_set;
+        if (halved) *halved = halv > 0;
+        return false;
+      }
       halv++;
     }
   }
+  if (set) *set = any_set;
+  if (halved) *halved = halv > 0;
   return true;
 }
 
@@ -277,6 +318,46 @@ extern "C" int orc_chem_dirk_step_forced(void *h, const orc_dirk *opt, double hs
 extern "C" int orc_chem_dirk_step(void *h, const orc_dirk *opt, double hstep, double *y,
                                   int *iters) {
   return orc_chem_dirk_step_forced(h, opt, hstep, y, nullptr, iters);
+}
+
+// nsteps steps of every cell in one call, with ARKODE's lsetup frequency: a step starts with
+// f(y_n) and a Jacobian setup at y_n when it is the call's first, when its predecessor
+// failed or was split, or when max_steps_between_setups steps (at least 1) have passed since
+// the cell's last setup (any setup, those after Newton failures included; ARKODE's nst >=
+// nstlp + msbp); the other steps start Newton with the factors and convergence-rate estimate
+// the cell holds. Returns the failed cell steps; iters[i] (may be null) the cell's Newton
+// iterations over all its steps.
+extern "C" int orc_chem_dirk_steps_forced(void *h, const orc_dirk *opt, double hstep,
+                                          int nsteps, double *y, const double *force,
+                                          int *iters) {
+  ChemState *c = (ChemState *)h;
+  const int msbp = opt->max_steps_between_setups > 1 ? opt->max_steps_between_setups : 1;
+  int failed = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : failed)
+  for (long i = 0; i < c->n; i++) {
+    Cell cl = cell_of(c, i);
+    double z[NSP];
+    for (int s = 0; s < NSP; s++) z[s] = y[NSP * i + s];
+    Carry cy{};
+    int lsetup = 0, nit_all = 0;
+    bool must_setup = true;
+    for (int k = 0; k < nsteps; k++) {
+      const bool reuse = !must_setup && k - lsetup < msbp;
+      bool ds = false, halved = false;
+      int nit = 0;
+      const bool ok = step_cell_sub(cl, opt, hstep, z, nit, force ? force + NSP * i : nullptr,
+                                    &cy, reuse, &ds, &halved);
+      if (ds) lsetup = k;
+      must_setup = !ok || halved;
+      nit_all += nit;
+      if (!ok) failed++;
+    }
+    for (int s = 0; s < NSP; s++) y[NSP * i + s] = z[s];
+    c->Ts[i] = cl.Ts;
+    c->dTs_ge[i] = cl.dTs;
+    if (iters) iters[i] = nit_all;
+  }
+  return failed;
 }
 
 extern "C" int orc_chem_exact_jac_cell(void *h, long i, const double *y, double *T,

@@ -23,8 +23,9 @@
 
 extern "C" int orc_fslow(const orc_grid *g, void *chem, double energy_units, double *const *w,
                          double *const *wdot);
-extern "C" int orc_chem_dirk_step_forced(void *h, const orc_dirk *opt, double hstep, double *y,
-                                         const double *force, int *iters);
+extern "C" int orc_chem_dirk_steps_forced(void *h, const orc_dirk *opt, double hstep,
+                                          int nsteps, double *y, const double *force,
+                                          int *iters);
 extern "C" double *orc_chem_array(void *h, int which);
 
 // Knoth-Wolke 3 with b appended: c, A (row-major, 4 x 4)
@@ -86,11 +87,13 @@ extern "C" int orc_mri_evolve(const orc_grid *g, void *chem, double energy_units
         }
       const int nfast = orc_mri_fast_steps(i, h, hmax);
       const double dt = dc * h / nfast;
-      for (int m = 0; m < nfast; m++) {
+      // the fluid fields and the chemistry do not interact within a slow stage: the
+      // fluid's nfast additions, then the chemistry's nfast steps in one call (its Jacobian
+      // setups by the fast options' max_steps_between_setups)
+      for (int m = 0; m < nfast; m++)
         for (int q = 0; q < 5; q++)
           for (long k = 0; k < n; k++) w[q][k] = w[q][k] + dt * R[q][k];
-        failed += orc_chem_dirk_step_forced(chem, fast, dt, w[5], R[5].data(), nullptr);
-      }
+      failed += orc_chem_dirk_steps_forced(chem, fast, dt, nfast, w[5], R[5].data(), nullptr);
     }
   }
   return illegal ? -1 : failed;

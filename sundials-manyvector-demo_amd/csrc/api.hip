@@ -564,7 +564,8 @@ int dirk_check(const mvd_dirk_opts *o, mvd::DirkDev &d) {
   if (!o || o->nstage < 1 || o->nstage > mvd::MAXSTAGE) return MVD_BAD_ARGUMENT;
   if (o->max_iters < 1 || o->max_iters > 100 || o->max_setups < 0 || o->max_setups > 20 ||
       !(o->newton_tol > 0) || !(o->rtol >= 0) || !(o->atol > 0) || !(o->rhs_scale > 0) ||
-      o->max_halvings < 0 || o->max_halvings > 20)
+      o->max_halvings < 0 || o->max_halvings > 20 || o->max_steps_between_setups < 0 ||
+      o->max_steps_between_setups > 1000)
     return MVD_BAD_ARGUMENT;
   for (int i = 0; i < o->nstage; i++) {
     if (!(o->A[i * 4 + i] > 0) || o->A[i * 4 + i] != o->A[0]) return MVD_UNSUPPORTED;  // SDIRK
@@ -577,6 +578,7 @@ int dirk_check(const mvd_dirk_opts *o, mvd::DirkDev &d) {
   d.max_iters = o->max_iters; d.max_setups = o->max_setups;
   d.rhs_scale = o->rhs_scale;
   d.max_halvings = o->max_halvings;
+  d.msbp = o->max_steps_between_setups > 1 ? o->max_steps_between_setups : 1;
   return MVD_SUCCESS;
 }
 

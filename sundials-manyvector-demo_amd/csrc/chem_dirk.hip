@@ -480,6 +480,8 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     bool setup = true, solved = !SUBSTEP && tdt != nullptr;
     int st = 0, it = 0, setups = 0, nit = 0, nsetup = 0, nst = 0, nfail = 0;
     double crate = 1.0, delp = 0.0;                        // Newton rate estimate, last norm
+    int lsetup = 0;                                        // step of the last Jacobian setup
+    bool fset = false;                                     // this step failed
     double hs = h;                                         // current (sub)step
     double hg = h * o.A[0];                                // SDIRK: one diagonal value
     int halv = 0, done = 0;                                // substep level, units done
@@ -506,6 +508,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
         setup = false;
         it = 0;
         crate = 1.0;
+        lsetup = nst;
         continue;
       }
       double d[NSP];
@@ -579,6 +582,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
             }
           }
           nfail++;                                         // the step failed
+          fset = true;
           step_end = true;
         } else {
           setup = true;
@@ -594,7 +598,10 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
         st = 0;
         it = 0;
         setups = 0;
-        setup = true;
+        // ARKODE's lsetup frequency: a setup at y_n after a failed or split step and once
+        // msbp steps have passed since the last one; else Newton starts with the held factors
+        setup = fset || halv > 0 || nst - lsetup >= o.msbp;
+        fset = false;
         hs = h;
         hg = h * o.A[0];
         halv = 0;

@@ -124,7 +124,7 @@ bool same_opts(const DirkDev &a, const DirkDev &b) {
   for (int k = 0; k < 16; k++)
     if (a.A[k] != b.A[k]) return false;
   return a.rtol == b.rtol && a.atol == b.atol && a.ntol == b.ntol &&
-         a.rhs_scale == b.rhs_scale && a.max_halvings == b.max_halvings;
+         a.rhs_scale == b.rhs_scale && a.max_halvings == b.max_halvings && a.msbp == b.msbp;
 }
 
 }  // namespace

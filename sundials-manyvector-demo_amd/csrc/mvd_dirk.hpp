@@ -24,6 +24,7 @@ struct DirkDev {
   int max_iters, max_setups;
   double rhs_scale;
   int max_halvings;     // per-cell step rejection (batched solver), 0: off
+  int msbp;             // steps between Jacobian setups within one multi-step launch (>= 1)
 };
 
 // force (nullable): constant forcing added to the scaled RHS, [cell][10]. reuse_order:

@@ -36,7 +36,7 @@ class DirkOpts(C.Structure):
     _fields_ = [("nstage", C.c_int), ("A", C.c_double * 16), ("c", C.c_double * 4),
                 ("rtol", C.c_double), ("atol", C.c_double), ("newton_tol", C.c_double),
                 ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double),
-                ("max_halvings", C.c_int)]
+                ("max_halvings", C.c_int), ("max_steps_between_setups", C.c_int)]
 
 
 class ArkTable(C.Structure):

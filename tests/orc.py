@@ -36,7 +36,7 @@ class Dirk(C.Structure):
     _fields_ = [("nstage", C.c_int), ("A", C.c_double * 16), ("c", C.c_double * 4),
                 ("rtol", C.c_double), ("atol", C.c_double), ("newton_tol", C.c_double),
                 ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double),
-                ("max_halvings", C.c_int)]
+                ("max_halvings", C.c_int), ("max_steps_between_setups", C.c_int)]
 
 
 def sdirk3_table():
@@ -113,7 +113,7 @@ def erk_evolve(g, tab, h, nsteps, w):
 
 
 def dirk_opts(A=None, c=None, nstage=3, rtol=1e-5, atol=1e-9, newton_tol=0.1, max_iters=5,
-              max_setups=2, rhs_scale=1.0, max_halvings=0):
+              max_setups=2, rhs_scale=1.0, max_halvings=0, max_steps_between_setups=0):
     if A is None:
         A, c = sdirk3_table()
     o = Dirk()
@@ -125,6 +125,7 @@ def dirk_opts(A=None, c=None, nstage=3, rtol=1e-5, atol=1e-9, newton_tol=0.1, ma
     o.rtol, o.atol, o.newton_tol = rtol, atol, newton_tol
     o.max_iters, o.max_setups, o.rhs_scale = max_iters, max_setups, rhs_scale
     o.max_halvings = max_halvings
+    o.max_steps_between_setups = max_steps_between_setups
     return o
 
 
@@ -172,6 +173,9 @@ def lib():
         L.orc_chem_dirk_step_forced.argtypes = [C.c_void_p, C.POINTER(Dirk), C.c_double, P, P,
                                                 C.POINTER(C.c_int)]
         L.orc_chem_dirk_step_forced.restype = C.c_int
+        L.orc_chem_dirk_steps_forced.argtypes = [C.c_void_p, C.POINTER(Dirk), C.c_double,
+                                                 C.c_int, P, P, C.POINTER(C.c_int)]
+        L.orc_chem_dirk_steps_forced.restype = C.c_int
         L.orc_mri_evolve.argtypes = [C.POINTER(Grid), C.c_void_p, C.c_double, C.POINTER(Dirk),
                                      C.c_double, C.c_int, C.c_double, C.c_int, PP]
         L.orc_mri_evolve.restype = C.c_int
@@ -410,6 +414,20 @@ def dirk_step_forced(chem, opts, h, y, force):
     """One oracle DIRK step of y' = s_rhs f(y) + force (force [cell][10], constant)."""
     force = np.ascontiguousarray(force, dtype=np.float64)
     return lib().orc_chem_dirk_step_forced(chem.h, C.byref(opts), h, _p(y), _p(force), None)
+
+
+def dirk_steps(chem, opts, h, nsteps, y, force=None):
+    """nsteps oracle DIRK steps of every cell in one call (Jacobian setups by the
+    max_steps_between_setups rule), optional constant forcing [cell][10]. Returns (failed
+    cell steps, per-cell Newton iterations)."""
+    iters = np.zeros(chem.n, dtype=np.int32)
+    fp = None
+    if force is not None:
+        force = np.ascontiguousarray(force, dtype=np.float64)
+        fp = _p(force)
+    failed = lib().orc_chem_dirk_steps_forced(chem.h, C.byref(opts), h, int(nsteps), _p(y), fp,
+                                              iters.ctypes.data_as(C.POINTER(C.c_int)))
+    return failed, iters
 
 
 def mri_kw3_table():

@@ -263,6 +263,48 @@ def test_dirk_steps_equal_repeated_steps(mvd, torch_gpu, variant):
     assert np.array_equal(y1, y0) and np.array_equal(T1, T0) and np.array_equal(d1, d0)
 
 
+@pytest.mark.parametrize("variant", ["plain", "forced", "substep", "failing"])
+def test_dirk_steps_setup_reuse_matches_oracle(mvd, torch_gpu, variant):
+    """ARKODE's lsetup frequency within one multi-step call (max_steps_between_setups = 7
+    over 16 steps): cells set up at y_n only on their first step, after a failed or split
+    step and every 7 steps since their last setup, and otherwise start Newton with the
+    factors and rate estimate they hold. The batched launch against the oracle's multi-step
+    call: states, temperatures, failed cell steps and (without failures) the per-cell Newton
+    iterations in total. The same call with setups every step differs from it (the reuse
+    is not a no-op)."""
+    torch = torch_gpu
+    n, seed, nsteps, msbp = 4096, 43, 16, 7
+    print(f"[dirk steps reuse] n={n} variant={variant} nsteps={nsteps} msbp={msbp}", flush=True)
+    rng = np.random.default_rng(seed)
+    force = rng.standard_normal(10 * n) * 1e-5 if variant == "forced" else None
+    kw = {"plain": {}, "forced": {}, "substep": dict(max_halvings=4, max_iters=2, max_setups=0),
+          "failing": dict(max_iters=1, max_setups=0)}[variant]
+    h = {"plain": 300.0, "forced": 300.0, "substep": 2e4, "failing": 1e5}[variant]
+    oc, yo, ctx, yd = setup(mvd, torch, n, seed)
+    fd = torch.from_numpy(force).to("cuda") if force is not None else None
+    failed_d, st = ctx.dirk_steps(yd, h, nsteps, fd,
+                                  mvd.sdirk3(max_steps_between_setups=msbp, **kw))
+    failed_o, iters_o = orc.dirk_steps(oc, orc.dirk_opts(max_steps_between_setups=msbp, **kw), h,
+                                       nsteps, yo, force)
+    assert failed_d == failed_o
+    if variant == "failing":
+        assert failed_o > 0
+    ok = np.ones(n, dtype=bool)
+    assert compare(yd.cpu().numpy(), yo, ok) <= TOL
+    Td = ctx.chem_state("Ts", torch).cpu().numpy()
+    assert np.max(np.abs(Td - oc.array("Ts")) / oc.array("Ts")) <= TOL
+    if failed_o == 0:
+        assert st["newton_iters"] == int(iters_o.sum())
+    # the oracle with a setup on every step takes another path (fewer reused factors)
+    y1, _ = problems.chem_cells(n, seed=seed)
+    oc1 = orc.Chem(n)
+    oc1.prepare(y1)
+    oc1.extra()
+    orc.dirk_steps(oc1, orc.dirk_opts(max_steps_between_setups=1, **kw), h, nsteps, y1, force)
+    assert not np.array_equal(y1, yo)
+    ctx.close()
+
+
 def test_nvec_axpy_steps_equals_repeated_combinations(mvd, torch_gpu):
     torch = torch_gpu
     ctx = mvd.Context(mvd.make_params(4, 4, 4, 10))
