@@ -226,11 +226,13 @@ double *mvd_chem_state(mvd_context *ctx, int which);
  * SUNLinSol_MagmaDense/KLU, primordial_ode_main.cpp:422-536, and the block-diagonal
  * SUNLinSol_RankLocalLS, multirate_chem_hydro_main.cpp:1312-1411) ---- */
 
-/* SDIRK table (equal diagonal entries, stiffly accurate) and Newton controls */
+/* DIRK table and Newton controls. The table is SDIRK (every diagonal entry the same
+ * positive value) or ESDIRK (an explicit first stage, z_1 = y_n: first row zero, the other
+ * diagonal entries equal and positive), stiffly accurate (y_{n+1} is the last stage). */
 typedef struct {
-  int nstage;              /* 1..4 */
-  double A[16];            /* row-major 4x4 Butcher matrix */
-  double c[4];
+  int nstage;              /* 1..7 (ESDIRK: 2..7) */
+  double A[49];            /* row-major 7x7 Butcher matrix (entries past nstage zero) */
+  double c[7];
   double rtol, atol;       /* WRMS weights 1/(rtol|z|+atol) of the Newton correction */
   double newton_tol;       /* ARKODE's test (arkStep_NlsConvTest, SUNDIALS 6.2.0): iteration m
                               of a stage solve converged when min(crate, 1) del / newton_tol
@@ -267,6 +269,12 @@ typedef struct {
 
 /* Alexander's 3-stage, 3rd-order, L-stable SDIRK and default Newton controls */
 int mvd_dirk_sdirk3(mvd_dirk_opts *o);
+/* The same defaults with the ESDIRK part of ARK4(3)7L[2]SA (Kennedy & Carpenter 2019; ARKODE's
+ * ARK437L2SA_DIRK_7_3_4, the implicit table the reference's multirate and IMEX inputs select,
+ * itable = 112 in tests/primordial_blast/input_primordial_blast_mr.txt:48): seven stages,
+ * gamma = 0.1235, fourth order, stiffly accurate. No counterpart call in the reference (it
+ * selects the table through ARKStepSetTableNum, multirate_chem_hydro_main.cpp). */
+int mvd_dirk_ark437(mvd_dirk_opts *o);
 /* One step of size h of every cell, y (ncell x 10, scaled, device) overwritten with
  * y_{n+1}. plain = 0: the batched persistent kernel; plain = 1: the reference-structured
  * multi-kernel path. Returns 0, or the number of failed cells (> 0, recoverable). */
@@ -279,7 +287,9 @@ int mvd_chem_dirk_step(mvd_context *ctx, double t, double h, const mvd_dirk_opts
  * and the work-queue and statistics counters live in a library-owned array of 22.4 MiB per
  * GPU instead of the context's lazily allocated buffer, so the call allocates nothing.
  * Limits: at most 1048576 cells (mvd_chem_setup's ncell) and 4096 blocks in the persistent
- * grid, else MVD_BAD_ARGUMENT before any device work. Every call on one GPU shares that
+ * grid, else MVD_BAD_ARGUMENT before any device work; tables of at most 4 stages (a longer
+ * one, such as mvd_dirk_ark437's, needs a stage buffer the array does not hold:
+ * MVD_UNSUPPORTED). Every call on one GPU shares that
  * GPU's array: calls that overlap on one GPU (other streams, threads or contexts) race and
  * must be serialised by the caller. stats->ms is host wall time of the call. */
 int mvd_chem_dirk_step_noalloc(mvd_context *ctx, double t, double h, const mvd_dirk_opts *o,
@@ -357,7 +367,8 @@ int mvd_imex_evolve(mvd_context *ctx, double t0, double h, int nsteps, const mvd
  * ckpt_every steps; then lam0 = (d y_N / d y_0)^T lamT (both [cell][10]) for the stage
  * equations solved exactly along that trajectory, with the exact stage Jacobians; each
  * segment is recomputed once from its checkpoint, keeping its steps' stage values. o must
- * be stiffly accurate (the last row of A sums to 1, as mvd_dirk_sdirk3). Ts/dTs end at the
+ * be an SDIRK table of at most 4 stages, stiffly accurate (the last row of A sums to 1, as
+ * mvd_dirk_sdirk3; an ESDIRK table gives MVD_UNSUPPORTED). Ts/dTs end at the
  * forward run's final state. Returns 0, the number (> 0) of cells whose forward Newton
  * budget ran out (lam0 is still computed along the computed trajectory), or an error code.
  * The cells run in the batched step's temperature order at y_0 (from 4096 cells on).

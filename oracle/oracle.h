@@ -15,9 +15,10 @@ typedef struct {
 
 /* one DIRK step of the chemistry (orc_dirk.cpp) */
 typedef struct {
-  int nstage;
-  double A[16];          /* 4x4 row-major Butcher matrix (lower triangular) */
-  double c[4];
+  int nstage;            /* 1..7 */
+  double A[49];          /* 7x7 row-major Butcher matrix (lower triangular); SDIRK, or
+                            ESDIRK when the first row is zero (explicit first stage) */
+  double c[7];
   double rtol, atol, newton_tol;
   int max_iters, max_setups;
   double rhs_scale;      /* 1 (frhs) or TimeUnits (ffast / fimpl) */

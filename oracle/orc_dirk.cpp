@@ -23,7 +23,10 @@
 //     converged when min(crate, 1) del / newton_tol <= 1, diverging (a failure, as
 //     max_iters) when m > 0 and del > 2.3 del_prev; crate starts at 1 at every setup and
 //     carries over from stage to stage;
-//   * F_i = (z_i - r_i)/(h a_ii) (ARKODE's implicit-stage derivative), y_{n+1} = z_s.
+//   * F_i = (z_i - r_i)/(h a_ii) (ARKODE's implicit-stage derivative), y_{n+1} = z_s;
+//   * ESDIRK tables (first row zero, e.g. ARK4(3)7L[2]SA's implicit part, the reference
+//     inputs' itable 112): the first stage is explicit, z_1 = y_n and F_1 = f(y_n) (plus the
+//     forcing), the Jacobian setup at y_n uses the implicit stages' a_ii.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -33,6 +36,8 @@
 using namespace orc;
 
 namespace {
+
+constexpr int OS = 7;   // row stride of orc_dirk::A
 
 void lu_factor(double (&a)[NSP][NSP], int (&piv)[NSP]) {
   for (int k = 0; k < NSP; k++) {
@@ -165,37 +170,51 @@ static bool step_cell(Cell &cl, const orc_dirk *opt, double hstep, double (&y)[N
                       double (*Y)[NSP], int &nit, const double *force = nullptr,
                       Carry *cy = nullptr, bool reuse = false, bool *set = nullptr) {
   const int ns = opt->nstage;
-  double yn[NSP], z[NSP], F[4][NSP], f[NSP];
+  // ESDIRK (first row zero): the first stage is explicit, z_1 = y_n, F_1 = f(y_n)
+  const bool esdirk = ns >= 2 && opt->A[0] == 0.0;
+  double yn[NSP], z[NSP], F[OS][NSP], f[NSP];
   for (int s = 0; s < NSP; s++) yn[s] = z[s] = y[s];
   double M[NSP][NSP];
   int piv[NSP];
   bool ok_cell = true, did_setup = !reuse;
   double crate = 1.0, delp = 0.0;     // convergence-rate estimate, last correction norm
+  bool have_f = false;                // f holds f(y_n) for the first Newton iteration
   if (reuse) {
     std::memcpy(M, cy->M, sizeof(M));
     std::memcpy(piv, cy->piv, sizeof(piv));
     crate = cy->crate;
-  } else {
-    cell_rhs(cl, yn, opt->rhs_scale, f);             // refresh Ts at y_n for J
+  }
+  if (!reuse || esdirk) {
+    cell_rhs(cl, yn, opt->rhs_scale, f);             // Ts at y_n for J; F_1 of an ESDIRK
+    if (esdirk) {
+      for (int s = 0; s < NSP; s++) F[0][s] = force ? f[s] + force[s] : f[s];
+      if (Y)
+        for (int s = 0; s < NSP; s++) Y[0][s] = yn[s];
+      // with held factors this RHS is also the first implicit stage's first iteration's
+      // (the batched solver's pass order); after a setup that iteration evaluates again
+      have_f = reuse;
+    }
   }
   nit = 0;
-  for (int st = 0; st < ns && ok_cell; st++) {
-    const double hg = hstep * opt->A[st * 4 + st];
-    if (st == 0 && !reuse) {
+  const int st0 = esdirk ? 1 : 0;
+  for (int st = st0; st < ns && ok_cell; st++) {
+    const double hg = hstep * opt->A[st * OS + st];
+    if (st == st0 && !reuse) {
       cell_matrix(cl, yn, hg, opt->rhs_scale, M);
       lu_factor(M, piv);
     }
     double r[NSP];
     for (int s = 0; s < NSP; s++) {
       double acc = 0.0;
-      for (int j = 0; j < st; j++) acc += opt->A[st * 4 + j] * F[j][s];
+      for (int j = 0; j < st; j++) acc += opt->A[st * OS + j] * F[j][s];
       r[s] = yn[s] + hstep * acc;
     }
     int setups = 0;
     bool conv = false;
     while (!conv) {
       for (int it = 0; it < opt->max_iters; it++) {
-        cell_rhs(cl, z, opt->rhs_scale, f);
+        if (have_f) have_f = false;
+        else cell_rhs(cl, z, opt->rhs_scale, f);
         double d[NSP];
         if (force)
           for (int s = 0; s < NSP; s++) d[s] = -(z[s] - hg * (f[s] + force[s]) - r[s]);
@@ -376,6 +395,7 @@ extern "C" int orc_chem_dirk_adjoint(void *h, const orc_dirk *opt, double hstep,
                                      double *y, const double *lamT, double *lam0) {
   ChemState *c = (ChemState *)h;
   const int ns = opt->nstage;
+  if (ns >= 2 && opt->A[0] == 0.0) return -1;        // SDIRK tables only
   int failed = 0;
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : failed)
   for (long i = 0; i < c->n; i++) {
@@ -403,16 +423,16 @@ extern "C" int orc_chem_dirk_adjoint(void *h, const orc_dirk *opt, double hstep,
       Cell cr = cl;
       cr.Ts = rec[NSP];
       cr.dTs = rec[NSP + 1];
-      double yy[NSP], Y[4][NSP];
+      double yy[NSP], Y[OS][NSP];
       for (int s = 0; s < NSP; s++) yy[s] = rec[s];
       int nit = 0;
       step_cell(cr, opt, hstep, yy, Y, nit);
-      double nu[4][NSP] = {};
+      double nu[OS][NSP] = {};
       for (int k = ns - 1; k >= 0; k--) {
         double w[NSP];
         for (int s = 0; s < NSP; s++) {
-          double acc = opt->A[(ns - 1) * 4 + k] * lam[s];
-          for (int m = k + 1; m < ns; m++) acc += opt->A[m * 4 + k] * nu[m][s];
+          double acc = opt->A[(ns - 1) * OS + k] * lam[s];
+          for (int m = k + 1; m < ns; m++) acc += opt->A[m * OS + k] * nu[m][s];
           w[s] = acc;
         }
         double J[NSP * NSP], T = cr.Ts;
@@ -423,7 +443,7 @@ extern "C" int orc_chem_dirk_adjoint(void *h, const orc_dirk *opt, double hstep,
           for (int s = 0; s < NSP; s++) acc += (opt->rhs_scale * J[s * NSP + q]) * w[s];
           yb[q] = hstep * acc;
         }
-        const double hg = hstep * opt->A[k * 4 + k];
+        const double hg = hstep * opt->A[k * OS + k];
         for (int r = 0; r < NSP; r++)
           for (int q = 0; q < NSP; q++)
             Mt[r][q] = (r == q ? 1.0 : 0.0) - hg * (opt->rhs_scale * J[q * NSP + r]);

@@ -544,10 +544,11 @@ int mvd_dirk_sdirk3(mvd_dirk_opts *o) {
   const double t2 = 0.5 * (1.0 + g);
   const double b1 = -(6.0 * g * g - 16.0 * g + 1.0) / 4.0;
   const double b2 = (6.0 * g * g - 20.0 * g + 5.0) / 4.0;
+  constexpr int S = mvd::MAXSTAGE;
   o->nstage = 3;
   o->A[0] = g;
-  o->A[4] = t2 - g;  o->A[5] = g;
-  o->A[8] = b1;      o->A[9] = b2;  o->A[10] = g;
+  o->A[S] = t2 - g;      o->A[S + 1] = g;
+  o->A[2 * S] = b1;      o->A[2 * S + 1] = b2;  o->A[2 * S + 2] = g;
   o->c[0] = g; o->c[1] = t2; o->c[2] = 1.0;
   o->rtol = 1e-5;            // the reference's chemistry test input (input_primordial_ode.txt)
   o->atol = 1e-9;
@@ -555,6 +556,51 @@ int mvd_dirk_sdirk3(mvd_dirk_opts *o) {
   o->max_iters = 5;
   o->max_setups = 2;
   o->rhs_scale = 1.0;
+  return MVD_SUCCESS;
+}
+
+// ESDIRK part of ARK4(3)7L[2]SA (Kennedy & Carpenter, Appl. Numer. Math. 136 (2019); ARKODE's
+// ARK437L2SA_DIRK_7_3_4), restated from the published method: gamma = 1235/10000, rows 2-6
+// and b_3, b_4 as the published rationals; b_5 and b_6 (to 40 digits below) follow exactly
+// from sum b = 1 and b.c = 1/2 given the rest. With them all eight fourth-order conditions
+// hold to round-off, which pins every entry (a wrong digit in any of them breaks at least
+// one). Stiffly accurate: the last row is b.
+int mvd_dirk_ark437(mvd_dirk_opts *o) {
+  if (mvd_dirk_sdirk3(o) != MVD_SUCCESS) return MVD_BAD_ARGUMENT;
+  constexpr int S = mvd::MAXSTAGE;
+  for (int k = 0; k < S * S; k++) o->A[k] = 0.0;
+  for (int k = 0; k < S; k++) o->c[k] = 0.0;
+  const double g = 1235.0 / 10000.0;
+  const double a21 = 624185399699.0 / 4186980696204.0;
+  const double a31 = 1258591069120.0 / 10082082980243.0;
+  const double a32 = -322722984531.0 / 8455138723562.0;
+  const double a41 = -436103496990.0 / 5971407786587.0;
+  const double a42 = -2689175662187.0 / 11046760208243.0;
+  const double a43 = 4431412449334.0 / 12995360898505.0;
+  const double a51 = -2207373168298.0 / 14430576638973.0;
+  const double a52 = 242511121179.0 / 3358618340039.0;
+  const double a53 = 3145666661981.0 / 7780404714551.0;
+  const double a54 = 5882073923981.0 / 14490790706663.0;
+  const double b3 = 9164257142617.0 / 17756377923965.0;
+  const double b4 = -10812980402763.0 / 74029279521829.0;
+  const double b5 = 0.2347304858901933114916721160721615843812;
+  const double b6 = 0.2717223497309537855083809988731081042653;
+  double *A = o->A;
+  A[1 * S + 0] = g;    A[1 * S + 1] = g;
+  A[2 * S + 0] = a21;  A[2 * S + 1] = a21;  A[2 * S + 2] = g;
+  A[3 * S + 0] = a31;  A[3 * S + 1] = a31;  A[3 * S + 2] = a32;  A[3 * S + 3] = g;
+  A[4 * S + 0] = a41;  A[4 * S + 1] = a41;  A[4 * S + 2] = a42;  A[4 * S + 3] = a43;
+  A[4 * S + 4] = g;
+  A[5 * S + 0] = a51;  A[5 * S + 1] = a51;  A[5 * S + 2] = a52;  A[5 * S + 3] = a53;
+  A[5 * S + 4] = a54;  A[5 * S + 5] = g;
+  A[6 * S + 0] = 0.0;  A[6 * S + 1] = 0.0;  A[6 * S + 2] = b3;   A[6 * S + 3] = b4;
+  A[6 * S + 4] = b5;   A[6 * S + 5] = b6;   A[6 * S + 6] = g;
+  o->nstage = 7;
+  for (int i = 0; i < 7; i++) {
+    double c = 0.0;
+    for (int j = 0; j <= i; j++) c += A[i * S + j];
+    o->c[i] = c;
+  }
   return MVD_SUCCESS;
 }
 
@@ -567,13 +613,22 @@ int dirk_check(const mvd_dirk_opts *o, mvd::DirkDev &d) {
       o->max_halvings < 0 || o->max_halvings > 20 || o->max_steps_between_setups < 0 ||
       o->max_steps_between_setups > 1000)
     return MVD_BAD_ARGUMENT;
+  constexpr int S = mvd::MAXSTAGE;
+  // SDIRK, or ESDIRK: a first row of zeros (explicit first stage), the rest SDIRK
+  const bool esdirk = o->nstage >= 2 && o->A[0] == 0.0;
+  const double gam = esdirk ? o->A[S + 1] : o->A[0];
+  if (!(gam > 0)) return MVD_UNSUPPORTED;
+  for (int i = 0; i < S; i++)
+    for (int j = i + 1; j < S; j++)
+      if (o->A[i * S + j] != 0.0) return MVD_BAD_ARGUMENT;     // lower triangular
   for (int i = 0; i < o->nstage; i++) {
-    if (!(o->A[i * 4 + i] > 0) || o->A[i * 4 + i] != o->A[0]) return MVD_UNSUPPORTED;  // SDIRK
-    for (int j = i + 1; j < 4; j++)
-      if (o->A[i * 4 + j] != 0.0) return MVD_BAD_ARGUMENT;
+    if (esdirk && i == 0) continue;                             // A[0] == 0 above
+    if (o->A[i * S + i] != gam) return MVD_UNSUPPORTED;
   }
   d.nstage = o->nstage;
-  for (int k = 0; k < 16; k++) d.A[k] = o->A[k];
+  for (int k = 0; k < S * S; k++) d.A[k] = o->A[k];
+  d.esdirk = esdirk ? 1 : 0;
+  d.gamma = gam;
   d.rtol = o->rtol; d.atol = o->atol; d.ntol = o->newton_tol;
   d.max_iters = o->max_iters; d.max_setups = o->max_setups;
   d.rhs_scale = o->rhs_scale;
@@ -650,7 +705,7 @@ int mvd_chem_dirk_step_noalloc(mvd_context *ctx, double, double h, const mvd_dir
   int *cnt = nullptr;
   rc = mvd::dirk_batched_launch_noalloc(ctx->chem, d, h, y, ctx->stream, default_waves(ctx),
                                         ctx->dirk_block, nullptr, &cnt);
-  if (rc == MVD_BAD_ARGUMENT) return rc;
+  if (rc == MVD_BAD_ARGUMENT || rc == MVD_UNSUPPORTED) return rc;
   if (rc) return MVD_DEVICE_ERROR;
   int *hc = ctx->chem.host_counters;                       // pinned, from mvd_chem_setup
   if (!dev_ok(hipMemcpyAsync(hc, cnt + 1, sizeof(int) * mvd::NSTAT, hipMemcpyDeviceToHost,
@@ -788,9 +843,10 @@ int mvd_chem_dirk_adjoint(mvd_context *ctx, double, double h, int nsteps, int ck
   int rc = dirk_check(o, d);
   if (rc) return rc;
   if (d.max_halvings > 0) return MVD_UNSUPPORTED;     // the adjoint replays fixed steps
+  if (d.esdirk || d.nstage > mvd::ADJ_MAXSTAGE) return MVD_UNSUPPORTED;
   // stiffly accurate tables only (y_{n+1} is the last stage): c_s = sum_j a_sj = 1
   double cs = 0.0;
-  for (int j = 0; j < d.nstage; j++) cs += d.A[(d.nstage - 1) * 4 + j];
+  for (int j = 0; j < d.nstage; j++) cs += d.A[(d.nstage - 1) * mvd::MAXSTAGE + j];
   if (std::fabs(cs - 1.0) > 1e-12) return MVD_BAD_ARGUMENT;
   const size_t need =
       mvd::chem_adjoint_scratch_doubles(ctx->chem.n, nsteps, ckpt_every, d.nstage);

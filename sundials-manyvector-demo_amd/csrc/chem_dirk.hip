@@ -228,7 +228,7 @@ __device__ inline double dirk_coef(const DirkDev &o, int st, int j) {
   double v = 0.0;
 #pragma unroll
   for (int q = 1; q < MAXSTAGE; q++)
-    if (q == st) v = o.A[q * 4 + j];
+    if (q == st) v = o.A[q * MAXSTAGE + j];
   return v;
 }
 
@@ -367,14 +367,18 @@ __device__ inline void lu_solve_regs(const double (&a)[NSP][NSP], unsigned long 
   }
 }
 
-// Stage derivatives of a 64-cell chunk, element-major (F_j[s] of lane l at (j*10+s)*64 + l).
-constexpr int STAGE_WORDS = (MAXSTAGE - 1) * NSP;
 // Per-cell values a wave keeps in LDS, element-major ([word][64 lanes]): scale, inv_scale,
 // mdensity, inv_mdensity, the two optical-depth factors, the stage derivatives F_j of the
-// first nstage - 1 stages and the (sub)step start y_n.
+// first nstage - 1 stages (F_j[s] of lane l at (j*10+s)*64 + l; for tables of more than
+// FST_LDS_MAX stages they live in a global buffer instead, [(j*10+s)][thread of the grid],
+// so that four waves still share a block) and the (sub)step start y_n.
 constexpr int CELL_WORDS = 2 * NSP + 4;
+constexpr int FST_LDS_MAX = 4;
+__host__ __device__ inline int dirk_fst_lds_words(int nstage) {
+  return nstage <= FST_LDS_MAX ? (nstage - 1) * NSP : 0;
+}
 __host__ __device__ inline int dirk_wave_words(int nstage) {
-  return CELL_WORDS + (nstage - 1) * NSP + NSP;
+  return CELL_WORDS + dirk_fst_lds_words(nstage) + NSP;
 }
 // dynamic LDS of the batched kernel for `wpb` waves per block
 inline size_t dirk_lds_bytes(int wpb, int nstage) {
@@ -405,11 +409,14 @@ int dirk_fit_block(int block, int nstage, size_t static_lds, size_t limit) {
 // its state and temperature staying in registers, every step exactly as a launch of its
 // own would run it (a failed step counts and the next starts from the state it kept).
 // With nsteps > 1 the statistics are per cell over all its steps (failures: failed steps).
-template <bool FORCED, bool SUBSTEP>
+// WIDE: tables with an explicit first stage (ESDIRK) or more than FST_LDS_MAX stages (the
+// stage derivatives then in the global buffer fglob); the other instantiations are the
+// plain SDIRK form, without that code.
+template <bool FORCED, bool SUBSTEP, bool WIDE>
 __global__ void __launch_bounds__(256)
 chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double *y,
                   const double *force, const int *order, const double2 *tdt, int *queue,
-                  int *rows, int nsteps) {
+                  int *rows, int nsteps, double *fglob) {
 #pragma clang fp contract(off)
   // block statistics in LDS (the per-cell updates are LDS atomics; one row per block).
   // Dynamic LDS: the gamma tables of the temperature solve (32 KB; the ten-step Newton loop
@@ -424,8 +431,17 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
   const GammaTabs gt = gamma_tabs(lds);
   const int lane = threadIdx.x & 63;
   double *const cst = lds + 4 * TLEN + (threadIdx.x >> 6) * (dirk_wave_words(o.nstage) * 64);
-  double *const Fst = cst + CELL_WORDS * 64 + lane;        // F_j[s] at (j*10 + s)*64
-  double *const yst = Fst + (o.nstage - 1) * NSP * 64;     // y_n[s] at s*64
+  // F_j[s] at (j*10 + s) * fstr: in LDS (fstr 64) or in the global buffer (one column per
+  // thread of the grid)
+  long fstr = 64;
+  double *Fst = cst + CELL_WORDS * 64 + lane;
+  if constexpr (WIDE) {
+    if (fglob) {
+      fstr = (long)gridDim.x * blockDim.x;
+      Fst = fglob + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    }
+  }
+  double *const yst = cst + (CELL_WORDS + dirk_fst_lds_words(o.nstage)) * 64 + lane;  // y_n[s] at s*64
   // The first chunk of each wave is its global wave index; later ones come from the queue
   // (which starts after them), each fetched while the previous chunk is being solved: lane
   // 0 issues the atomic after the chunk's prologue loads and the wave reads its result at
@@ -483,7 +499,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
     int lsetup = 0;                                        // step of the last Jacobian setup
     bool fset = false;                                     // this step failed
     double hs = h;                                         // current (sub)step
-    double hg = h * o.A[0];                                // SDIRK: one diagonal value
+    double hg = h * o.gamma;                               // one implicit diagonal value
     int halv = 0, done = 0;                                // substep level, units done
     unsigned long long pv = 0;
     for (;;) {
@@ -501,6 +517,20 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
       double f[NSP];
       cell_rhs<true>(cc, gt, c, Ts, dTs, z, o.rhs_scale, f, !solved);
       solved = false;
+      if (WIDE && o.esdirk && st == 0) {
+        // explicit first stage (ESDIRK): its value is y_n and F_0 = f(y_n), this pass's
+        // RHS; the first implicit stage's r = y_n + hs a_10 F_0. The pass goes on as that
+        // stage's setup or first Newton iteration, with this f at its predictor y_n.
+#pragma unroll
+        for (int s = 0; s < NSP; s++) {
+          const double F0 = FORCED ? f[s] + force[NSP * i + s] : f[s];
+          Fst[s * fstr] = F0;
+          double a = 0.0;
+          a += dirk_coef(o, 1, 0) * F0;
+          r[s] = yst[s * 64] + hs * a;
+        }
+        st = 1;
+      }
       if (setup) {
         cell_matrix_regs(cc, c, Ts, dTs, z, hg, o.rhs_scale, lu);
         pv = lu_factor_regs(lu);
@@ -548,13 +578,13 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
         } else {
           // F_{st-1} = (z - r)/hg (ARKODE's implicit-stage derivative); the next stage's r
 #pragma unroll
-          for (int s = 0; s < NSP; s++) Fst[((st - 1) * NSP + s) * 64] = (z[s] - r[s]) / hg;
+          for (int s = 0; s < NSP; s++) Fst[((st - 1) * NSP + s) * fstr] = (z[s] - r[s]) / hg;
 #pragma unroll
           for (int s = 0; s < NSP; s++) {
             double a = 0.0;
 #pragma unroll
             for (int j = 0; j < MAXSTAGE - 1; j++)
-              if (j < st) a += dirk_coef(o, st, j) * Fst[(j * NSP + s) * 64];
+              if (j < st) a += dirk_coef(o, st, j) * Fst[(j * NSP + s) * fstr];
             r[s] = yst[s * 64] + hs * a;                   // the (sub)step start
           }
           it = 0;
@@ -571,7 +601,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
             if (halv < o.max_halvings) {
               halv++;
               hs = ldexp(h, -halv);
-              hg = hs * o.A[0];
+              hg = hs * o.gamma;
 #pragma unroll
               for (int s = 0; s < NSP; s++) r[s] = z[s] + hs * 0.0;
               st = 0;
@@ -603,7 +633,7 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
         setup = fset || halv > 0 || nst - lsetup >= o.msbp;
         fset = false;
         hs = h;
-        hg = h * o.A[0];
+        hg = h * o.gamma;
         halv = 0;
         done = 0;
       }
@@ -985,12 +1015,12 @@ tord_scatter(long n, const unsigned short *key, int *cnt, int *order) {
 // running offsets | keys | temperatures at y_n).
 struct BatchedPlan {
   void (*kern)(long, ChemConst, ChemPtrsD, DirkDev, double, double *, const double *,
-               const int *, const double2 *, int *, int *, int);
+               const int *, const double2 *, int *, int *, int, double *);
   int block, wpb;
   long grid, nrow;
   bool sorted;
-  size_t b_rows, b_ord, b_cnt, b_key, b_tdt;
-  size_t bytes() const { return b_rows + b_ord + b_cnt + b_key + b_tdt; }
+  size_t b_rows, b_ord, b_cnt, b_key, b_tdt, b_fst;
+  size_t bytes() const { return b_rows + b_ord + b_cnt + b_key + b_tdt + b_fst; }
 };
 
 static int batched_plan(long n, const DirkDev &o, int grid_waves, int block, bool forced,
@@ -999,8 +1029,16 @@ static int batched_plan(long n, const DirkDev &o, int grid_waves, int block, boo
   if (block != 64 && block != 128 && block != 256) return -1;
   if (o.nstage < 1 || o.nstage > MAXSTAGE) return -1;
   const bool sub = o.max_halvings > 0;
-  p.kern = forced ? (sub ? chem_dirk_batched<true, true> : chem_dirk_batched<true, false>)
-                  : (sub ? chem_dirk_batched<false, true> : chem_dirk_batched<false, false>);
+  if (o.esdirk || o.nstage > FST_LDS_MAX)
+    p.kern = forced ? (sub ? chem_dirk_batched<true, true, true>
+                           : chem_dirk_batched<true, false, true>)
+                    : (sub ? chem_dirk_batched<false, true, true>
+                           : chem_dirk_batched<false, false, true>);
+  else
+    p.kern = forced ? (sub ? chem_dirk_batched<true, true, false>
+                           : chem_dirk_batched<true, false, false>)
+                    : (sub ? chem_dirk_batched<false, true, false>
+                           : chem_dirk_batched<false, false, false>);
   // blocks whose LDS would not fit are split: the kernel's static LDS as compiled and the
   // device's per-block limit, both asked of the runtime
   hipFuncAttributes fa{};
@@ -1023,6 +1061,10 @@ static int batched_plan(long n, const DirkDev &o, int grid_waves, int block, boo
   p.b_cnt = p.sorted ? align256((size_t)2 * TORD_NBK * TORD_PAD * sizeof(int)) : 0;
   p.b_key = p.sorted ? align256((size_t)n * sizeof(unsigned short)) : 0;
   p.b_tdt = p.sorted ? align256((size_t)n * sizeof(double2)) : 0;
+  // stage derivatives of tables too long for LDS: one column per thread of the grid
+  p.b_fst = o.nstage > FST_LDS_MAX
+                ? align256((size_t)p.grid * p.block * (o.nstage - 1) * NSP * sizeof(double))
+                : 0;
   return 0;
 }
 
@@ -1054,9 +1096,11 @@ static int batched_enqueue(const BatchedPlan &p, ChemState &s, const DirkDev &o,
     hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(2 + NSTAT), counters, 0);
   const unsigned lds = (unsigned)dirk_lds_bytes(p.wpb, o.nstage);
   const bool sub = o.max_halvings > 0;
+  double *fglob =
+      p.b_fst ? (double *)(work + p.b_rows + p.b_ord + p.b_cnt + p.b_key + p.b_tdt) : nullptr;
   hipLaunchKernelGGL(p.kern, dim3((unsigned)p.grid), dim3(p.block), lds, st, n, s.cc,
                      ptrs_d(s), o, h, y, force, (const int *)order,
-                     sub ? nullptr : (const double2 *)tdt, counters, rows, nsteps);
+                     sub ? nullptr : (const double2 *)tdt, counters, rows, nsteps, fglob);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1113,6 +1157,7 @@ int dirk_batched_launch_noalloc(ChemState &s, const DirkDev &o, double h, double
                                 hipStream_t st, int grid_waves, int block, const double *force,
                                 int **counters_out) {
   if (s.n > NOALLOC_MAX_CELLS) return MVD_BAD_ARGUMENT;
+  if (o.nstage > FST_LDS_MAX) return MVD_UNSUPPORTED;     // no room for its stage buffer
   BatchedPlan p;
   if (batched_plan(s.n, o, grid_waves, block, force != nullptr, p)) return -1;
   if (p.nrow > NOALLOC_MAX_ROWS || p.bytes() > NOALLOC_WORK) return MVD_BAD_ARGUMENT;
@@ -1153,14 +1198,18 @@ int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStrea
   // temperature at y_n, Jacobian at y_n, factor (the first stage's a_ii)
   hipLaunchKernelGGL(plain_rhs, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, o.rhs_scale, z, f,
                      active, PLAIN_UNCONVERGED);
-  hipLaunchKernelGGL(plain_jac, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, h * o.A[0],
+  // an explicit first stage (ESDIRK) keeps that RHS as F_0
+  if (o.esdirk)
+    hipLaunchKernelGGL(copy_double, dim3(nb(n * NSP)), dim3(256), 0, st, n * NSP,
+                       (const double *)f, acc);
+  hipLaunchKernelGGL(plain_jac, dim3(nb(n)), dim3(256), 0, st, n, s.cc, p, h * o.gamma,
                      o.rhs_scale, z, LU, active);
   hipLaunchKernelGGL(plain_getrf, dim3(nb(n)), dim3(256), 0, st, n, LU, piv, active);
   hipLaunchKernelGGL(count_setups, dim3(nb(n)), dim3(256), 0, st, n, active, csetups);
   hipLaunchKernelGGL(plain_resume, dim3(nb(n)), dim3(256), 0, st, n, active, crate);
   int *hcnt = s.host_counters;
-  for (int stg = 0; stg < o.nstage; stg++) {
-    const double hg = h * o.A[stg * 4 + stg];
+  for (int stg = o.esdirk ? 1 : 0; stg < o.nstage; stg++) {
+    const double hg = h * o.A[stg * MAXSTAGE + stg];
     hipLaunchKernelGGL(plain_stage_init, dim3(nb(n)), dim3(256), 0, st, n, stg, h, o, yn, acc, r,
                        active, failed);
     int setups = 0;
@@ -1292,7 +1341,7 @@ __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, doub
                           const DirkDev &o, double h, double (&y)[NSP], double *Ys, long n,
                           long i) {
 #pragma clang fp contract(off)
-  double yn[NSP], z[NSP], f[NSP], F[MAXSTAGE][NSP];
+  double yn[NSP], z[NSP], f[NSP], F[ADJ_MAXSTAGE][NSP];
 #pragma unroll
   for (int s = 0; s < NSP; s++) yn[s] = z[s] = y[s];
   cell_rhs<true>(cc, c, Ts, dTs, yn, o.rhs_scale, f);      // refresh Ts at y_n for J
@@ -1301,9 +1350,9 @@ __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, doub
   bool ok = true;
   double crate = 1.0, delp = 0.0;                     // Newton rate estimate, last norm
 #pragma unroll
-  for (int st = 0; st < MAXSTAGE; st++) {
+  for (int st = 0; st < ADJ_MAXSTAGE; st++) {
     if (st < o.nstage && ok) {
-      const double hg = h * o.A[st * 4 + st];
+      const double hg = h * o.A[st * MAXSTAGE + st];
       if (st == 0) {
         cell_matrix_regs(cc, c, Ts, dTs, yn, hg, o.rhs_scale, a);
         pv = lu_factor_regs(a);
@@ -1313,7 +1362,7 @@ __device__ bool cell_step(const ChemConst &cc, const CellIn &c, double &Ts, doub
       for (int s = 0; s < NSP; s++) {
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < st; j++) acc += o.A[st * 4 + j] * F[j][s];
+        for (int j = 0; j < st; j++) acc += o.A[st * MAXSTAGE + j] * F[j][s];
         r[s] = yn[s] + h * acc;
       }
       // the stage's Newton iterations; after max_iters or a growing correction, a setup at
@@ -1450,13 +1499,13 @@ chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, in
     for (int st = n1 - 1; st >= n0; st--) {
       const double Ts = Te[(long)(st - n0) * n + u];
       const double *Yst = Ys + (long)(st - n0) * ns * NSP * n;
-      double nu[MAXSTAGE][NSP];
+      double nu[ADJ_MAXSTAGE][NSP];
 #pragma unroll
-      for (int k = 0; k < MAXSTAGE; k++)
+      for (int k = 0; k < ADJ_MAXSTAGE; k++)
 #pragma unroll
         for (int s = 0; s < NSP; s++) nu[k][s] = 0.0;
 #pragma unroll
-      for (int k = MAXSTAGE - 1; k >= 0; k--) {
+      for (int k = ADJ_MAXSTAGE - 1; k >= 0; k--) {
         if (k < ns) {
           double w[NSP], Y[NSP];
 #pragma unroll
@@ -1464,13 +1513,13 @@ chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, in
             double acc = dirk_coef(o, ns - 1, k) * lam[s];
             if (ns - 1 == 0) acc = o.A[k] * lam[s];
 #pragma unroll
-            for (int m = k + 1; m < MAXSTAGE; m++)
+            for (int m = k + 1; m < ADJ_MAXSTAGE; m++)
               if (m < ns) acc += dirk_coef(o, m, k) * nu[m][s];
             w[s] = acc;
             Y[s] = Yst[(long)(k * NSP + s) * n + u];
           }
           double Mt[NSP][NSP], yb[NSP];
-          stage_adjoint_matrix(cc, c, Ts, Y, o.rhs_scale, h, h * o.A[k * 4 + k], w, Mt, yb);
+          stage_adjoint_matrix(cc, c, Ts, Y, o.rhs_scale, h, h * o.A[k * MAXSTAGE + k], w, Mt, yb);
           const unsigned long long pv = lu_factor_regs(Mt);
           lu_solve_regs(Mt, pv, yb);
 #pragma unroll
@@ -1478,7 +1527,7 @@ chem_adjoint_backward(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, in
         }
       }
 #pragma unroll
-      for (int k = 0; k < MAXSTAGE; k++)
+      for (int k = 0; k < ADJ_MAXSTAGE; k++)
         if (k < ns)
 #pragma unroll
           for (int s = 0; s < NSP; s++) lam[s] += nu[k][s];

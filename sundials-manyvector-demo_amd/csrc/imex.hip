@@ -77,8 +77,10 @@ int imex_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, I
   // the chemistry solves: backward Euler form z - hg f(z) = R_i
   DirkDev be = newton;
   be.nstage = 1;
-  for (int k = 0; k < 16; k++) be.A[k] = 0.0;
+  for (int k = 0; k < MAXSTAGE * MAXSTAGE; k++) be.A[k] = 0.0;
   be.A[0] = 1.0;
+  be.esdirk = 0;
+  be.gamma = 1.0;
   for (int q = 0; q < 5; q++)
     if (hipMemcpyAsync(Yn + q * n, w[q], sizeof(double) * n, hipMemcpyDeviceToDevice, st) !=
         hipSuccess)

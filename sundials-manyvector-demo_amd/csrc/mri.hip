@@ -121,7 +121,7 @@ int enqueue_slow_step(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_
 bool same_opts(const DirkDev &a, const DirkDev &b) {
   if (a.nstage != b.nstage || a.max_iters != b.max_iters || a.max_setups != b.max_setups)
     return false;
-  for (int k = 0; k < 16; k++)
+  for (int k = 0; k < MAXSTAGE * MAXSTAGE; k++)
     if (a.A[k] != b.A[k]) return false;
   return a.rtol == b.rtol && a.atol == b.atol && a.ntol == b.ntol &&
          a.rhs_scale == b.rhs_scale && a.max_halvings == b.max_halvings && a.msbp == b.msbp;

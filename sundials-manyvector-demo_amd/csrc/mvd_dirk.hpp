@@ -5,7 +5,8 @@
 
 namespace mvd {
 
-constexpr int MAXSTAGE = 4;   // stages of the DIRK tables supported (A is 4x4)
+constexpr int MAXSTAGE = 7;   // stages of the DIRK tables supported (A is 7x7)
+constexpr int ADJ_MAXSTAGE = 4;   // stages of the SDIRK tables the adjoint supports
 constexpr int NHIST = 16;     // Newton-iteration histogram bins (last bin: >= 15)
 
 // statistics words in ChemState::counters (after the work-queue counter)
@@ -19,7 +20,9 @@ struct ChemPtrsD {
 // Method and Newton controls, by value in kernel arguments.
 struct DirkDev {
   int nstage;
-  double A[16];
+  double A[MAXSTAGE * MAXSTAGE];   // row-major
+  int esdirk;           // explicit first stage (z_1 = y_n, F_1 = f(y_n))
+  double gamma;         // the implicit stages' diagonal entry
   double rtol, atol, ntol;
   int max_iters, max_setups;
   double rhs_scale;

@@ -31,9 +31,12 @@ class Params(C.Structure):
                 ("length_units", C.c_double), ("time_units", C.c_double)]
 
 
+DIRK_MAXSTAGE = 7
+
+
 class DirkOpts(C.Structure):
-    """mvd_dirk_opts: SDIRK table (row-major 4x4 A) and Newton controls."""
-    _fields_ = [("nstage", C.c_int), ("A", C.c_double * 16), ("c", C.c_double * 4),
+    """mvd_dirk_opts: SDIRK or ESDIRK table (row-major 7x7 A) and Newton controls."""
+    _fields_ = [("nstage", C.c_int), ("A", C.c_double * 49), ("c", C.c_double * 7),
                 ("rtol", C.c_double), ("atol", C.c_double), ("newton_tol", C.c_double),
                 ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double),
                 ("max_halvings", C.c_int), ("max_steps_between_setups", C.c_int)]
@@ -118,6 +121,7 @@ def lib():
         "mvd_exchange": ([vp, _PP, _PP], C.c_int),
         "mvd_pack1d": ([vp, _PP, C.c_int, C.c_long, C.c_void_p, _P], C.c_int),
         "mvd_dirk_sdirk3": ([C.POINTER(DirkOpts)], C.c_int),
+        "mvd_dirk_ark437": ([C.POINTER(DirkOpts)], C.c_int),
         "mvd_chem_dirk_step": ([vp, C.c_double, C.c_double, C.POINTER(DirkOpts), _P, C.c_int,
                                 C.POINTER(DirkStats)], C.c_int),
         "mvd_chem_dirk_tune": ([vp, C.c_double, C.POINTER(DirkOpts), _P, _P], C.c_int),
@@ -623,6 +627,32 @@ def sdirk3(**kw):
     _check("mvd_dirk_sdirk3", lib().mvd_dirk_sdirk3(C.byref(o)))
     for k, v in kw.items():
         setattr(o, k, v)
+    return o
+
+
+def ark437(**kw):
+    """The ESDIRK part of ARK4(3)7L[2]SA (the reference inputs' implicit table, ARKODE's
+    ARK437L2SA_DIRK_7_3_4) with the default Newton controls; keyword overrides."""
+    o = DirkOpts()
+    _check("mvd_dirk_ark437", lib().mvd_dirk_ark437(C.byref(o)))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def set_table(o, A, c=None):
+    """Puts a square Butcher matrix (up to 7x7, rows as sequences) and its nodes (default:
+    the row sums) into DirkOpts o."""
+    s = len(A)
+    for k in range(DIRK_MAXSTAGE * DIRK_MAXSTAGE):
+        o.A[k] = 0.0
+    for i in range(s):
+        for j in range(s):
+            o.A[DIRK_MAXSTAGE * i + j] = float(A[i][j])
+    cc = [sum(float(x) for x in A[i]) for i in range(s)] if c is None else list(c)
+    for k in range(DIRK_MAXSTAGE):
+        o.c[k] = float(cc[k]) if k < s else 0.0
+    o.nstage = s
     return o
 
 

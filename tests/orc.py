@@ -33,7 +33,7 @@ def make_grid(nx, ny, nz, nchem=0, bc=(0,) * 6, gamma=1.4, extent=(1.0, 1.0, 1.0
 
 class Dirk(C.Structure):
     """orc_dirk (same layout as the library's mvd_dirk_opts)."""
-    _fields_ = [("nstage", C.c_int), ("A", C.c_double * 16), ("c", C.c_double * 4),
+    _fields_ = [("nstage", C.c_int), ("A", C.c_double * 49), ("c", C.c_double * 7),
                 ("rtol", C.c_double), ("atol", C.c_double), ("newton_tol", C.c_double),
                 ("max_iters", C.c_int), ("max_setups", C.c_int), ("rhs_scale", C.c_double),
                 ("max_halvings", C.c_int), ("max_steps_between_setups", C.c_int)]
@@ -53,6 +53,40 @@ def sdirk3_table():
     A[1, 0], A[1, 1] = c2 - g, g
     A[2, 0], A[2, 1], A[2, 2] = b1, b2, g
     return A, np.array([g, c2, 1.0, 0.0])
+
+
+def ark437_table():
+    """The ESDIRK part of ARK4(3)7L[2]SA (Kennedy & Carpenter 2019), independently of the
+    library: gamma = 1235/10000 and the published rationals of rows 2-6 and b_3, b_4, with
+    b_5, b_6 solved exactly (fractions) from sum b = 1 and b.c = 1/2. Returns the 7x7 A
+    (stiffly accurate: last row b) and the published nodes c."""
+    from fractions import Fraction as F
+    g = F(1235, 10000)
+    A = [[F(0)] * 7 for _ in range(7)]
+    A[1][0] = A[1][1] = g
+    A[2][0] = A[2][1] = F(624185399699, 4186980696204)
+    A[2][2] = g
+    A[3][0] = A[3][1] = F(1258591069120, 10082082980243)
+    A[3][2] = F(-322722984531, 8455138723562)
+    A[3][3] = g
+    A[4][0] = A[4][1] = F(-436103496990, 5971407786587)
+    A[4][2] = F(-2689175662187, 11046760208243)
+    A[4][3] = F(4431412449334, 12995360898505)
+    A[4][4] = g
+    A[5][0] = A[5][1] = F(-2207373168298, 14430576638973)
+    A[5][2] = F(242511121179, 3358618340039)
+    A[5][3] = F(3145666661981, 7780404714551)
+    A[5][4] = F(5882073923981, 14490790706663)
+    A[5][5] = g
+    c = [sum(r) for r in A[:6]] + [F(1)]
+    b3, b4 = F(9164257142617, 17756377923965), F(-10812980402763, 74029279521829)
+    s1 = 1 - g - b3 - b4
+    s2 = F(1, 2) - g - b3 * c[2] - b4 * c[3]
+    b6 = (s2 - c[4] * s1) / (c[5] - c[4])
+    b5 = s1 - b6
+    A[6] = [F(0), F(0), b3, b4, b5, b6, g]
+    published_c = [0.0, 0.247, 4276536705230 / 10142255878289, 67 / 200, 3 / 40, 7 / 10, 1.0]
+    return np.array([[float(x) for x in r] for r in A]), np.array(published_c)
 
 
 class Erk(C.Structure):
@@ -118,9 +152,12 @@ def dirk_opts(A=None, c=None, nstage=3, rtol=1e-5, atol=1e-9, newton_tol=0.1, ma
         A, c = sdirk3_table()
     o = Dirk()
     o.nstage = nstage
-    for k, v in enumerate(np.asarray(A, dtype=np.float64).reshape(-1)):
+    A = np.asarray(A, dtype=np.float64)
+    A7 = np.zeros((7, 7))
+    A7[:A.shape[0], :A.shape[1]] = A
+    for k, v in enumerate(A7.reshape(-1)):
         o.A[k] = v
-    for k, v in enumerate(c):
+    for k, v in enumerate(list(c)[:7]):
         o.c[k] = v
     o.rtol, o.atol, o.newton_tol = rtol, atol, newton_tol
     o.max_iters, o.max_setups, o.rhs_scale = max_iters, max_setups, rhs_scale

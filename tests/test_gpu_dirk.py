@@ -87,12 +87,7 @@ def test_dirk_four_stage_table(mvd, torch_gpu):
     _, _, ctx_p, y_plain = setup(mvd, torch, n, 7)
     failed_o, iters_o = orc.dirk_step(oc, orc.dirk_opts(A=A, c=c, nstage=4), 1e3, yo)
     ok = iters_o >= 0
-    o = mvd.sdirk3()
-    o.nstage = 4
-    for k, v in enumerate(A.reshape(-1)):
-        o.A[k] = v
-    for k, v in enumerate(c):
-        o.c[k] = v
+    o = mvd.set_table(mvd.sdirk3(), A, c)
     failed_b, st_b = ctx.dirk_step(yd, 1e3, o)
     failed_p, st_p = ctx_p.dirk_step(y_plain, 1e3, o, plain=True)
     assert failed_b == failed_o == failed_p
@@ -101,6 +96,53 @@ def test_dirk_four_stage_table(mvd, torch_gpu):
     assert st_b["hist"] == st_p["hist"]
     ctx.close()
     ctx_p.close()
+
+
+@pytest.mark.parametrize("n,seed,h", [(4096, 51, 1e3), (2048, 52, 1e5), (1, 53, 1e3)])
+def test_dirk_esdirk_batched_and_plain_match_oracle(mvd, torch_gpu, n, seed, h):
+    """The reference inputs' implicit table, ARK4(3)7L[2]SA's ESDIRK part (seven stages, an
+    explicit first stage): the batched kernel (its stage derivatives in the global buffer),
+    the plain path and the oracle agree as for SDIRK3, with the same Newton work."""
+    torch = torch_gpu
+    print(f"[dirk esdirk] n={n} seed={seed} h={h}", flush=True)
+    oc, yo, ctx, yd = setup(mvd, torch, n, seed)
+    _, _, ctx_p, y_plain = setup(mvd, torch, n, seed)
+    A, c = orc.ark437_table()
+    failed_o, iters_o = orc.dirk_step(oc, orc.dirk_opts(A=A, c=c, nstage=7), h, yo)
+    ok = iters_o >= 0
+    o = mvd.ark437()
+    failed_b, st_b = ctx.dirk_step(yd, h, o)
+    failed_p, st_p = ctx_p.dirk_step(y_plain, h, o, plain=True)
+    assert failed_b == failed_o == failed_p
+    assert compare(yd.cpu().numpy(), yo, ok) <= TOL
+    assert compare(y_plain.cpu().numpy(), yo, ok) <= TOL
+    Ts_b = ctx.chem_state("Ts", torch).cpu().numpy()
+    assert np.max(np.abs(Ts_b[ok] - oc.array("Ts")[ok]) / oc.array("Ts")[ok]) <= TOL
+    if failed_o == 0:
+        hist_o = np.bincount(np.minimum(iters_o, 15), minlength=16)
+        assert st_b["newton_iters"] == st_p["newton_iters"] == int(iters_o.sum())
+        assert list(hist_o) == st_b["hist"] == st_p["hist"]
+    # the explicit stage takes no Newton iteration: six implicit stages per cell
+    assert st_b["newton_iters"] >= 6 * (n - failed_o)
+    ctx.close()
+    ctx_p.close()
+
+
+def test_dirk_esdirk_rejected_where_unsupported(mvd, torch_gpu):
+    """The adjoint replays SDIRK tables of up to four stages only, and the allocation-free
+    entry point has no room for the stage buffer of a seven-stage table."""
+    torch = torch_gpu
+    n = 64
+    _, _, ctx, yd = setup(mvd, torch, n, 7)
+    lam = torch.zeros_like(yd)
+    lam0 = torch.empty_like(yd)
+    with pytest.raises(mvd.MvdError) as e:
+        ctx.dirk_adjoint(yd, 100.0, 2, 1, lam, lam0, mvd.ark437())
+    assert e.value.code == mvd.MVD_UNSUPPORTED
+    with pytest.raises(mvd.MvdError) as e:
+        ctx.dirk_step_noalloc(yd, 1e3, mvd.ark437())
+    assert e.value.code == mvd.MVD_UNSUPPORTED
+    ctx.close()
 
 
 def test_dirk_reports_newton_failure(mvd, torch_gpu):
@@ -120,7 +162,12 @@ def test_dirk_rejects_bad_options(mvd, torch_gpu):
     torch = torch_gpu
     _, _, ctx, yd = setup(mvd, torch, 64, 7)
     o = mvd.sdirk3()
-    o.A[5] = 0.3                                      # unequal diagonal: not SDIRK
+    o.A[mvd.DIRK_MAXSTAGE + 1] = 0.3                  # unequal diagonal: not SDIRK
+    with pytest.raises(mvd.MvdError) as e:
+        ctx.dirk_step(yd, 1e3, opts=o)
+    assert e.value.code == mvd.MVD_UNSUPPORTED
+    o = mvd.ark437()
+    o.A[3 * mvd.DIRK_MAXSTAGE + 3] = 0.2              # ESDIRK, unequal implicit diagonal
     with pytest.raises(mvd.MvdError) as e:
         ctx.dirk_step(yd, 1e3, opts=o)
     assert e.value.code == mvd.MVD_UNSUPPORTED
@@ -151,7 +198,7 @@ def test_dirk_tune_keeps_results(mvd, torch_gpu):
     assert s1["newton_iters"] == s2["newton_iters"]
 
 
-@pytest.mark.parametrize("variant", ["plain", "forced", "substep"])
+@pytest.mark.parametrize("variant", ["plain", "forced", "substep", "esdirk"])
 def test_dirk_work_queue_matches_default_shape(mvd, torch_gpu, variant):
     """8192 cells are 128 chunks of 64: on 16 resident waves each wave takes its first chunk
     by index and ~7 more from the atomic work queue (at the default one wave per SIMD every
@@ -172,7 +219,9 @@ def test_dirk_work_queue_matches_default_shape(mvd, torch_gpu, variant):
     yo = y.copy()
     oc.prepare(yo)
     oc.extra()
-    oo = orc.dirk_opts(max_halvings=mh, **kw)
+    A7, c7 = orc.ark437_table()
+    tab = dict(A=A7, c=c7, nstage=7) if variant == "esdirk" else {}
+    oo = orc.dirk_opts(max_halvings=mh, **kw, **tab)
     if force is not None:
         fo = orc.dirk_step_forced(oc, oo, hstep, yo, force)
         iters_o = None
@@ -187,7 +236,7 @@ def test_dirk_work_queue_matches_default_shape(mvd, torch_gpu, variant):
         ctx.dirk_set_shape(block, waves)
         b, w = ctx.dirk_get_shape()
         assert b == block and (waves == 0 or w == waves)
-        o = mvd.sdirk3(max_halvings=mh, **kw)
+        o = (mvd.ark437 if variant == "esdirk" else mvd.sdirk3)(max_halvings=mh, **kw)
         if force is not None:
             fd = ctx.dirk_step_forced(yd, hstep, torch.from_numpy(force).to("cuda"), o)
             st = None
@@ -263,7 +312,8 @@ def test_dirk_steps_equal_repeated_steps(mvd, torch_gpu, variant):
     assert np.array_equal(y1, y0) and np.array_equal(T1, T0) and np.array_equal(d1, d0)
 
 
-@pytest.mark.parametrize("variant", ["plain", "forced", "substep", "failing"])
+@pytest.mark.parametrize("variant", ["plain", "forced", "substep", "failing", "esdirk-forced",
+                                     "esdirk-substep"])
 def test_dirk_steps_setup_reuse_matches_oracle(mvd, torch_gpu, variant):
     """ARKODE's lsetup frequency within one multi-step call (max_steps_between_setups = 7
     over 16 steps): cells set up at y_n only on their first step, after a failed or split
@@ -275,6 +325,11 @@ def test_dirk_steps_setup_reuse_matches_oracle(mvd, torch_gpu, variant):
     torch = torch_gpu
     n, seed, nsteps, msbp = 4096, 43, 16, 7
     print(f"[dirk steps reuse] n={n} variant={variant} nsteps={nsteps} msbp={msbp}", flush=True)
+    esd = variant.startswith("esdirk-")
+    variant = variant[len("esdirk-"):] if esd else variant
+    A7, c7 = orc.ark437_table()
+    tab = dict(A=A7, c=c7, nstage=7) if esd else {}
+    lib_opts = mvd.ark437 if esd else mvd.sdirk3
     rng = np.random.default_rng(seed)
     force = rng.standard_normal(10 * n) * 1e-5 if variant == "forced" else None
     kw = {"plain": {}, "forced": {}, "substep": dict(max_halvings=4, max_iters=2, max_setups=0),
@@ -283,9 +338,9 @@ def test_dirk_steps_setup_reuse_matches_oracle(mvd, torch_gpu, variant):
     oc, yo, ctx, yd = setup(mvd, torch, n, seed)
     fd = torch.from_numpy(force).to("cuda") if force is not None else None
     failed_d, st = ctx.dirk_steps(yd, h, nsteps, fd,
-                                  mvd.sdirk3(max_steps_between_setups=msbp, **kw))
-    failed_o, iters_o = orc.dirk_steps(oc, orc.dirk_opts(max_steps_between_setups=msbp, **kw), h,
-                                       nsteps, yo, force)
+                                  lib_opts(max_steps_between_setups=msbp, **kw))
+    failed_o, iters_o = orc.dirk_steps(oc, orc.dirk_opts(max_steps_between_setups=msbp, **kw,
+                                                         **tab), h, nsteps, yo, force)
     assert failed_d == failed_o
     if variant == "failing":
         assert failed_o > 0
@@ -300,7 +355,8 @@ def test_dirk_steps_setup_reuse_matches_oracle(mvd, torch_gpu, variant):
     oc1 = orc.Chem(n)
     oc1.prepare(y1)
     oc1.extra()
-    orc.dirk_steps(oc1, orc.dirk_opts(max_steps_between_setups=1, **kw), h, nsteps, y1, force)
+    orc.dirk_steps(oc1, orc.dirk_opts(max_steps_between_setups=1, **kw, **tab), h, nsteps, y1,
+                   force)
     assert not np.array_equal(y1, yo)
     ctx.close()
 

@@ -89,8 +89,76 @@ def test_library_sdirk3_table(mvd):
     A, c = orc.sdirk3_table()
     assert o.nstage == 3
     # np.roots (companion-matrix eigenvalues) is accurate to a few ulp of the coefficients
-    assert np.max(np.abs(np.array(list(o.A)).reshape(4, 4) - A)) <= 1e-14
-    assert np.max(np.abs(np.array(list(o.c)) - c)) <= 1e-14
+    LA = np.array(list(o.A)).reshape(7, 7)
+    assert np.max(np.abs(LA[:4, :4] - A)) <= 1e-14 and not LA[4:].any() and not LA[:, 4:].any()
+    assert np.max(np.abs(np.array(list(o.c))[:4] - c)) <= 1e-14
+
+
+def _order_conditions(A):
+    """Residuals of the eight fourth-order conditions of the stiffly accurate table A."""
+    b = A[-1]
+    c = A.sum(axis=1)
+    Ac = A @ c
+    return np.array([b.sum() - 1, b @ c - 1 / 2, b @ c ** 2 - 1 / 3, b @ Ac - 1 / 6,
+                     b @ c ** 3 - 1 / 4, b @ (c * Ac) - 1 / 8, b @ (A @ c ** 2) - 1 / 12,
+                     b @ (A @ Ac) - 1 / 24])
+
+
+def test_ark437_table_order_conditions_and_L_stability():
+    """ARK4(3)7L[2]SA's ESDIRK part: the published nodes are the row sums, all eight
+    fourth-order conditions hold (six of them independent of the two that fixed b_5, b_6),
+    the method is L-stable; a wrong digit in one entry breaks the conditions."""
+    A, c = orc.ark437_table()
+    assert np.allclose(A.sum(axis=1), c, rtol=0, atol=1e-15)
+    assert np.all(np.diag(A)[1:] == 0.1235) and A[0, 0] == 0.0 and not np.triu(A, 1).any()
+    assert np.max(np.abs(_order_conditions(A))) < 1e-15
+    b = A[-1]
+    for z in (-1e12, -1e8):
+        R = 1 + z * b @ np.linalg.solve(np.eye(7) - z * A, np.ones(7))
+        assert abs(R) < 1e-6
+    for z in -np.logspace(-3, 6, 50):
+        R = 1 + z * b @ np.linalg.solve(np.eye(7) - z * A, np.ones(7))
+        assert abs(R) <= 1.0 + 1e-14
+    bad = A.copy()
+    bad[4, 2] *= 1 + 1e-9                       # one entry off in its tenth digit
+    bad[4, 4] -= bad[4].sum() - A[4].sum()      # (keeping the node)
+    assert np.max(np.abs(_order_conditions(bad))) > 1e-12
+
+
+def test_library_ark437_table(mvd):
+    o = mvd.ark437()
+    A, c = orc.ark437_table()
+    assert o.nstage == 7
+    LA = np.array(list(o.A)).reshape(7, 7)
+    assert np.array_equal(LA, A)                # the same doubles (correctly rounded)
+    assert np.max(np.abs(np.array(list(o.c)) - c)) <= 1e-15
+
+
+def test_oracle_esdirk_fourth_order():
+    """Global error of N fixed ARK4(3)7L[2]SA-ESDIRK steps against N=512 steps: ratio ~ 16
+    per halving (14.6 and 15.8 here, where SDIRK3 gives ~7), which checks the oracle's
+    explicit first stage and stage sums; on the warm gas of the third-order test."""
+    n, T = 32, 1e8
+    y0, _ = problems.chem_cells(n, seed=7, ntot=(1e-1, 1e1), temp=(3e3, 8e3))
+    A, c = orc.ark437_table()
+    opts = orc.dirk_opts(A=A, c=c, nstage=7, rtol=1e-13, atol=1e-16, newton_tol=1e-3,
+                         max_iters=30, max_setups=5)
+
+    def run(N):
+        ch = orc.Chem(n)
+        y = y0.copy()
+        ch.prepare(y)
+        ch.extra()
+        for _ in range(N):
+            failed, _ = orc.dirk_step(ch, opts, T / N, y)
+            assert failed == 0
+        return y.reshape(n, 10) * ch.array("scale").reshape(n, 10)
+
+    ref = run(512)
+    scale = np.abs(ref).max(axis=0)
+    errs = [np.max(np.abs(run(N) - ref) / scale) for N in (16, 32, 64)]
+    ratios = [errs[k] / errs[k + 1] for k in range(2)]
+    assert all(13.0 < r < 19.0 for r in ratios), (errs, ratios)
 
 
 def _prepared(n, seed, **kw):
