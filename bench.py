@@ -340,15 +340,16 @@ def bench_sod(api, problems, torch, cpu=True):
     return out
 
 
-def bench_mri(api, problems, torch, n, nslow, cpu=True, cpu_n=16):
+def bench_mri(api, problems, torch, n, nslow, cpu=True, cpu_n=16, table="ark437"):
     """Config-5 multirate primordial blast: n^3 cells, 10 species, reflecting boundaries,
     gas at rest with a x100 Gaussian overdensity of width 0.05 at 1e4 K in 10 K gas (the
     reference's species recipe), units of input_primordial_blast_mr.txt. Slow step =
     min(CFL 0.3 bound, the input's h0 = 0.05); fast steps no longer than the input's
-    hmax = 1e-4 (501 per slow step); SDIRK3 fast stepper with ffast's TimeUnits scaling and
-    per-cell step rejection; chemistry forcing converted to the scaled fast variables. The
-    oracle runs the same options on a cpu_n^3 sample for one slow step (host baseline), and
-    the device is compared with it there."""
+    hmax = 1e-4 (501 per slow step); fast stepper: the input's implicit table (itable 112,
+    ARK4(3)7L[2]SA's ESDIRK part; table="sdirk3" for Alexander's SDIRK3) with ffast's
+    TimeUnits scaling and per-cell step rejection; chemistry forcing converted to the scaled
+    fast variables. The oracle runs the same options on a cpu_n^3 sample for one slow step
+    (host baseline), and the device is compared with it there."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
     units = (3.0e70, 3.0857e30, 1.0e11)
@@ -365,7 +366,9 @@ def bench_mri(api, problems, torch, n, nslow, cpu=True, cpu_n=16):
         ctx.chem_prepare(wd[5])
         return ctx, wd, fluid, chem
 
-    fast = api.sdirk3(rhs_scale=units[2], max_halvings=10, **MRI_NEWTON)
+    make = api.ark437 if table == "ark437" else api.sdirk3
+    fast = make(rhs_scale=units[2], max_halvings=10, **MRI_NEWTON)
+    tname = "ARK4(3)7L[2]SA-ESDIRK" if table == "ark437" else "SDIRK3"
     ctx, wd, _, _ = setup(n)
     h = min(ctx.stability(wd), h0)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -379,7 +382,7 @@ def bench_mri(api, problems, torch, n, nslow, cpu=True, cpu_n=16):
     ctx.close()
     nfast = sum(api.mri_fast_steps(i, h, hmax) for i in (2, 3, 4))
     out = {"workload": f"multirate primordial blast {n}^3, 10 species, reflecting, "
-                       f"{nslow} slow steps of {h:g} (MIS Knoth-Wolke 3), {nfast} SDIRK3 fast "
+                       f"{nslow} slow steps of {h:g} (MIS Knoth-Wolke 3), {nfast} {tname} fast "
                        f"steps each (hmax {hmax:g}, Newton tol {MRI_NEWTON['newton_tol']:g} / "
                        f"{MRI_NEWTON['max_iters']} iterations, a Jacobian setup every "
                        f"{MRI_NEWTON['max_steps_between_setups']} fast steps), "
@@ -398,10 +401,14 @@ def bench_mri(api, problems, torch, n, nslow, cpu=True, cpu_n=16):
         oc.extra()
         wo = [a.copy() for a in fluid] + [y]
         g = orc.make_grid(m, m, m, 10, (3,) * 6, gamma=5 / 3)
+        tab_o = {}
+        if table == "ark437":
+            A7, c7 = orc.ark437_table()
+            tab_o = dict(A=A7, c=c7, nstage=7)
         orc.set_threads(1)
         t0 = time.perf_counter()
-        orc.mri_evolve(g, oc, eu, orc.dirk_opts(rhs_scale=units[2], max_halvings=10, **MRI_NEWTON),
-                       hs, 1, hmax, wo, consistent=True)
+        orc.mri_evolve(g, oc, eu, orc.dirk_opts(rhs_scale=units[2], max_halvings=10, **MRI_NEWTON,
+                                                **tab_o), hs, 1, hmax, wo, consistent=True)
         dt = time.perf_counter() - t0
         diffs = []
         for q in range(6):
@@ -469,6 +476,9 @@ def main():
     ap.add_argument("--no-mri", action="store_true")
     ap.add_argument("--mri-n", type=int, default=128)
     ap.add_argument("--mri-steps", type=int, default=20)
+    ap.add_argument("--mri-table", choices=["ark437", "sdirk3"], default="ark437",
+                    help="fast DIRK table of the multirate run (default: the reference "
+                         "input's itable 112, ARK4(3)7L[2]SA's ESDIRK part)")
     ap.add_argument("--only", choices=["vjp", "sod", "mri", "chem", "fe10"], default=None,
                     help="run one extra workload alone and print its JSON (profiling runs)")
     ap.add_argument("--vjp-n", type=int, default=128)
@@ -493,7 +503,7 @@ def main():
             print(json.dumps(bench_vjp(api, problems, torch, args.vjp_n, 777, args.vjp_reps)))
         elif args.only == "mri":
             print(json.dumps(bench_mri(api, problems, torch, args.mri_n, args.mri_steps,
-                                       cpu=not args.no_cpu)))
+                                       cpu=not args.no_cpu, table=args.mri_table)))
         elif args.only == "chem":
             print(json.dumps(bench_chem(api, problems, torch, args.chem_n, 4321,
                                         args.chem_steps)))
@@ -596,7 +606,7 @@ def main():
     if not args.no_mri:
         quiet_gc()
         extra["mri"] = bench_mri(api, problems, torch, args.mri_n, args.mri_steps,
-                                 cpu=not args.no_cpu)
+                                 cpu=not args.no_cpu, table=args.mri_table)
 
     cpu = None
     if not args.no_cpu:

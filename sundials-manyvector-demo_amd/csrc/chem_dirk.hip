@@ -223,11 +223,14 @@ __device__ inline bool conv_diverging(double del, double delp) {
   return del > 2.3 * delp;
 }
 
-// a_{st,j} with constant indices only (no dynamic indexing of the by-value argument)
+// a_{st,j} for a stage index held per lane, with constant indices only (no dynamic indexing
+// of the by-value argument): selected among the rows 1..QMAX-1 (QMAX: the most stages the
+// caller's kernel handles)
+template <int QMAX = MAXSTAGE>
 __device__ inline double dirk_coef(const DirkDev &o, int st, int j) {
   double v = 0.0;
 #pragma unroll
-  for (int q = 1; q < MAXSTAGE; q++)
+  for (int q = 1; q < QMAX; q++)
     if (q == st) v = o.A[q * MAXSTAGE + j];
   return v;
 }
@@ -369,29 +372,31 @@ __device__ inline void lu_solve_regs(const double (&a)[NSP][NSP], unsigned long 
 
 // Per-cell values a wave keeps in LDS, element-major ([word][64 lanes]): scale, inv_scale,
 // mdensity, inv_mdensity, the two optical-depth factors, the stage derivatives F_j of the
-// first nstage - 1 stages (F_j[s] of lane l at (j*10+s)*64 + l; for tables of more than
-// FST_LDS_MAX stages they live in a global buffer instead, [(j*10+s)][thread of the grid],
-// so that four waves still share a block) and the (sub)step start y_n.
+// first nstage - 1 stages (F_j[s] of lane l at (j*10+s)*64 + l; for the wide tables, with
+// an explicit first stage or more than FST_LDS_MAX stages, they live in a global buffer
+// instead, [(j*10+s)][thread of the grid], so that four waves still share a block) and the
+// (sub)step start y_n.
 constexpr int CELL_WORDS = 2 * NSP + 4;
 constexpr int FST_LDS_MAX = 4;
-__host__ __device__ inline int dirk_fst_lds_words(int nstage) {
-  return nstage <= FST_LDS_MAX ? (nstage - 1) * NSP : 0;
+__host__ __device__ inline bool dirk_wide(int nstage, bool esdirk) {
+  return esdirk || nstage > FST_LDS_MAX;
 }
-__host__ __device__ inline int dirk_wave_words(int nstage) {
-  return CELL_WORDS + dirk_fst_lds_words(nstage) + NSP;
+__host__ __device__ inline int dirk_wave_words(int nstage, bool wide) {
+  return CELL_WORDS + (wide ? 0 : (nstage - 1) * NSP) + NSP;
 }
 // dynamic LDS of the batched kernel for `wpb` waves per block
-inline size_t dirk_lds_bytes(int wpb, int nstage) {
-  return sizeof(double) * ((size_t)4 * TLEN + (size_t)wpb * dirk_wave_words(nstage) * 64);
+inline size_t dirk_lds_bytes(int wpb, int nstage, bool wide) {
+  return sizeof(double) * ((size_t)4 * TLEN + (size_t)wpb * dirk_wave_words(nstage, wide) * 64);
 }
 
 // Threads per block of a batched launch asked for `block` threads: blocks whose LDS (the
 // dynamic part for their waves plus the kernel's static part) would exceed the per-block
 // limit are halved (fewer waves then share one copy of the gamma tables); 0 when not even
 // one wave fits.
-int dirk_fit_block(int block, int nstage, size_t static_lds, size_t limit) {
+int dirk_fit_block(int block, int nstage, size_t static_lds, size_t limit, bool esdirk) {
+  const bool wide = dirk_wide(nstage, esdirk);
   for (; block >= 64; block /= 2)
-    if (dirk_lds_bytes(block / 64, nstage) + static_lds <= limit) return block;
+    if (dirk_lds_bytes(block / 64, nstage, wide) + static_lds <= limit) return block;
   return 0;
 }
 
@@ -430,18 +435,14 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
   __syncthreads();
   const GammaTabs gt = gamma_tabs(lds);
   const int lane = threadIdx.x & 63;
-  double *const cst = lds + 4 * TLEN + (threadIdx.x >> 6) * (dirk_wave_words(o.nstage) * 64);
-  // F_j[s] at (j*10 + s) * fstr: in LDS (fstr 64) or in the global buffer (one column per
-  // thread of the grid)
-  long fstr = 64;
-  double *Fst = cst + CELL_WORDS * 64 + lane;
-  if constexpr (WIDE) {
-    if (fglob) {
-      fstr = (long)gridDim.x * blockDim.x;
-      Fst = fglob + (long)blockIdx.x * blockDim.x + threadIdx.x;
-    }
-  }
-  double *const yst = cst + (CELL_WORDS + dirk_fst_lds_words(o.nstage)) * 64 + lane;  // y_n[s] at s*64
+  double *const cst = lds + 4 * TLEN + (threadIdx.x >> 6) * (dirk_wave_words(o.nstage, WIDE) * 64);
+  // F_j[s] at (j*10 + s) * fstr: in LDS (fstr 64), or for the wide tables in the global
+  // buffer (one column per thread of the grid)
+  const long fstr = WIDE ? (long)gridDim.x * blockDim.x : 64;
+  constexpr int QS = WIDE ? MAXSTAGE : FST_LDS_MAX;        // most stages of this instantiation
+  double *const Fst = WIDE ? fglob + (long)blockIdx.x * blockDim.x + threadIdx.x
+                           : cst + CELL_WORDS * 64 + lane;
+  double *const yst = cst + (CELL_WORDS + (WIDE ? 0 : (o.nstage - 1) * NSP)) * 64 + lane;  // y_n[s] at s*64
   // The first chunk of each wave is its global wave index; later ones come from the queue
   // (which starts after them), each fetched while the previous chunk is being solved: lane
   // 0 issues the atomic after the chunk's prologue loads and the wave reads its result at
@@ -583,8 +584,8 @@ chem_dirk_batched(long n, ChemConst cc, ChemPtrsD p, DirkDev o, double h, double
           for (int s = 0; s < NSP; s++) {
             double a = 0.0;
 #pragma unroll
-            for (int j = 0; j < MAXSTAGE - 1; j++)
-              if (j < st) a += dirk_coef(o, st, j) * Fst[(j * NSP + s) * fstr];
+            for (int j = 0; j < QS - 1; j++)
+              if (j < st) a += dirk_coef<QS>(o, st, j) * Fst[(j * NSP + s) * fstr];
             r[s] = yst[s * 64] + hs * a;                   // the (sub)step start
           }
           it = 0;
@@ -1029,7 +1030,8 @@ static int batched_plan(long n, const DirkDev &o, int grid_waves, int block, boo
   if (block != 64 && block != 128 && block != 256) return -1;
   if (o.nstage < 1 || o.nstage > MAXSTAGE) return -1;
   const bool sub = o.max_halvings > 0;
-  if (o.esdirk || o.nstage > FST_LDS_MAX)
+  const bool wide = dirk_wide(o.nstage, o.esdirk != 0);
+  if (wide)
     p.kern = forced ? (sub ? chem_dirk_batched<true, true, true>
                            : chem_dirk_batched<true, false, true>)
                     : (sub ? chem_dirk_batched<false, true, true>
@@ -1047,7 +1049,7 @@ static int batched_plan(long n, const DirkDev &o, int grid_waves, int block, boo
       hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
     return -1;
-  p.block = dirk_fit_block(block, o.nstage, fa.sharedSizeBytes, (size_t)lim);
+  p.block = dirk_fit_block(block, o.nstage, fa.sharedSizeBytes, (size_t)lim, o.esdirk != 0);
   if (p.block == 0) return -1;
   p.wpb = p.block / 64;
   const size_t chunks = (size_t)(n + 63) / 64;
@@ -1062,7 +1064,7 @@ static int batched_plan(long n, const DirkDev &o, int grid_waves, int block, boo
   p.b_key = p.sorted ? align256((size_t)n * sizeof(unsigned short)) : 0;
   p.b_tdt = p.sorted ? align256((size_t)n * sizeof(double2)) : 0;
   // stage derivatives of tables too long for LDS: one column per thread of the grid
-  p.b_fst = o.nstage > FST_LDS_MAX
+  p.b_fst = wide
                 ? align256((size_t)p.grid * p.block * (o.nstage - 1) * NSP * sizeof(double))
                 : 0;
   return 0;
@@ -1094,7 +1096,7 @@ static int batched_enqueue(const BatchedPlan &p, ChemState &s, const DirkDev &o,
   }
   if (!(p.sorted && !reuse_order))   // (else cleared with the bucket counters above)
     hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, st, (long)(2 + NSTAT), counters, 0);
-  const unsigned lds = (unsigned)dirk_lds_bytes(p.wpb, o.nstage);
+  const unsigned lds = (unsigned)dirk_lds_bytes(p.wpb, o.nstage, dirk_wide(o.nstage, o.esdirk != 0));
   const bool sub = o.max_halvings > 0;
   double *fglob =
       p.b_fst ? (double *)(work + p.b_rows + p.b_ord + p.b_cnt + p.b_key + p.b_tdt) : nullptr;
@@ -1157,7 +1159,7 @@ int dirk_batched_launch_noalloc(ChemState &s, const DirkDev &o, double h, double
                                 hipStream_t st, int grid_waves, int block, const double *force,
                                 int **counters_out) {
   if (s.n > NOALLOC_MAX_CELLS) return MVD_BAD_ARGUMENT;
-  if (o.nstage > FST_LDS_MAX) return MVD_UNSUPPORTED;     // no room for its stage buffer
+  if (dirk_wide(o.nstage, o.esdirk != 0)) return MVD_UNSUPPORTED;   // no room for its stage buffer
   BatchedPlan p;
   if (batched_plan(s.n, o, grid_waves, block, force != nullptr, p)) return -1;
   if (p.nrow > NOALLOC_MAX_ROWS || p.bytes() > NOALLOC_WORK) return MVD_BAD_ARGUMENT;

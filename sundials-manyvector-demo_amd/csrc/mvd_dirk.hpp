@@ -50,7 +50,7 @@ int dirk_batched_launch_noalloc(ChemState &s, const DirkDev &o, double h, double
 unsigned char *dirk_noalloc_scratch();
 int dirk_plain_run(ChemState &s, const DirkDev &o, double h, double *y, hipStream_t st);
 // threads per block of a batched launch (blocks split to fit the LDS limit; 0: none fits)
-int dirk_fit_block(int block, int nstage, size_t static_lds, size_t limit);
+int dirk_fit_block(int block, int nstage, size_t static_lds, size_t limit, bool esdirk = false);
 // checkpointed discrete adjoint of nsteps DIRK steps (checkpoint every K steps)
 size_t chem_adjoint_scratch_doubles(long n, int nsteps, int K, int nstage);
 int chem_adjoint_run(ChemState &s, const DirkDev &o, double h, int nsteps, int K, double *y,

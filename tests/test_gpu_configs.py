@@ -186,16 +186,24 @@ def c5_setup(mvd, torch, m):
 H0, HMAX = 0.05, 1e-4
 
 
-def test_c5_multirate_16_one_slow_step_matches_oracle(mvd, torch_gpu):
+@pytest.mark.parametrize("table", ["sdirk3", "ark437"])
+def test_c5_multirate_16_one_slow_step_matches_oracle(mvd, torch_gpu, table):
+    """One slow step at 16^3 with the input's options, fast stepper SDIRK3 or the input's
+    itable 112 (ARK4(3)7L[2]SA's ESDIRK part): device and oracle agree to 1e-8."""
     torch = torch_gpu
     m = 16
-    print(f"[C5] multirate {m}^3 one slow step vs oracle", flush=True)
+    print(f"[C5] multirate {m}^3 one slow step vs oracle, {table}", flush=True)
+    tab_o = {}
+    if table == "ark437":
+        A7, c7 = orc.ark437_table()
+        tab_o = dict(A=A7, c=c7, nstage=7)
+    make = mvd.ark437 if table == "ark437" else mvd.sdirk3
     ctx, wd, fluid, chem = c5_setup(mvd, torch, m)
     g = orc.make_grid(m, m, m, 10, (3,) * 6, gamma=5 / 3)
     dt_cfl = ctx.stability(wd)
     assert dt_cfl == orc.stability_dt(g, [a.copy() for a in fluid] + [chem.copy()], 0.3)
     h = min(dt_cfl, H0)
-    fast = mvd.sdirk3(rhs_scale=UNITS[2], max_halvings=10, **C5_NEWTON)
+    fast = make(rhs_scale=UNITS[2], max_halvings=10, **C5_NEWTON)
     assert ctx.mri_evolve(wd, h, 1, HMAX, fast, consistent=True) == 0
     oc = orc.Chem(m ** 3)
     y = chem.copy()
@@ -204,7 +212,7 @@ def test_c5_multirate_16_one_slow_step_matches_oracle(mvd, torch_gpu):
     wo = [a.copy() for a in fluid] + [y]
     orc.set_threads(orc.max_threads())
     assert orc.mri_evolve(g, oc, EU, orc.dirk_opts(rhs_scale=UNITS[2], max_halvings=10,
-                                                   **C5_NEWTON), h, 1,
+                                                   **C5_NEWTON, **tab_o), h, 1,
                           HMAX, wo, consistent=True) == 0
     for q in (0, 4):
         assert normwise_err(wd[q].cpu().numpy(), wo[q]) <= 1e-8, q
@@ -218,13 +226,15 @@ def test_c5_multirate_16_one_slow_step_matches_oracle(mvd, torch_gpu):
     ctx.close()
 
 
-def test_c5_multirate_128_two_slow_steps_properties(mvd, torch_gpu):
+@pytest.mark.parametrize("table", ["sdirk3", "ark437"])
+def test_c5_multirate_128_two_slow_steps_properties(mvd, torch_gpu, table):
     torch = torch_gpu
     m = 128
-    print(f"[C5] multirate {m}^3 two slow steps", flush=True)
+    print(f"[C5] multirate {m}^3 two slow steps, {table}", flush=True)
     ctx, wd, _, _ = c5_setup(mvd, torch, m)
     h = min(ctx.stability(wd), H0)
-    fast = mvd.sdirk3(rhs_scale=UNITS[2], max_halvings=10, **C5_NEWTON)
+    make = mvd.ark437 if table == "ark437" else mvd.sdirk3
+    fast = make(rhs_scale=UNITS[2], max_halvings=10, **C5_NEWTON)
     mass0 = float(wd[0].sum())
     assert ctx.mri_evolve(wd, h, 2, HMAX, fast, consistent=True) == 0
     for q in range(6):
