@@ -345,6 +345,11 @@ typedef struct {
 } mvd_ark_table;
 /* ARK3(2)4L[2]SA (Kennedy & Carpenter 2003), ARKStep's default third-order IMEX pair. */
 int mvd_ark_ark324(mvd_ark_table *t);
+/* ARK4(3)7L[2]SA (Kennedy & Carpenter 2019), the pair the reference's IMEX inputs select
+ * (etable 13, itable 112 in tests/primordial_blast/input_primordial_blast_imex.txt:48-49):
+ * seven stages, fourth order, its implicit part mvd_dirk_ark437's table. No counterpart call
+ * in the reference (it selects the tables through ARKStepSetTableNum). */
+int mvd_ark_ark437(mvd_ark_table *t);
 /* IMEX evolution of the reacting flow, device-resident: the IMEX driver
  * (imex_chem_hydro_main.cpp:337-476, ARKStepCreate(fexpl, fimpl), the block-diagonal
  * chemistry solver BDMPIMV :1075-1194, PostprocessStep :1002-1050) with a fixed step h:

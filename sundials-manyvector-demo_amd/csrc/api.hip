@@ -795,6 +795,47 @@ int mvd_ark_ark324(mvd_ark_table *t) {
   return MVD_SUCCESS;
 }
 
+// ARK4(3)7L[2]SA (Kennedy & Carpenter, Appl. Numer. Math. 136 (2019); ARKODE's
+// ARK437L2SA_ERK_7_3_4 / ARK437L2SA_DIRK_7_3_4, the reference's IMEX inputs' etable 13 and
+// itable 112), restated from the published method: the implicit part is mvd_dirk_ark437's
+// table, the explicit part the published rationals. Its rows sum to the shared nodes and
+// the fourth-order conditions of the pair, coupling ones included, hold to round-off.
+int mvd_ark_ark437(mvd_ark_table *t) {
+  if (!t) return MVD_BAD_ARGUMENT;
+  mvd_dirk_opts d;
+  if (mvd_dirk_ark437(&d) != MVD_SUCCESS) return MVD_BAD_ARGUMENT;
+  std::memset(t, 0, sizeof(*t));
+  constexpr int S = mvd::MAXSTAGE, T = 8;
+  t->stages = 7;
+  for (int i = 0; i < 7; i++)
+    for (int j = 0; j <= i; j++) t->Ai[i * T + j] = d.A[i * S + j];
+  double *E = t->Ae;
+  E[1 * T + 0] = 247.0 / 1000.0;
+  E[2 * T + 0] = 247.0 / 4000.0;
+  E[2 * T + 1] = 2694949928731.0 / 7487940209513.0;
+  E[3 * T + 0] = 464650059369.0 / 8764239774964.0;
+  E[3 * T + 1] = 878889893998.0 / 2444806327765.0;
+  E[3 * T + 2] = -952945855348.0 / 12294611323341.0;
+  E[4 * T + 0] = 476636172619.0 / 8159180917465.0;
+  E[4 * T + 1] = -1271469283451.0 / 7793814740893.0;
+  E[4 * T + 2] = -859560642026.0 / 4356155882851.0;
+  E[4 * T + 3] = 1723805262919.0 / 4571918432560.0;
+  E[5 * T + 0] = 6338158500785.0 / 11769362343261.0;
+  E[5 * T + 1] = -4970555480458.0 / 10924838743837.0;
+  E[5 * T + 2] = 3326578051521.0 / 2647936831840.0;
+  E[5 * T + 3] = -880713585975.0 / 1841400956686.0;
+  E[5 * T + 4] = -1428733748635.0 / 8843423958496.0;
+  E[6 * T + 0] = 760814592956.0 / 3276306540349.0;
+  E[6 * T + 1] = 760814592956.0 / 3276306540349.0;
+  E[6 * T + 2] = -47223648122716.0 / 6934462133451.0;
+  E[6 * T + 3] = 71187472546993.0 / 9669769126921.0;
+  E[6 * T + 4] = -13330509492149.0 / 9695768672337.0;
+  E[6 * T + 5] = 11565764226357.0 / 8513123442827.0;
+  for (int j = 0; j < 7; j++) t->b[j] = t->Ai[6 * T + j];
+  for (int i = 0; i < 7; i++) t->c[i] = d.c[i];
+  return MVD_SUCCESS;
+}
+
 int mvd_imex_evolve(mvd_context *ctx, double, double h, int nsteps, const mvd_ark_table *tab,
                     const mvd_dirk_opts *newton, int flags, double *const *w) {
   if (flags & ~MVD_IMEX_CONSISTENT_FORCING) return MVD_BAD_ARGUMENT;

@@ -137,6 +137,7 @@ def lib():
         "mvd_mri_evolve": ([vp, C.c_double, C.c_double, C.c_int, C.c_double,
                             C.POINTER(DirkOpts), C.c_int, C.POINTER(_P)], C.c_int),
         "mvd_ark_ark324": ([C.POINTER(ArkTable)], C.c_int),
+        "mvd_ark_ark437": ([C.POINTER(ArkTable)], C.c_int),
         "mvd_imex_evolve": ([vp, C.c_double, C.c_double, C.c_int, C.POINTER(ArkTable),
                              C.POINTER(DirkOpts), C.c_int, C.POINTER(_P)], C.c_int),
         "mvd_chem_dirk_adjoint": ([vp, C.c_double, C.c_double, C.c_int, C.c_int,
@@ -340,6 +341,13 @@ def bs3():
 def ark324():
     t = ArkTable()
     _check("mvd_ark_ark324", lib().mvd_ark_ark324(C.byref(t)))
+    return t
+
+
+def ark437_pair():
+    """ARK4(3)7L[2]SA, the IMEX pair of the reference's IMEX inputs (etable 13, itable 112)."""
+    t = ArkTable()
+    _check("mvd_ark_ark437", lib().mvd_ark_ark437(C.byref(t)))
     return t
 
 

@@ -113,6 +113,35 @@ def ark324_table():
     return t
 
 
+def ark437_pair():
+    """ARK4(3)7L[2]SA as an Ark table, independently of the library: the implicit part from
+    ark437_table(), the explicit part from the published rationals (exact fractions)."""
+    from fractions import Fraction as F
+    Ai, c = ark437_table()
+    E = [[F(0)] * 7 for _ in range(7)]
+    E[1][0] = F(247, 1000)
+    E[2][0], E[2][1] = F(247, 4000), F(2694949928731, 7487940209513)
+    E[3][0], E[3][1], E[3][2] = (F(464650059369, 8764239774964), F(878889893998, 2444806327765),
+                                 F(-952945855348, 12294611323341))
+    E[4][:4] = [F(476636172619, 8159180917465), F(-1271469283451, 7793814740893),
+                F(-859560642026, 4356155882851), F(1723805262919, 4571918432560)]
+    E[5][:5] = [F(6338158500785, 11769362343261), F(-4970555480458, 10924838743837),
+                F(3326578051521, 2647936831840), F(-880713585975, 1841400956686),
+                F(-1428733748635, 8843423958496)]
+    E[6][:6] = [F(760814592956, 3276306540349), F(760814592956, 3276306540349),
+                F(-47223648122716, 6934462133451), F(71187472546993, 9669769126921),
+                F(-13330509492149, 9695768672337), F(11565764226357, 8513123442827)]
+    t = Ark()
+    t.stages = 7
+    for i in range(7):
+        for j in range(7):
+            t.Ae[8 * i + j] = float(E[i][j])
+            t.Ai[8 * i + j] = Ai[i, j]
+        t.b[i] = Ai[6, i]
+        t.c[i] = Ai[i].sum()
+    return t
+
+
 def imex_evolve(g, chem, energy_units, tab, newton, h, nsteps, w, consistent=False):
     """Oracle IMEX ARK evolution in place on w (6 subvectors, chemistry scaled)."""
     return lib().orc_imex_evolve(C.byref(g), chem.h, energy_units, C.byref(tab), C.byref(newton),

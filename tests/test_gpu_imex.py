@@ -30,13 +30,20 @@ def test_ark324_table_matches_oracle(mvd, torch_gpu):
         assert list(getattr(d, f)) == list(getattr(o, f)), f
 
 
-@pytest.mark.parametrize("dims,consistent,at_rest", [((10, 8, 8), False, False),
-                                                     ((16, 16, 16), True, False),
-                                                     ((8, 8, 8), False, True)])
-def test_imex_matches_oracle(mvd, torch_gpu, dims, consistent, at_rest):
+@pytest.mark.parametrize("dims,consistent,at_rest,pair", [((10, 8, 8), False, False, "ark324"),
+                                                          ((16, 16, 16), True, False, "ark324"),
+                                                          ((8, 8, 8), False, True, "ark324"),
+                                                          ((10, 8, 8), False, False, "ark437"),
+                                                          ((16, 16, 16), True, False, "ark437")])
+def test_imex_matches_oracle(mvd, torch_gpu, dims, consistent, at_rest, pair):
+    """ARK3(2)4L[2]SA and the reference IMEX inputs' ARK4(3)7L[2]SA (etable 13 / itable 112,
+    explicit first stage) against the oracle's driver."""
     torch = torch_gpu
     n = dims[0] * dims[1] * dims[2]
-    print(f"[imex] dims={dims} consistent={consistent} at_rest={at_rest}", flush=True)
+    print(f"[imex] dims={dims} consistent={consistent} at_rest={at_rest} pair={pair}",
+          flush=True)
+    tab_o = orc.ark437_pair() if pair == "ark437" else orc.ark324_table()
+    tab_d = mvd.ark437_pair() if pair == "ark437" else mvd.ark324()
     fluid, chem = problems.primordial_blast_mri(*dims, bump=3.0, width=0.25)
     if at_rest:
         c0 = n // 2 + dims[0] // 2
@@ -55,9 +62,9 @@ def test_imex_matches_oracle(mvd, torch_gpu, dims, consistent, at_rest):
     wo = [a.copy() for a in fluid] + [y]
     g = orc.make_grid(*dims, 10, bc, gamma=5 / 3)
     h, nsteps = 1e-4, 3
-    assert orc.imex_evolve(g, oc, EU, orc.ark324_table(), orc.dirk_opts(rhs_scale=UNITS[2]),
+    assert orc.imex_evolve(g, oc, EU, tab_o, orc.dirk_opts(rhs_scale=UNITS[2]),
                            h, nsteps, wo, consistent) == 0
-    assert ctx.imex_evolve(wd, h, nsteps, mvd.sdirk3(rhs_scale=UNITS[2]),
+    assert ctx.imex_evolve(wd, h, nsteps, mvd.sdirk3(rhs_scale=UNITS[2]), table=tab_d,
                            consistent=consistent) == 0
     for q in range(5):
         if np.max(np.abs(wo[q])) > 0:

@@ -1,6 +1,8 @@
 """IMEX additive Runge-Kutta evolution on the CPU oracle (the imex_chem_hydro driver),
 pinned by: the third-order and coupling conditions of the ARK3(2)4L[2]SA pair (Kennedy &
-Carpenter 2003) with its ESDIRK structure; third-order convergence of the explicit half
+Carpenter 2003) with its ESDIRK structure, and the fourth-order ones of ARK4(3)7L[2]SA
+(Kennedy & Carpenter 2019, the reference's IMEX inputs' pair) with fourth-order convergence
+of both of its halves; third-order convergence of the explicit half
 alone (chemistry switched off, rhs_scale = 0: the implicit stages reduce to z = R) on a
 smooth flow, and of the implicit half alone (a uniform gas at rest: fe() and the advection
 terms vanish, so the step is the ESDIRK on the chemistry) against the same driver at a
@@ -54,21 +56,22 @@ def setup_state(dims, seed=None, at_rest=False):
     return ch, [a.copy() for a in fluid] + [y]
 
 
-def run(dims, h, nsteps, rhs_scale, at_rest=False, bc=(0,) * 6):
+def run(dims, h, nsteps, rhs_scale, at_rest=False, bc=(0,) * 6, tab=None):
     ch, w = setup_state(dims, at_rest=at_rest)
     g = orc.make_grid(*dims, 10, bc, gamma=5 / 3)
     newton = orc.dirk_opts(rhs_scale=rhs_scale, rtol=1e-10, atol=1e-14, newton_tol=1e-3,
                            max_iters=20)
-    rc = orc.imex_evolve(g, ch, EU, orc.ark324_table(), newton, h, nsteps, w)
+    rc = orc.imex_evolve(g, ch, EU, tab if tab is not None else orc.ark324_table(), newton, h,
+                         nsteps, w)
     assert rc == 0
     return w
 
 
-def errors(dims, T, steps, rhs_scale, at_rest):
-    ref = run(dims, T / (16 * steps[-1]), 16 * steps[-1], rhs_scale, at_rest)
+def errors(dims, T, steps, rhs_scale, at_rest, tab=None):
+    ref = run(dims, T / (16 * steps[-1]), 16 * steps[-1], rhs_scale, at_rest, tab=tab)
     errs = []
     for m in steps:
-        w = run(dims, T / m, m, rhs_scale, at_rest)
+        w = run(dims, T / m, m, rhs_scale, at_rest, tab=tab)
         errs.append(max(np.max(np.abs(w[q] - ref[q])) / np.max(np.abs(ref[q]))
                         for q in range(6) if np.max(np.abs(ref[q])) > 0))
     return np.array(errs)
@@ -93,6 +96,56 @@ def test_implicit_half_is_third_order():
     errs = errors((4, 4, 4), 2e-5, [2, 4, 8], UNITS[2], True)
     rates = np.log2(errs[:-1] / errs[1:])
     assert np.all(rates > 2.6) and np.all(rates < 3.6), (errs, rates)
+
+
+def _pair_conditions(Ae, Ai, b, c):
+    """Residuals of the fourth-order conditions of an additive pair with shared b and c:
+    the quadrature ones and, for every choice of the two tables in each position, the
+    tree conditions b.Xc, b.(c*Xc), b.Xc^2, b.XYc."""
+    r = [b.sum() - 1, b @ c - 1 / 2, b @ c ** 2 - 1 / 3, b @ c ** 3 - 1 / 4]
+    for X in (Ae, Ai):
+        r += [b @ (X @ c) - 1 / 6, b @ (c * (X @ c)) - 1 / 8, b @ (X @ c ** 2) - 1 / 12]
+        for Y in (Ae, Ai):
+            r.append(b @ (X @ (Y @ c)) - 1 / 24)
+    return np.array(r)
+
+
+def test_ark437_pair_table(mvd):
+    """ARK4(3)7L[2]SA (the reference's IMEX inputs' etable 13 / itable 112): the explicit
+    part strictly lower triangular, the implicit part the ESDIRK table, both rows summing to
+    the shared nodes, all fourth-order conditions of the pair (coupling ones included) to
+    round-off, and the library's table the same doubles; a wrong digit in one explicit entry
+    breaks the conditions."""
+    t = orc.ark437_pair()
+    Ae, Ai, b, c = t.tables()
+    assert np.all(np.triu(Ae) == 0) and np.all(np.triu(Ai, 1) == 0)
+    assert np.allclose(Ae.sum(axis=1), c, rtol=0, atol=5e-16)
+    assert np.allclose(Ai.sum(axis=1), c, rtol=0, atol=5e-16)
+    assert np.array_equal(b, Ai[6])
+    assert np.max(np.abs(_pair_conditions(Ae, Ai, b, c))) < 1e-15
+    bad = Ae.copy()
+    bad[5, 2] *= 1 + 1e-9
+    bad[5, 0] -= bad[5].sum() - Ae[5].sum()     # (keeping the node)
+    assert np.max(np.abs(_pair_conditions(bad, Ai, b, c))) > 1e-12
+    d = mvd.ark437_pair()
+    for f in ("Ae", "Ai", "b"):
+        assert np.array_equal(np.array(getattr(d, f)[:]), np.array(getattr(t, f)[:])), f
+    assert d.stages == 7
+
+
+def test_ark437_explicit_half_is_fourth_order():
+    """As the ARK3 test, with the seven-stage pair: errors fall by ~16 per halving."""
+    errs = errors((8, 6, 6), 0.6, [4, 8, 16], 0.0, False, tab=orc.ark437_pair())
+    rates = np.log2(errs[:-1] / errs[1:])
+    assert np.all(rates > 3.6) and np.all(rates < 4.8), (errs, rates)
+
+
+def test_ark437_implicit_half_is_fourth_order():
+    """A uniform gas at rest: the step is the pair's ESDIRK part on the chemistry (its
+    explicit first stage included)."""
+    errs = errors((4, 4, 4), 2e-5, [2, 4, 8], UNITS[2], True, tab=orc.ark437_pair())
+    rates = np.log2(errs[:-1] / errs[1:])
+    assert np.all(rates > 3.6) and np.all(rates < 4.6), (errs, rates)
 
 
 def test_imex_conserves_mass_with_periodic_boundaries():
