@@ -43,6 +43,8 @@ PEAK_FP64_TFLOPS = 78.6             # MI355X FP64 (vector and matrix) peak
 # ARKODE's default frequency (msbp 20; the input leaves it)
 CHEM_NEWTON = {"newton_tol": 1e-3, "max_iters": 5}
 MRI_NEWTON = {"newton_tol": 1e-2, "max_iters": 10, "max_steps_between_setups": 20}
+# input_primordial_blast_imex.txt:67-68 (maxniters, nlconvcoef)
+IMEX_NEWTON = {"newton_tol": 1e-2, "max_iters": 10}
 PEAK_HBM_GBS = 8000.0               # MI355X HBM3E
 
 
@@ -456,6 +458,78 @@ def quiet_gc():
     gc.disable()
 
 
+def bench_imex(api, problems, torch, n, nsteps, cpu=True, cpu_n=10, cpu_steps=500):
+    """IMEX reacting flow (the imex_chem_hydro driver): the primordial blast of the multirate
+    configuration on n^3 cells, reflecting boundaries, the units, pair and controls of
+    input_primordial_blast_imex.txt (ARK4(3)7L[2]SA: etable 13 / itable 112; Newton 0.01 /
+    10 iterations; rtol 1e-5, atol 1e-9), fixed steps of the input's hmax = 1e-4 where the
+    reference adapts below it, chemistry forcing in the scaled variables. The oracle runs the
+    same options on the input's own 10^3 grid for cpu_steps steps (host baseline), and the
+    device is compared with it there."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    units = (3.0e70, 3.0857e30, 1.0e11)
+    eu = units[0] / units[1] / units[2] / units[2]
+    h = 1e-4
+
+    def setup(m):
+        fluid, chem = problems.primordial_blast_mri(m)
+        ctx = api.Context(api.make_params(m, m, m, 10, (3,) * 6, gamma=5 / 3, units=units))
+        ctx.chem_setup(api.load_tables(), m ** 3)
+        wd = [torch.from_numpy(a.copy()).to("cuda") for a in fluid] + \
+             [torch.from_numpy(chem.copy()).to("cuda")]
+        ctx.chem_prepare(wd[5])
+        return ctx, wd, fluid, chem
+
+    newton = api.sdirk3(rhs_scale=units[2], **IMEX_NEWTON)
+    tab = api.ark437_pair()
+    ctx, wd, _, _ = setup(n)
+    ctx.imex_evolve(wd, h, 1, newton, table=tab, consistent=True)     # allocations
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    failed = ctx.imex_evolve(wd, h, nsteps, newton, table=tab, consistent=True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    ok = all(bool(torch.isfinite(a).all()) for a in wd)
+    ctx.close()
+    out = {"workload": f"IMEX primordial blast {n}^3, 10 species, reflecting, {nsteps} steps of "
+                       f"{h:g} with ARK4(3)7L[2]SA (etable 13 / itable 112), Newton tol "
+                       f"{IMEX_NEWTON['newton_tol']:g} / {IMEX_NEWTON['max_iters']} iterations, "
+                       f"consistent forcing units (after one untimed step)",
+           "failed_cell_solves": failed, "finite": ok, "gpu_ms": ms,
+           "cell_steps_per_s": n ** 3 * nsteps / (ms * 1e-3)}
+    if cpu:
+        m = cpu_n
+        ctx, wd, fluid, chem = setup(m)
+        ctx.imex_evolve(wd, h, cpu_steps, newton, table=tab, consistent=True)
+        oc = orc.Chem(m ** 3)
+        y = chem.copy()
+        oc.prepare(y)
+        oc.extra()
+        wo = [a.copy() for a in fluid] + [y]
+        g = orc.make_grid(m, m, m, 10, (3,) * 6, gamma=5 / 3)
+        orc.set_threads(1)
+        t0 = time.perf_counter()
+        orc.imex_evolve(g, oc, eu, orc.ark437_pair(), orc.dirk_opts(rhs_scale=units[2],
+                                                                    **IMEX_NEWTON),
+                        h, cpu_steps, wo, consistent=True)
+        dt = time.perf_counter() - t0
+        diffs = []
+        for q in range(6):
+            den = float(np.abs(wo[q]).max())
+            num = float(np.abs(wd[q].cpu().numpy() - wo[q]).max())
+            diffs.append(num / den if den > 0 else num)
+        ctx.close()
+        out["cpu_baseline"] = {"value": m ** 3 * cpu_steps / dt, "unit": "cell*steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"oracle IMEX run, {m}^3 cells (the input's grid), "
+                                         f"{cpu_steps} steps, same options, {dt:.1f} s"}
+        out["sample_max_rel_diff_vs_oracle"] = max(diffs)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -476,10 +550,12 @@ def main():
     ap.add_argument("--no-mri", action="store_true")
     ap.add_argument("--mri-n", type=int, default=128)
     ap.add_argument("--mri-steps", type=int, default=20)
+    ap.add_argument("--imex-steps", type=int, default=20)
+    ap.add_argument("--no-imex", action="store_true")
     ap.add_argument("--mri-table", choices=["ark437", "sdirk3"], default="ark437",
                     help="fast DIRK table of the multirate run (default: the reference "
                          "input's itable 112, ARK4(3)7L[2]SA's ESDIRK part)")
-    ap.add_argument("--only", choices=["vjp", "sod", "mri", "chem", "fe10"], default=None,
+    ap.add_argument("--only", choices=["vjp", "sod", "mri", "chem", "fe10", "imex"], default=None,
                     help="run one extra workload alone and print its JSON (profiling runs)")
     ap.add_argument("--vjp-n", type=int, default=128)
     ap.add_argument("--vjp-reps", type=int, default=10)
@@ -504,6 +580,9 @@ def main():
         elif args.only == "mri":
             print(json.dumps(bench_mri(api, problems, torch, args.mri_n, args.mri_steps,
                                        cpu=not args.no_cpu, table=args.mri_table)))
+        elif args.only == "imex":
+            print(json.dumps(bench_imex(api, problems, torch, args.mri_n, args.imex_steps,
+                                        cpu=not args.no_cpu)))
         elif args.only == "chem":
             print(json.dumps(bench_chem(api, problems, torch, args.chem_n, 4321,
                                         args.chem_steps)))
@@ -607,6 +686,10 @@ def main():
         quiet_gc()
         extra["mri"] = bench_mri(api, problems, torch, args.mri_n, args.mri_steps,
                                  cpu=not args.no_cpu, table=args.mri_table)
+    if not args.no_imex:
+        quiet_gc()
+        extra["imex"] = bench_imex(api, problems, torch, args.mri_n, args.imex_steps,
+                                   cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
