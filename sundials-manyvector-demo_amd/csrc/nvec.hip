@@ -74,16 +74,22 @@ int max_wave_speed_launch(const Grid &g, const Fields &f, FeScratch &s, hipStrea
   return 0;
 }
 
-// fslow, first half: chem *= scale; et <- chem[nchem-1]/EnergyUnits + 0.5/rho |m|^2
+// fslow, first half: chem *= scale; et <- chem[nchem-1]/EnergyUnits + 0.5/rho |m|^2. One
+// thread per chemistry value (both arrays read and written coalesced); the thread of each
+// cell's last value (the gas energy) also sets that cell's et.
 __global__ void __launch_bounds__(256)
-fslow_prepare(long n, int nchem, double *rho, double *mx, double *my, double *mz, double *et,
-              double *chem, const double *scale, double eunit_inv) {
+fslow_prepare(long n, int nchem, const double *rho, const double *mx, const double *my,
+              const double *mz, double *et, double *chem, const double *scale,
+              double eunit_inv) {
 #pragma clang fp contract(off)
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  for (int v = 0; v < nchem; v++) chem[(long)nchem * c + v] *= scale[(long)nchem * c + v];
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nchem) return;
+  const double v = chem[e] * scale[e];
+  chem[e] = v;
+  const long c = e / nchem;
+  if (e - c * nchem != nchem - 1) return;
   const double r = rho[c], a = mx[c], b = my[c], d = mz[c];
-  et[c] = chem[(long)nchem * c + nchem - 1] * eunit_inv + 0.5 / r * (a * a + b * b + d * d);
+  et[c] = v * eunit_inv + 0.5 / r * (a * a + b * b + d * d);
 }
 
 // fslow, second half: chemdot[nchem-1] <- etdot; etdot <- 0
@@ -106,27 +112,31 @@ chem_scale(long n, double *chem, const double *scale, int apply) {
 // chemistry rates of one slow / explicit RHS into the scaled variables the chemistry is
 // integrated in: species divided by their scale, the gas-energy slot (a code-unit energy
 // rate) times EnergyUnits divided by the ge scale
+// (one thread per value, coalesced)
 __global__ void __launch_bounds__(256)
-chem_forcing_to_scaled(long n, double *Fc, const double *scale, double eunits) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  for (int s = 0; s < 9; s++) Fc[10 * i + s] /= scale[10 * i + s];
-  Fc[10 * i + 9] = Fc[10 * i + 9] * eunits / scale[10 * i + 9];
+chem_forcing_to_scaled(long m, double *Fc, const double *scale, double eunits) {
+#pragma clang fp contract(off)
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  if (e % 10 != 9) Fc[e] /= scale[e];
+  else Fc[e] = Fc[e] * eunits / scale[e];
 }
 
 int chem_forcing_to_scaled_launch(long n, double *Fc, const double *scale, double eunits,
                                   hipStream_t st) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(chem_forcing_to_scaled, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     st, n, Fc, scale, eunits);
+  hipLaunchKernelGGL(chem_forcing_to_scaled, dim3((unsigned)((10 * n + 255) / 256)), dim3(256), 0,
+                     st, 10 * n, Fc, scale, eunits);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 int fslow_prepare_launch(const Grid &g, double *const *w, const double *scale,
                          double eunit_inv, hipStream_t st) {
   const long n = (long)g.nx * g.ny * g.nz;
-  hipLaunchKernelGGL(fslow_prepare, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
-                     g.nchem, w[0], w[1], w[2], w[3], w[4], w[5], scale, eunit_inv);
+  if (g.nchem < 1) return 0;
+  hipLaunchKernelGGL(fslow_prepare, dim3((unsigned)((n * g.nchem + 255) / 256)), dim3(256), 0,
+                     st, n, g.nchem, (const double *)w[0], (const double *)w[1],
+                     (const double *)w[2], (const double *)w[3], w[4], w[5], scale, eunit_inv);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
