@@ -93,6 +93,9 @@ int imex_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, I
   fe.timer.on = false;
   int rc = 0;
   for (int step = 0; step < nsteps && rc == 0; step++) {
+    // the cells' temperature order is computed for the step's first implicit stage and
+    // reused by its others (an ordering only: each cell's arithmetic does not depend on it)
+    bool ordered = false;
     for (int i = 0; i < S && rc == 0; i++) {
       double *Fei = Fe + i * len, *Fii = Fi + i * nc;
       // R_i: fluid, then chemistry (explicit and implicit terms interleaved)
@@ -123,10 +126,11 @@ int imex_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, I
         if (linear_combination_launch(1, cs, Xs, Fii, nc, st)) { rc = -1; break; }
       } else {
         const double hg = h * aii;
-        if (dirk_batched_launch(ch, be, hg, Z + 5 * n, st, waves, block, nullptr)) {
+        if (dirk_batched_launch(ch, be, hg, Z + 5 * n, st, waves, block, nullptr, ordered)) {
           rc = -1;
           break;
         }
+        ordered = true;
         hipLaunchKernelGGL(imex_add_int, dim3(1), dim3(1), 0, st,
                            (const int *)(ch.counters + 1 + STAT_FAILED), m.nfail);
         hipLaunchKernelGGL(stage_deriv, dim3(nblk(nc)), dim3(256), 0, st, nc,
