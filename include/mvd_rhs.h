@@ -400,8 +400,11 @@ int mvd_chem_dirk_set_shape(mvd_context *ctx, int block, int waves);
  * kernel's static LDS is static_lds bytes and the device allows lds_limit bytes per block
  * (the launch asks the runtime for both). The dynamic LDS is 8 * (4*1024 + waves * (24 +
  * 10*nstage) * 64) bytes: the gamma tables, then per wave its cells' constants, the stage
- * derivatives of the earlier stages and the step start. Halves the block until it fits;
- * 0 when not even one wave fits. */
+ * derivatives of the earlier stages and the step start; for tables of more than four stages
+ * (and explicit-first-stage tables, which this call does not see) the stage derivatives go
+ * to a device buffer of 80 * (nstage - 1) bytes per thread of the persistent grid instead,
+ * 8 * (4*1024 + waves * 34 * 64) bytes of LDS. Halves the block until it fits; 0 when not
+ * even one wave fits. */
 int mvd_chem_dirk_fit_block(int block, int nstage, long static_lds, long lds_limit);
 int mvd_chem_dirk_get_shape(mvd_context *ctx, int *block, int *waves);
 
