@@ -181,18 +181,21 @@ def test_dirk_rejects_bad_options(mvd, torch_gpu):
     assert e.value.code == mvd.MVD_BAD_ARGUMENT
 
 
-def test_dirk_tune_keeps_results(mvd, torch_gpu):
+@pytest.mark.parametrize("table", ["sdirk3", "ark437"])
+def test_dirk_tune_keeps_results(mvd, torch_gpu, table):
     """The autotuner times launch shapes on a scratch copy and restores the temperatures:
-    a step after tuning equals a step without it."""
+    a step after tuning equals a step without it (with the seven-stage table too, whose
+    stage buffer is sized per shape)."""
     torch = torch_gpu
     n = 20000
-    print(f"[dirk tune] n={n}", flush=True)
+    print(f"[dirk tune] n={n} table={table}", flush=True)
     _, _, ctx, yd = setup(mvd, torch, n, 8)
     _, _, ctx2, yd2 = setup(mvd, torch, n, 8)
-    ms = ctx.dirk_tune(yd, 1e3)
+    o = mvd.ark437() if table == "ark437" else mvd.sdirk3()
+    ms = ctx.dirk_tune(yd, 1e3, o)
     assert ms > 0
-    f1, s1 = ctx.dirk_step(yd, 1e3)
-    f2, s2 = ctx2.dirk_step(yd2, 1e3)
+    f1, s1 = ctx.dirk_step(yd, 1e3, o)
+    f2, s2 = ctx2.dirk_step(yd2, 1e3, o)
     assert f1 == f2 == 0
     assert torch.equal(yd, yd2)
     assert s1["newton_iters"] == s2["newton_iters"]
