@@ -112,13 +112,18 @@ int imex_evolve(const Grid &g, FeScratch &fe, ChemState &ch, double eunit_inv, I
         cc[2 + 2 * j] = h * tab.Ai[i * 8 + j];
         Xc[2 + 2 * j] = Fi + j * nc;
       }
-      if (linear_combination_launch(1 + i, cf, Xf, R, 5 * n, st) ||
-          linear_combination_launch(1 + 2 * i, cc, Xc, R + 5 * n, nc, st) ||
-          hipMemcpyAsync(Z, R, sizeof(double) * len, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+      // the fluid part of R is the stage value itself: it goes straight into Z; the
+      // chemistry part is kept in R (the implicit stage derivative needs it) and copied into
+      // Z as the Newton predictor, or straight into Z for an explicit stage
+      const double aii = tab.Ai[i * 8 + i];
+      double *Rc = aii == 0.0 ? Z + 5 * n : R + 5 * n;
+      if (linear_combination_launch(1 + i, cf, Xf, Z, 5 * n, st) ||
+          linear_combination_launch(1 + 2 * i, cc, Xc, Rc, nc, st) ||
+          (aii != 0.0 && hipMemcpyAsync(Z + 5 * n, Rc, sizeof(double) * nc,
+                                        hipMemcpyDeviceToDevice, st) != hipSuccess)) {
         rc = -1;
         break;
       }
-      const double aii = tab.Ai[i * 8 + i];
       if (aii == 0.0) {
         if (chem_rhs(ch, Z + 5 * n, Fii, st, false)) { rc = -1; break; }
         const double cs[1] = {newton.rhs_scale};
